@@ -1,0 +1,270 @@
+// f2m_common.hip -- context, errors, staging and the device scan used by the
+// graph builders.
+#include <cstdio>
+#include <cstring>
+
+#include "f2m_internal.h"
+
+namespace f2m {
+
+f2m_status fail(f2m_ctx* c, f2m_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+f2m_status hip_fail(f2m_ctx* c, hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what,
+           file, line);
+  return fail(c, F2M_ERR_HIP, buf);
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof a);
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+const void* stage_in(f2m_ctx* c, const void* p, size_t bytes, int slot) {
+  if (!p) return nullptr;
+  if (is_device_ptr(p)) return p;
+  hipError_t e = c->stage[slot].reserve(bytes);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->stage[slot].p, p, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "stage_in", __FILE__, __LINE__);
+    return nullptr;
+  }
+  return c->stage[slot].p;
+}
+
+hipError_t copy_out(f2m_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!dst || !bytes) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c->stream);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan: tiles of 2048 (256 threads x 8), tile sums scanned
+// recursively, then offsets added.  Sums are carried in int64.
+// ---------------------------------------------------------------------------
+constexpr int SCAN_T = 256, SCAN_V = 8, SCAN_TILE = SCAN_T * SCAN_V;
+
+template <class Tin>
+__global__ void __launch_bounds__(SCAN_T) k_scan_tiles(const Tin* __restrict__ in, int64_t* __restrict__ out,
+                                                       int64_t* __restrict__ tile_sum, int64_t n) {
+  __shared__ int64_t wsum[SCAN_T / 64];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_V;
+  int64_t v[SCAN_V];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_V; ++i) {
+    int64_t idx = base + i;
+    v[i] = idx < n ? (int64_t)in[idx] : 0;
+    s += v[i];
+  }
+  // inclusive wave scan of per-thread sums
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t inc = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int64_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int64_t wpre = 0;
+  for (int k = 0; k < wid; ++k) wpre += wsum[k];
+  int64_t excl = wpre + inc - s;
+#pragma unroll
+  for (int i = 0; i < SCAN_V; ++i) {
+    int64_t idx = base + i;
+    if (idx < n) out[idx] = excl;
+    excl += v[i];
+  }
+  if (threadIdx.x == SCAN_T - 1) tile_sum[blockIdx.x] = wpre + inc;
+}
+
+template <class Tout>
+__global__ void k_scan_finish(const int64_t* __restrict__ loc, const int64_t* __restrict__ tile_off,
+                              Tout* __restrict__ out, int64_t n, const int64_t* __restrict__ total) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (Tout)(loc[i] + tile_off[i / SCAN_TILE]);
+  if (i == 0) out[n] = (Tout)(*total);
+}
+
+// Scans int64 values in place-ish: out[0..n] exclusive, out[n] = total.
+static hipError_t scan_i64_rec(f2m_ctx* c, const int64_t* in, int64_t* out, int64_t n,
+                               int64_t* scratch, size_t scratch_elems) {
+  int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (tiles < 1) tiles = 1;
+  int64_t* loc = scratch;
+  int64_t* tsum = loc + n;
+  int64_t* toff = tsum + tiles;
+  int64_t* rest = toff + tiles + 1;
+  size_t used = (size_t)(n + 2 * tiles + 1);
+  if (used > scratch_elems) return hipErrorOutOfMemory;
+  hipLaunchKernelGGL(k_scan_tiles<int64_t>, dim3((unsigned)tiles), dim3(SCAN_T), 0, c->stream, in, loc, tsum, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_scan_tiles<int64_t>, dim3(1), dim3(SCAN_T), 0, c->stream, tsum, toff, toff + 1, (int64_t)1);
+  } else {
+    e = scan_i64_rec(c, tsum, toff, tiles, rest, scratch_elems - used);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_finish<int64_t>, dim3(grid_for(n + 1, 256)), dim3(256), 0, c->stream, loc, toff,
+                     out, n, toff + tiles);
+  return hipGetLastError();
+}
+
+template <class Tout>
+static hipError_t scan_i32_impl(f2m_ctx* c, const int32_t* in, Tout* out, int64_t n, Tout* total_host) {
+  int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (tiles < 1) tiles = 1;
+  size_t need = (size_t)(n + 4 * tiles + 64) * 2 + 64;
+  hipError_t e = c->k_scan.reserve(need * sizeof(int64_t));
+  if (e != hipSuccess) return e;
+  int64_t* loc = c->k_scan.as<int64_t>();
+  int64_t* tsum = loc + n;
+  int64_t* toff = tsum + tiles;
+  int64_t* rest = toff + tiles + 1;
+  hipLaunchKernelGGL(k_scan_tiles<int32_t>, dim3((unsigned)tiles), dim3(SCAN_T), 0, c->stream, in, loc, tsum, n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_scan_tiles<int64_t>, dim3(1), dim3(SCAN_T), 0, c->stream, tsum, toff, toff + 1, (int64_t)1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  } else {
+    e = scan_i64_rec(c, tsum, toff, tiles, rest, need - (size_t)(n + 2 * tiles + 1));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_finish<Tout>, dim3(grid_for(n + 1, 256)), dim3(256), 0, c->stream, loc, toff, out,
+                     n, toff + tiles);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (total_host) {
+    e = hipMemcpyAsync(total_host, out + n, sizeof(Tout), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(c->stream);
+  }
+  return e;
+}
+
+hipError_t scan_i32_to_i32(f2m_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* total_host) {
+  return scan_i32_impl<int32_t>(c, in, out, n, total_host);
+}
+hipError_t scan_i32_to_i64(f2m_ctx* c, const int32_t* in, int64_t* out, int64_t n, int64_t* total_host) {
+  return scan_i32_impl<int64_t>(c, in, out, n, total_host);
+}
+
+}  // namespace f2m
+
+// ===========================================================================
+extern "C" {
+
+void f2m_default_opts(f2m_opts* o) {
+  if (!o) return;
+  o->err_grad = 1e-7;
+  o->err_comb_mult = 1e2;
+  o->glob_k = 5;
+  o->glob_rounds = 200;
+  o->glob_sweeps = 50;
+  o->glob_exit = 0.01;
+  o->max_outer = 10000;
+  o->factor0 = 100;
+  o->factor_step = 300;
+  o->factor_max = 5000;
+  o->check_every = 50;
+  o->early_exit = 0;
+  o->repair = 1;
+  o->shift0 = 0.5;
+  o->shift_max = 32;
+  o->fix_rounds = 100;
+  o->fix_factor = 500;
+  o->frozen_sweeps = 20;
+  o->stop_phase = 0;
+}
+
+const char* f2m_strerror(f2m_status s) {
+  switch (s) {
+    case F2M_OK: return "ok";
+    case F2M_ERR_ARG: return "invalid argument";
+    case F2M_ERR_HIP: return "HIP runtime error";
+    case F2M_ERR_FORMAT: return "not a TSPLIB coordinate file";
+    case F2M_ERR_IO: return "file I/O error";
+    case F2M_ERR_DEGENERATE: return "degenerate instance";
+    case F2M_ERR_LIMIT: return "size limit exceeded";
+    default: return "unknown error";
+  }
+}
+
+const char* f2m_last_error(const f2m_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+const char* f2m_version(void) { return "f2m_hip gfx950 " __DATE__ " " __TIME__; }
+
+f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out) {
+  if (!out) return F2M_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) return F2M_ERR_HIP;
+  if (device < 0 || device >= ndev) return F2M_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return F2M_ERR_HIP;
+  f2m_ctx* c = new f2m_ctx;
+  c->device = device;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return F2M_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  if (hipHostMalloc(&c->pinned, 1 << 16, hipHostMallocDefault) != hipSuccess) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return F2M_ERR_HIP;
+  }
+  *out = c;
+  return F2M_OK;
+}
+
+static void release_ws(f2m::SolveWs& w) {
+  f2m::DevBuf* all[] = {&w.nbr, &w.D, &w.orig, &w.F0, &w.F1, &w.g, &w.act, &w.clen, &w.deg,
+                        &w.A, &w.T, &w.root, &w.irr, &w.frozen, &w.xw, &w.xc, &w.red, &w.axb,
+                        &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp};
+  for (auto* b : all) b->release();
+}
+
+void f2m_ctx_destroy(f2m_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  f2m::DevBuf* all[] = {&c->g_rowptr, &c->g_nbr, &c->g_cost, &c->k_x, &c->k_y, &c->k_cell,
+                        &c->k_cstart, &c->k_order, &c->k_sx, &c->k_sy, &c->k_knn, &c->k_kcost,
+                        &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->b_node_off,
+                        &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_inst_of, &c->b_stats,
+                        &c->b_obj, &c->b_work};
+  for (auto* b : all) b->release();
+  for (auto& s : c->stage) s.release();
+  release_ws(c->ws);
+  release_ws(c->bws);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+f2m_status f2m_ctx_synchronize(f2m_ctx* c) {
+  if (!c) return F2M_ERR_ARG;
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  return F2M_OK;
+}
+
+}  // extern "C"

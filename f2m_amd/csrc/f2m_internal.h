@@ -1,0 +1,189 @@
+// f2m_internal.h -- shared declarations of the f2m_hip library (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "f2m.h"
+
+namespace f2m {
+
+// Grow-only device buffer.  Reallocation frees first, so callers must not hold
+// raw pointers across a reserve() of the same buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes < 256 ? 256 : bytes);
+    if (e == hipSuccess) cap = bytes < 256 ? 256 : bytes;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Solve workspace for one graph (single instance or a packed batch).
+struct SolveWs {
+  DevBuf nbr, D, orig;          // working slots (permuted in place), per slot
+  DevBuf F0, F1, g;             // potentials (double-buffered), gradient
+  DevBuf act, clen, deg, A, T;  // per node
+  DevBuf root, irr, frozen;     // per node
+  DevBuf xw, xc;                // x per working slot / per canonical slot
+  DevBuf red;                   // reduction scratch (doubles / max bits)
+  DevBuf axb;                   // least-squares scratch
+  DevBuf sortk, sortv, sortk2, sortv2, sorttmp;  // comb node sort
+  int cur = 0;                  // which F buffer holds the current potentials
+  int64_t n = 0, m = 0;
+};
+
+}  // namespace f2m
+
+struct f2m_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  // graph storage (f2m_knn_build / f2m_graph_from_csr)
+  f2m::DevBuf g_rowptr, g_nbr, g_cost;
+  // kNN scratch
+  f2m::DevBuf k_x, k_y, k_cell, k_cstart, k_order, k_sx, k_sy, k_knn, k_kcost, k_rev, k_cur,
+      k_deg, k_red, k_scan;
+  // staging of host arrays
+  f2m::DevBuf stage[6];
+  f2m::SolveWs ws;
+  // batched storage
+  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_inst_of, b_stats, b_obj, b_work;
+  f2m::SolveWs bws;
+  int64_t b_n_total = 0, b_nslots = 0;
+  int32_t b_n_inst = 0;
+  // pinned host scalars for small read-backs
+  void* pinned = nullptr;
+  // last solved graph (for f2m_result_copy)
+  int64_t last_n = 0, last_m = 0;
+};
+
+namespace f2m {
+
+f2m_status fail(f2m_ctx* c, f2m_status s, const std::string& msg);
+f2m_status hip_fail(f2m_ctx* c, hipError_t e, const char* what, const char* file, int line);
+
+// Device pointer for p: p itself if device-accessible, else a staged copy in
+// ctx->stage[slot].  Returns nullptr and sets the error on failure.
+const void* stage_in(f2m_ctx* c, const void* p, size_t bytes, int slot);
+bool is_device_ptr(const void* p);
+// Copy bytes from device src to dst (device or host), stream-ordered + synced.
+hipError_t copy_out(f2m_ctx* c, void* dst, const void* src, size_t bytes);
+
+// Exclusive scan of int32/int64 counts (n entries) into out (n+1 entries,
+// out[n] = total); total_host may be null.  tmp is scratch.
+hipError_t scan_i32_to_i32(f2m_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* total_host);
+hipError_t scan_i32_to_i64(f2m_ctx* c, const int32_t* in, int64_t* out, int64_t n, int64_t* total_host);
+
+inline int grid_for(int64_t threads, int block) { return (int)((threads + block - 1) / block); }
+
+}  // namespace f2m
+
+#define F2M_HIP(c, expr)                                                   \
+  do {                                                                     \
+    hipError_t e__ = (expr);                                               \
+    if (e__ != hipSuccess) return f2m::hip_fail((c), e__, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define F2M_LAUNCH_CHECK(c) F2M_HIP((c), hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Device helpers shared by the solve, batch and comb kernels.
+// ---------------------------------------------------------------------------
+namespace f2m {
+
+// Reduced cost of slot (v -> w): d = (D + 0.5*F[lo]) + 0.5*F[hi], lo = min id
+// (graph_cuda_grad.hpp:9; Edges_v1 < Edges_v2, graph_cuda_init.hpp:25-27).
+// Compiled with -ffp-contract=off; 0.5*F is exact, so the order of the two
+// roundings is the only thing that matters and it is fixed here.
+__device__ __forceinline__ double dval(double D, double Fv, double Fw, int v, int w) {
+  double a = v < w ? Fv : Fw;
+  double b = v < w ? Fw : Fv;
+  double t = D + 0.5 * a;
+  return t + 0.5 * b;
+}
+
+// Three smallest values, sentinels 1e6 (graph_cuda_grad.hpp:68-79).
+struct Top3 {
+  double a, b, c;
+  __device__ __forceinline__ Top3() : a(1e6), b(1e6), c(1e6) {}
+  __device__ __forceinline__ void push(double v) {
+    if (v < a) { c = b; b = a; a = v; }
+    else if (v < b) { c = b; b = v; }
+    else if (v < c) { c = v; }
+  }
+  __device__ __forceinline__ double grad() const { return -(b + c) / 2; }
+};
+
+// Hoare partition of node v's slots [base, base+size) around thr, permuting
+// (nbr, D, orig) in place (qpart_ptr_B, graph_cuda_partition.hpp:3-25).
+__device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
+                              double* __restrict__ D, int* __restrict__ orig,
+                              const double* __restrict__ F) {
+  const double Fv = F[v];
+  int i = 0, j = size - 1;
+  while (i < j) {
+    while (i <= size - 1) {
+      int w = nbr[base + i];
+      if (!(dval(D[base + i], Fv, F[w], v, w) <= thr)) break;
+      ++i;
+    }
+    while (j >= 0) {
+      int w = nbr[base + j];
+      if (!(dval(D[base + j], Fv, F[w], v, w) > thr)) break;
+      --j;
+    }
+    if (i < j) {
+      int64_t si = base + i, sj = base + j;
+      int tn = nbr[si]; nbr[si] = nbr[sj]; nbr[sj] = tn;
+      double tD = D[si]; D[si] = D[sj]; D[sj] = tD;
+      int to = orig[si]; orig[si] = orig[sj]; orig[sj] = to;
+    }
+  }
+  return j + 1;
+}
+
+// qpart_ptr_EF (graph_cuda_partition.hpp:27-54): raise thr by 1 until >= 3
+// entries lie at or below it.  Re-partitions that admit no new entry swap
+// nothing, so only thresholds admitting a new entry are partitioned.
+__device__ inline int qpart_EF(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
+                               double* __restrict__ D, int* __restrict__ orig,
+                               const double* __restrict__ F) {
+  int p = qpart_B(v, base, size, thr, nbr, D, orig, F);
+  if (size < 3) return p;
+  const double Fv = F[v];
+  while (p < 3) {
+    double m = 1.0 / 0.0;
+    for (int k = p; k < size; ++k) {
+      int w = nbr[base + k];
+      double d = dval(D[base + k], Fv, F[w], v, w);
+      m = d < m ? d : m;
+    }
+    long long guard = 0;
+    do { thr += 1.0; } while (!(m <= thr) && ++guard < (1ll << 40));
+    p = qpart_B(v, base, size, thr, nbr, D, orig, F);
+  }
+  return p;
+}
+
+// |x| of a double as an order-preserving 64-bit key (for atomicMax).
+__device__ __forceinline__ unsigned long long abs_bits(double x) {
+  return (unsigned long long)__double_as_longlong(fabs(x));
+}
+
+}  // namespace f2m

@@ -1,0 +1,412 @@
+// f2m_knn.hip -- init: grid-bucketed kNN edge construction on the GPU.
+//
+// Replaces the reference's O(n^2)-per-pass radius search and candidate sort
+// (cuda_D2_stat / cuda_D2_fill / thrust::sort, FXY.hpp:58-144, 313-534) and
+// the host greedy selection (graph_tour.hpp:28-98).  The selected edge set is
+// the same: the union over v of the K nearest neighbours of v, where "nearest"
+// orders candidates by the key (cost, id) -- the reference's global
+// (d, min id, max id) candidate order restricted to one node's edges.
+//
+// Pipeline (all on the context stream):
+//   bbox -> cell id + count -> scan -> scatter into cell order
+//   -> per-point ring search, best-K list in LDS  (k_knn)
+//   -> reverse-edge count -> scan -> fill -> per-node sort by (cost, nbr).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "f2m_internal.h"
+
+namespace f2m {
+
+// Edge cost (FXY.hpp:376-383).  -ffp-contract=off keeps dx*dx + dy*dy as two
+// roundings, identical to the host; gfx950 f64 sqrt and division lower to
+// correctly rounded sequences.
+__device__ __forceinline__ double edge_cost(double xu, double yu, double xw, double yw, int mode) {
+  double dx = xu - xw, dy = yu - yw;
+  double s = dx * dx + dy * dy;
+  double c = sqrt(s);
+  if (mode == F2M_COST_NINT) return floor(c + 0.5);
+  if (mode == F2M_COST_RAW_FP32) return (double)(float)c;
+  return c;
+}
+__device__ __forceinline__ double round_like_cost(double dist, int mode) {
+  if (mode == F2M_COST_NINT) return floor(dist + 0.5);
+  if (mode == F2M_COST_RAW_FP32) return (double)(float)dist;
+  return dist;
+}
+
+__device__ __forceinline__ unsigned long long ordered_bits(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double from_ordered_bits(unsigned long long b) {
+  b = (b >> 63) ? (b & 0x7fffffffffffffffull) : ~b;
+  double v;
+  memcpy(&v, &b, 8);
+  return v;
+}
+
+// bbox as ordered bits: [min x, min y, max x, max y]; min via max of ~bits.
+__global__ void k_bbox(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                       unsigned long long* __restrict__ out) {
+  unsigned long long mnx = 0, mny = 0, mxx = 0, mxy = 0;  // all as "max" keys
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    unsigned long long bx = ordered_bits(x[i]), by = ordered_bits(y[i]);
+    mnx = max(mnx, ~bx); mny = max(mny, ~by);
+    mxx = max(mxx, bx); mxy = max(mxy, by);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mnx = max(mnx, (unsigned long long)__shfl_xor((long long)mnx, off, 64));
+    mny = max(mny, (unsigned long long)__shfl_xor((long long)mny, off, 64));
+    mxx = max(mxx, (unsigned long long)__shfl_xor((long long)mxx, off, 64));
+    mxy = max(mxy, (unsigned long long)__shfl_xor((long long)mxy, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], mnx); atomicMax(&out[1], mny);
+    atomicMax(&out[2], mxx); atomicMax(&out[3], mxy);
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_zero_i32(int32_t* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+struct GridGeom {
+  double mnx, mny, cs;
+  int64_t gx, gy;
+};
+
+__global__ void k_cell(const double* __restrict__ x, const double* __restrict__ y, int64_t n, GridGeom g,
+                       int32_t* __restrict__ cell, int32_t* __restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t cx = (int64_t)((x[i] - g.mnx) / g.cs);
+  int64_t cy = (int64_t)((y[i] - g.mny) / g.cs);
+  cx = cx < 0 ? 0 : (cx >= g.gx ? g.gx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= g.gy ? g.gy - 1 : cy);
+  int32_t c = (int32_t)(cy * g.gx + cx);
+  cell[i] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_scatter(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                          const int32_t* __restrict__ cell, const int32_t* __restrict__ cstart,
+                          int32_t* __restrict__ cur, int32_t* __restrict__ order, double* __restrict__ sx,
+                          double* __restrict__ sy) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = cell[i];
+  int32_t p = cstart[c] + atomicAdd(&cur[c], 1);
+  order[p] = (int32_t)i;
+  sx[p] = x[i];
+  sy[p] = y[i];
+}
+
+// One thread per point, visited in cell order so a wave scans nearby cells.
+// best-K list (cost, id) sorted ascending in LDS, column-major per thread.
+template <int T>
+__global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom g,
+                                           const int32_t* __restrict__ cstart,
+                                           const int32_t* __restrict__ order,
+                                           const double* __restrict__ sx, const double* __restrict__ sy,
+                                           const int32_t* __restrict__ cell,
+                                           int32_t* __restrict__ out_id, double* __restrict__ out_c) {
+  extern __shared__ unsigned char smem[];
+  double* bc = reinterpret_cast<double*>(smem);
+  int32_t* bw = reinterpret_cast<int32_t*>(bc + (size_t)K * T);
+  int64_t p = (int64_t)blockIdx.x * T + threadIdx.x;
+  if (p >= n) return;
+  const int t = threadIdx.x;
+  const int32_t v = order[p];
+  const double xv = sx[p], yv = sy[p];
+  const int64_t cx = cell[v] % g.gx, cy = cell[v] / g.gx;
+  int cnt = 0;
+  const int64_t Rmax = g.gx > g.gy ? g.gx : g.gy;
+  for (int64_t R = 0;; ++R) {
+    for (int64_t yy = cy - R; yy <= cy + R; ++yy) {
+      if (yy < 0 || yy >= g.gy) continue;
+      const bool edge_row = (yy == cy - R || yy == cy + R);
+      const int64_t step = (edge_row || R == 0) ? 1 : 2 * R;
+      for (int64_t xx = cx - R; xx <= cx + R; xx += step) {
+        if (xx < 0 || xx >= g.gx) continue;
+        const int64_t c = yy * g.gx + xx;
+        const int32_t e = cstart[c + 1];
+        for (int32_t q = cstart[c]; q < e; ++q) {
+          const int32_t u = order[q];
+          if (u == v) continue;
+          const double cu = edge_cost(xv, yv, sx[q], sy[q], mode);
+          if (cnt == K) {
+            const double lc = bc[(K - 1) * T + t];
+            const int32_t lw = bw[(K - 1) * T + t];
+            if (!(cu < lc || (cu == lc && u < lw))) continue;
+          }
+          int i = cnt < K ? cnt++ : K - 1;
+          while (i > 0) {
+            const double pc = bc[(i - 1) * T + t];
+            const int32_t pw = bw[(i - 1) * T + t];
+            if (!(cu < pc || (cu == pc && u < pw))) break;
+            bc[i * T + t] = pc;
+            bw[i * T + t] = pw;
+            --i;
+          }
+          bc[i * T + t] = cu;
+          bw[i * T + t] = u;
+        }
+      }
+    }
+    if (R > Rmax) break;
+    if (cnt == K) {
+      // every unvisited point is farther than R*cs from v
+      const double lb = round_like_cost((double)R * g.cs * (1.0 - 1e-9), mode);
+      if (bc[(K - 1) * T + t] < lb) break;
+    }
+  }
+  for (int j = 0; j < K; ++j) {
+    out_id[(int64_t)v * K + j] = bw[j * T + t];
+    out_c[(int64_t)v * K + j] = bc[j * T + t];
+  }
+}
+
+__device__ __forceinline__ bool in_list(const int32_t* __restrict__ knn, int64_t w, int K, int32_t v) {
+  const int32_t* l = knn + w * K;
+  for (int j = 0; j < K; ++j)
+    if (l[j] == v) return true;
+  return false;
+}
+
+__global__ void k_rev_count(int64_t n, int K, const int32_t* __restrict__ knn, int32_t* __restrict__ rev) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * K) return;
+  int32_t v = (int32_t)(i / K);
+  int32_t w = knn[i];
+  if (!in_list(knn, w, K, v)) atomicAdd(&rev[w], 1);
+}
+
+__global__ void k_deg(int64_t n, int K, const int32_t* __restrict__ rev, int32_t* __restrict__ deg) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) deg[v] = K + rev[v];
+}
+
+template <class RP>
+__global__ void k_fill(int64_t n, int K, const int32_t* __restrict__ knn, const double* __restrict__ kc,
+                       const RP* __restrict__ rp, int32_t* __restrict__ cur, int32_t* __restrict__ nbr,
+                       double* __restrict__ cost) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * K) return;
+  int32_t v = (int32_t)(i / K);
+  int j = (int)(i % K);
+  int32_t w = knn[i];
+  double c = kc[i];
+  int64_t s = (int64_t)rp[v] + j;
+  nbr[s] = w;
+  cost[s] = c;
+  if (!in_list(knn, w, K, v)) {
+    int64_t q = (int64_t)rp[w] + K + atomicAdd(&cur[w], 1);
+    nbr[q] = v;
+    cost[q] = c;
+  }
+}
+
+// Sort each node's slots by (cost, nbr): the first K are already sorted, the
+// reverse-only entries are inserted.
+template <class RP>
+__global__ void k_sort_seg(int64_t n, const RP* __restrict__ rp, int32_t* __restrict__ nbr,
+                           double* __restrict__ cost) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int64_t b = rp[v], e = rp[v + 1];
+  for (int64_t i = b + 1; i < e; ++i) {
+    double c = cost[i];
+    int32_t w = nbr[i];
+    int64_t j = i;
+    while (j > b && (c < cost[j - 1] || (c == cost[j - 1] && w < nbr[j - 1]))) {
+      cost[j] = cost[j - 1];
+      nbr[j] = nbr[j - 1];
+      --j;
+    }
+    cost[j] = c;
+    nbr[j] = w;
+  }
+}
+
+// Directed kNN of one instance into ctx->k_knn / k_kcost.
+static f2m_status knn_directed_dev(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode) {
+  F2M_HIP(c, c->k_red.reserve(64));
+  unsigned long long* bb = c->k_red.as<unsigned long long>();
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, bb, (int64_t)4);
+  int nb = grid_for(n, 256);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, c->stream, x, y, n, bb);
+  F2M_LAUNCH_CHECK(c);
+  unsigned long long hb[4];
+  F2M_HIP(c, hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, c->stream));
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  GridGeom g;
+  g.mnx = from_ordered_bits(~hb[0]);
+  g.mny = from_ordered_bits(~hb[1]);
+  double mxx = from_ordered_bits(hb[2]), mxy = from_ordered_bits(hb[3]);
+  double w = mxx - g.mnx, h = mxy - g.mny;
+  double cs = std::sqrt(std::max(w * h, 1e-300) / (double)n);
+  cs = std::max(cs, std::max(w, h) / (double)n);
+  if (!(cs > 0) || !std::isfinite(cs)) cs = 1.0;
+  int64_t gx = (int64_t)(w / cs) + 1, gy = (int64_t)(h / cs) + 1;
+  while (gx * gy > 4 * n + 16) {
+    cs *= 1.5;
+    gx = (int64_t)(w / cs) + 1;
+    gy = (int64_t)(h / cs) + 1;
+  }
+  g.cs = cs;
+  g.gx = gx;
+  g.gy = gy;
+  int64_t ncell = gx * gy;
+  F2M_HIP(c, c->k_cell.reserve(n * 4));
+  F2M_HIP(c, c->k_cstart.reserve((ncell + 1) * 4));
+  F2M_HIP(c, c->k_cur.reserve((ncell + 1) * 4 > n * 4 ? (ncell + 1) * 4 : n * 4));
+  F2M_HIP(c, c->k_order.reserve(n * 4));
+  F2M_HIP(c, c->k_sx.reserve(n * 8));
+  F2M_HIP(c, c->k_sy.reserve(n * 8));
+  F2M_HIP(c, c->k_knn.reserve(n * K * 4));
+  F2M_HIP(c, c->k_kcost.reserve(n * K * 8));
+  int32_t* cnt = c->k_cur.as<int32_t>();
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, cnt, ncell);
+  hipLaunchKernelGGL(k_cell, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, y, n, g,
+                     c->k_cell.as<int32_t>(), cnt);
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, scan_i32_to_i32(c, cnt, c->k_cstart.as<int32_t>(), ncell, nullptr));
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, cnt, ncell);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, y, n,
+                     c->k_cell.as<int32_t>(), c->k_cstart.as<int32_t>(), cnt, c->k_order.as<int32_t>(),
+                     c->k_sx.as<double>(), c->k_sy.as<double>());
+  F2M_LAUNCH_CHECK(c);
+  size_t lds;
+  if (K <= 32) {
+    lds = (size_t)K * 128 * 12;
+    hipLaunchKernelGGL(k_knn<128>, dim3(grid_for(n, 128)), dim3(128), lds, c->stream, n, K, mode, g,
+                       c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(), c->k_sx.as<double>(),
+                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), c->k_knn.as<int32_t>(),
+                       c->k_kcost.as<double>());
+  } else {
+    lds = (size_t)K * 64 * 12;
+    hipLaunchKernelGGL(k_knn<64>, dim3(grid_for(n, 64)), dim3(64), lds, c->stream, n, K, mode, g,
+                       c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(), c->k_sx.as<double>(),
+                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), c->k_knn.as<int32_t>(),
+                       c->k_kcost.as<double>());
+  }
+  F2M_LAUNCH_CHECK(c);
+  return F2M_OK;
+}
+
+// Symmetrise ctx->k_knn into a CSR at (rowptr, nbr, cost); RP = int32/int64.
+template <class RP>
+static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, DevBuf& rpb, DevBuf& nbb, DevBuf& cb,
+                             int64_t rp_base_unused, int64_t* nslots_out) {
+  (void)rp_base_unused;
+  F2M_HIP(c, c->k_rev.reserve(n * 4));
+  F2M_HIP(c, c->k_deg.reserve(n * 4));
+  int32_t* rev = c->k_rev.as<int32_t>();
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
+  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K,
+                     c->k_knn.as<int32_t>(), rev);
+  hipLaunchKernelGGL(k_deg, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, K, rev, c->k_deg.as<int32_t>());
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, rpb.reserve((n + 1) * sizeof(RP)));
+  RP total = 0;
+  if (sizeof(RP) == 4)
+    F2M_HIP(c, scan_i32_to_i32(c, c->k_deg.as<int32_t>(), (int32_t*)rpb.p, n, (int32_t*)&total));
+  else
+    F2M_HIP(c, scan_i32_to_i64(c, c->k_deg.as<int32_t>(), (int64_t*)rpb.p, n, (int64_t*)&total));
+  int64_t m = (int64_t)total;
+  F2M_HIP(c, nbb.reserve(m * 4));
+  F2M_HIP(c, cb.reserve(m * 8));
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
+  hipLaunchKernelGGL(k_fill<RP>, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K,
+                     c->k_knn.as<int32_t>(), c->k_kcost.as<double>(), (const RP*)rpb.p, rev, nbb.as<int32_t>(),
+                     cb.as<double>());
+  hipLaunchKernelGGL(k_sort_seg<RP>, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, n, (const RP*)rpb.p,
+                     nbb.as<int32_t>(), cb.as<double>());
+  F2M_LAUNCH_CHECK(c);
+  *nslots_out = m;
+  return F2M_OK;
+}
+
+}  // namespace f2m
+
+using namespace f2m;
+
+extern "C" {
+
+f2m_status f2m_knn_directed(f2m_ctx* c, int32_t n, const double* x, const double* y, int32_t k,
+                            int32_t cost_mode, int32_t* out) {
+  if (!c || !x || !y || !out || n < 2 || k < 1 || cost_mode < 0 || cost_mode > 2) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  int K = k > n - 1 ? n - 1 : k;
+  if (K > 80) return fail(c, F2M_ERR_LIMIT, "k > 80");
+  const double* dx = (const double*)stage_in(c, x, n * 8ull, 0);
+  const double* dy = (const double*)stage_in(c, y, n * 8ull, 1);
+  if (!dx || !dy) return F2M_ERR_HIP;
+  f2m_status s = knn_directed_dev(c, n, dx, dy, K, cost_mode);
+  if (s) return s;
+  F2M_HIP(c, copy_out(c, out, c->k_knn.p, (size_t)n * K * 4));
+  return F2M_OK;
+}
+
+f2m_status f2m_knn_build(f2m_ctx* c, int32_t n, const double* x, const double* y, int32_t k,
+                         int32_t cost_mode, f2m_graph* out) {
+  if (!c || !x || !y || !out || n < 2 || k < 1 || cost_mode < 0 || cost_mode > 2) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  int K = k > n - 1 ? n - 1 : k;
+  if (K > 80) return fail(c, F2M_ERR_LIMIT, "k > 80");
+  if ((int64_t)n * K * 2 > 2000000000ll) return fail(c, F2M_ERR_LIMIT, "graph exceeds 2^31 slots");
+  const double* dx = (const double*)stage_in(c, x, n * 8ull, 0);
+  const double* dy = (const double*)stage_in(c, y, n * 8ull, 1);
+  if (!dx || !dy) return F2M_ERR_HIP;
+  f2m_status s = knn_directed_dev(c, n, dx, dy, K, cost_mode);
+  if (s) return s;
+  int64_t m = 0;
+  s = symmetrise<int32_t>(c, n, K, c->g_rowptr, c->g_nbr, c->g_cost, 0, &m);
+  if (s) return s;
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  out->n = n;
+  out->k = k;
+  out->cost_mode = cost_mode;
+  out->reserved = 0;
+  out->nslots = m;
+  out->d_min = 0;
+  out->rowptr = c->g_rowptr.as<int32_t>();
+  out->nbr = c->g_nbr.as<int32_t>();
+  out->cost = c->g_cost.as<double>();
+  return F2M_OK;
+}
+
+f2m_status f2m_graph_from_csr(f2m_ctx* c, int32_t n, const int32_t* rowptr, const int32_t* nbr,
+                              const double* cost, int64_t nslots, f2m_graph* out) {
+  if (!c || !rowptr || !nbr || !cost || !out || n < 1 || nslots < 0) return F2M_ERR_ARG;
+  if (nslots > 2000000000ll) return fail(c, F2M_ERR_LIMIT, "graph exceeds 2^31 slots");
+  F2M_HIP(c, hipSetDevice(c->device));
+  F2M_HIP(c, c->g_rowptr.reserve((n + 1) * 4ull));
+  F2M_HIP(c, c->g_nbr.reserve(nslots * 4ull));
+  F2M_HIP(c, c->g_cost.reserve(nslots * 8ull));
+  F2M_HIP(c, hipMemcpyAsync(c->g_rowptr.p, rowptr, (n + 1) * 4ull, hipMemcpyDefault, c->stream));
+  F2M_HIP(c, hipMemcpyAsync(c->g_nbr.p, nbr, nslots * 4ull, hipMemcpyDefault, c->stream));
+  F2M_HIP(c, hipMemcpyAsync(c->g_cost.p, cost, nslots * 8ull, hipMemcpyDefault, c->stream));
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  out->n = n;
+  out->k = 0;
+  out->cost_mode = -1;
+  out->reserved = 0;
+  out->nslots = nslots;
+  out->d_min = 0;
+  out->rowptr = c->g_rowptr.as<int32_t>();
+  out->nbr = c->g_nbr.as<int32_t>();
+  out->cost = c->g_cost.as<double>();
+  return F2M_OK;
+}
+
+}  // extern "C"

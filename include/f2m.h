@@ -1,0 +1,202 @@
+/*
+ * f2m.h -- C ABI of the MI355X fractional 2-matching (F2M) solver.
+ *
+ * The reference (aav-antonov/F2M) has no library or FFI: its stable surface
+ * is the CLI `F2M.solver.exe <tsplib> <K>` (RUN.cpp:12-31) and the internal
+ * C++ seam  graph(file,K) -> graph_c(graph*) -> graph::RUN()
+ * (F2M.h:175,181,320).  Each entry point below names the part of that seam
+ * it replaces.  A maintainer binding this from another language (ctypes,
+ * cgo, JNI) needs only these declarations: plain pointers, sizes and PODs.
+ *
+ * Conventions
+ *  - Status codes: 0 ok, < 0 error (the reference calls exit(); this ABI
+ *    never does).  f2m_strerror() names a code, f2m_last_error() gives the
+ *    message of the last failing call on a context.
+ *  - Array arguments marked [dev|host] may be device memory or host memory;
+ *    host memory is staged through the context.  [dev] must be device memory.
+ *  - A context owns all device workspace.  Calls on one context are ordered on
+ *    its stream and are not thread-safe; use one context per host thread.
+ *    Device arrays returned through f2m_graph / f2m_batch stay valid until the
+ *    next build call on the same context or f2m_ctx_destroy().
+ *  - Node ids are the input order after exact-duplicate removal.  Each node's
+ *    slots are sorted by (cost, neighbour id); an undirected edge {u,w}
+ *    appears once in u's slots and once in w's.
+ */
+#ifndef F2M_H
+#define F2M_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t f2m_status;
+#define F2M_OK 0
+#define F2M_ERR_ARG (-1)        /* bad argument                                   */
+#define F2M_ERR_HIP (-2)        /* HIP runtime error (reference: cudaCheckErrors,
+                                   graph_cuda_define.hpp:12-22, exit(1))           */
+#define F2M_ERR_FORMAT (-3)     /* not a TSPLIB coordinate file (FXY.hpp:221-226)  */
+#define F2M_ERR_IO (-4)         /* file cannot be opened or written               */
+#define F2M_ERR_DEGENERATE (-5) /* no positive edge cost / fewer than 2 points     */
+#define F2M_ERR_LIMIT (-6)      /* size beyond a documented limit                  */
+
+/* Edge-cost modes.  RAW_FP64 is the reference's (FXY.hpp:382: unrounded double
+ * distance).  NINT = floor(dist + 0.5).  RAW_FP32 = dist rounded to float. */
+#define F2M_COST_RAW_FP64 0
+#define F2M_COST_NINT 1
+#define F2M_COST_RAW_FP32 2
+
+/* Solve schedule; f2m_default_opts() fills the reference's hard-coded values. */
+typedef struct f2m_opts {
+  double err_grad;        /* 1e-7  convergence of max|g|   graph_cuda.cu:56         */
+  double err_comb_mult;   /* 1e2   eps = err_grad*mult     graph_cuda.cu:57         */
+  int32_t glob_k;         /* 5     warm-start list length  graph_cuda.cu:370        */
+  int32_t glob_rounds;    /* 200                           graph_cuda.cu:375        */
+  int32_t glob_sweeps;    /* 50                            graph_cuda.cu:377        */
+  double glob_exit;       /* 0.01                          graph_cuda.cu:394        */
+  int32_t max_outer;      /* 10000                         graph_cuda.cu:178        */
+  int32_t factor0;        /* 100                           graph_cuda.cu:177        */
+  int32_t factor_step;    /* 300                           graph_cuda.cu:186        */
+  int32_t factor_max;     /* 5000                          graph_cuda.cu:186        */
+  int32_t check_every;    /* 50                            graph_cuda.cu:323        */
+  int32_t early_exit;     /* 0 reference block_EF; 1 break on the current max|g|  */
+  int32_t repair;         /* 1 run the dual-perturbation loop graph_cuda.cu:69-126 */
+  double shift0;          /* 0.5                           graph_cuda.cu:72         */
+  double shift_max;       /* 32                            graph_cuda.cu:124        */
+  int32_t fix_rounds;     /* 100                           graph_cuda.cu:214        */
+  int32_t fix_factor;     /* 500                           graph_cuda.cu:216        */
+  int32_t frozen_sweeps;  /* 20                            graph_cuda_comb.hpp:437  */
+  int32_t stop_phase;     /* 0 full; 1 after warm start; 2 after dual descent      */
+} f2m_opts;
+
+typedef struct f2m_stats {
+  double objective;       /* sum_e x_e cost_e in FP64 (graph_tour.hpp:165-184)     */
+  double dual;            /* d_min * (-sum F + sum_e min(0, d_e))                  */
+  double max_grad;        /* final max |g| (normalised units)                      */
+  double d_min;           /* smallest positive edge cost                           */
+  int64_t sweeps;         /* Jacobi sweeps executed                                */
+  int32_t glob_rounds;
+  int32_t outer_rounds;
+  int32_t n_combs;        /* tight components with >= 2 nodes                      */
+  int32_t n_irregular_a;  /* components failing 2A+B=4                             */
+  int32_t n_irregular_b;  /* ... whose least-squares solve did not converge        */
+  int32_t n_repairs;      /* dual-perturbation attempts                            */
+  int32_t status;         /* 0 "Optimal Found: Yes", 1 "ERROR: Optimal Solution
+                             not Found" (graph_tour.hpp:218-241)                   */
+  int32_t max_rank;       /* MAX edge rank (graph_tour.hpp:169-180)                */
+} f2m_stats;
+
+/* Device CSR graph (replaces graph + graph_c::init/init_cuda,
+ * graph_tour.hpp:2-110 and graph_cuda_init.hpp:3-193). */
+typedef struct f2m_graph {
+  int32_t n;              /* nodes                                                 */
+  int32_t k;              /* neighbours per node used to build it (0: user CSR)    */
+  int32_t cost_mode;
+  int32_t reserved;
+  int64_t nslots;         /* 2E                                                    */
+  double d_min;           /* smallest positive cost (graph_tour.hpp:94)            */
+  const int32_t* rowptr;  /* [dev] n+1                                             */
+  const int32_t* nbr;     /* [dev] nslots                                          */
+  const double* cost;     /* [dev] nslots, raw cost                                */
+} f2m_graph;
+
+/* A batch of independent instances packed back to back (new; no reference
+ * counterpart).  Instance i owns nodes [node_off[i], node_off[i+1]) and slots
+ * [rowptr[node_off[i]], rowptr[node_off[i+1]]); node ids in nbr are local to
+ * the instance. */
+typedef struct f2m_batch {
+  int32_t n_inst;
+  int32_t k;
+  int32_t cost_mode;
+  int32_t reserved;
+  int64_t n_total;
+  int64_t nslots;
+  const int64_t* node_off; /* [dev] n_inst+1                                       */
+  const int64_t* rowptr;   /* [dev] n_total+1, global slot offsets                 */
+  const int32_t* nbr;      /* [dev] nslots, instance-local node ids                */
+  const double* cost;      /* [dev] nslots                                         */
+} f2m_batch;
+
+typedef struct f2m_ctx f2m_ctx;
+typedef struct f2m_points f2m_points;
+
+void f2m_default_opts(f2m_opts* o);
+const char* f2m_strerror(f2m_status s);
+const char* f2m_last_error(const f2m_ctx* ctx);
+/* Library build identifier, e.g. "f2m_hip gfx950 <date>". */
+const char* f2m_version(void);
+
+/* Context on HIP device `device`; `stream` is a hipStream_t (NULL: the context
+ * creates its own non-blocking stream at default priority). */
+f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out);
+void f2m_ctx_destroy(f2m_ctx* ctx);
+f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
+
+/* ---- TSPLIB input (replaces get_size + FXY::get_KXY, FXY.hpp:186-309) ------
+ * Reads NODE_COORD_SECTION up to EOF, rounds coordinates to 1e-3, drops exact
+ * duplicate points (kept label -> dropped labels).  Host-side. */
+f2m_status f2m_tsplib_read(const char* path, f2m_points** out);
+int64_t f2m_points_count(const f2m_points* p);
+int64_t f2m_points_duplicates(const f2m_points* p);
+/* Copies labels/x/y (any may be NULL) into caller host arrays of count(). */
+void f2m_points_copy(const f2m_points* p, int64_t* labels, double* x, double* y);
+/* Writes `<path>` in the reference's _duplicates format (FXY.hpp:168-183). */
+f2m_status f2m_points_write_duplicates(const f2m_points* p, const char* path);
+void f2m_points_free(f2m_points* p);
+
+/* ---- init: kNN edge construction (replaces FXY::FXY kNN + graph::graph
+ * selection, FXY.hpp:23-144 and graph_tour.hpp:28-98) ------------------------
+ * x, y [dev|host] n doubles.  Builds the union over v of the K nearest
+ * neighbours of v under the key (cost, id), as a CSR on the device. */
+f2m_status f2m_knn_build(f2m_ctx* ctx, int32_t n, const double* x, const double* y, int32_t k,
+                         int32_t cost_mode, f2m_graph* out);
+/* Directed kNN only: out [dev|host] n*k int32, row v = v's k nearest sorted. */
+f2m_status f2m_knn_directed(f2m_ctx* ctx, int32_t n, const double* x, const double* y,
+                            int32_t k, int32_t cost_mode, int32_t* out);
+/* Wrap a caller CSR (copied into the context); slots must be sorted as above. */
+f2m_status f2m_graph_from_csr(f2m_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* nbr,
+                              const double* cost, int64_t nslots, f2m_graph* out);
+
+/* ---- solve (replaces graph::RUN, graph_cuda.cu:47-155) ----------------------
+ * Dual descent with the reference schedule, tight components, primal
+ * recovery, objective.  Results stay on the device; copy with f2m_result_copy. */
+f2m_status f2m_solve(f2m_ctx* ctx, const f2m_graph* g, const f2m_opts* o, f2m_stats* st);
+/* F [n] potentials, x [nslots] per canonical slot, comb_root [n] smallest node
+ * id of the node's tight component or -1.  Any pointer may be NULL; [dev|host]. */
+f2m_status f2m_result_copy(f2m_ctx* ctx, double* F, double* x, int32_t* comb_root);
+
+/* ---- kernel-level entry points of the dual descent (graph_cuda_grad.hpp,
+ * graph_cuda_partition.hpp, graph_cuda_maxmin.hpp), used by the parity tests --
+ * One or more Jacobi sweeps over the first len[v] slots of each node, slots in
+ * the given order: g = -(t2+t3)/2, F <- F + g.  D [dev|host] normalised costs
+ * per slot; len [dev|host] n; F_in [dev|host] n; F_out, g_out [dev|host] n.
+ * max_abs_g receives max_v |g_v| of the last sweep (may be NULL). */
+f2m_status f2m_sweep(f2m_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* nbr,
+                     const double* D, const int32_t* len, const double* F_in, double* F_out,
+                     double* g_out, int32_t nsweeps, double* max_abs_g);
+
+/* ---- batched form (new) ------------------------------------------------------
+ * node_off [host] n_inst+1 offsets into x, y [dev|host]; one kNN graph per
+ * instance, all with the same k and cost mode. */
+f2m_status f2m_knn_build_batched(f2m_ctx* ctx, int32_t n_inst, const int64_t* node_off,
+                                 const double* x, const double* y, int32_t k, int32_t cost_mode,
+                                 f2m_batch* out);
+/* Solves every instance with the plain schedule, all in one persistent launch
+ * per phase.  st [host] n_inst stats (may be NULL). */
+f2m_status f2m_solve_batched(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, f2m_stats* st);
+/* x [dev|host] nslots per canonical slot, F [dev|host] n_total. */
+f2m_status f2m_batch_result_copy(f2m_ctx* ctx, double* F, double* x, double* objective);
+
+/* ---- differentiable bound (new) -------------------------------------------
+ * dV/dp for V = sum_e x_e |p_u - p_w| (envelope theorem: x* is fixed).
+ * x_slot [dev] per canonical slot (f2m_solve / f2m_solve_batched output),
+ * px, py [dev] float per node, gx, gy [dev] float per node, scaled by
+ * grad_scale[instance] (NULL: 1).  Batch form: node_off/rowptr from f2m_batch. */
+f2m_status f2m_bound_backward(f2m_ctx* ctx, const f2m_batch* b, const double* x_slot,
+                              const float* px, const float* py, const float* grad_scale,
+                              float* gx, float* gy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
