@@ -409,4 +409,13 @@ f2m_status f2m_graph_from_csr(f2m_ctx* c, int32_t n, const int32_t* rowptr, cons
   return F2M_OK;
 }
 
+f2m_status f2m_graph_copy(f2m_ctx* c, const f2m_graph* g, int32_t* rowptr, int32_t* nbr, double* cost) {
+  if (!c || !g) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  F2M_HIP(c, copy_out(c, rowptr, g->rowptr, (g->n + 1) * 4ull));
+  F2M_HIP(c, copy_out(c, nbr, g->nbr, g->nslots * 4ull));
+  F2M_HIP(c, copy_out(c, cost, g->cost, g->nslots * 8ull));
+  return F2M_OK;
+}
+
 }  // extern "C"

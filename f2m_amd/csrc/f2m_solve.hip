@@ -1,0 +1,1057 @@
+// f2m_solve.hip -- dual descent, partitions, max reductions, tight components,
+// primal recovery and objective for one graph (replaces graph::RUN and the
+// graph_c GPU methods, graph_cuda.cu:47-430 and graph_cuda_*.hpp).
+//
+// Layout.  The canonical CSR (rowptr, nbr, cost) is never modified.  The
+// solver works on a copy of the slots, (nbr, D = cost/d_min, orig slot),
+// which the partitions permute in place within each node's segment, exactly
+// as the reference permutes its per-node pointer lists
+// (graph_cuda_partition.hpp).  A node's active list (the reference's EF
+// array, graph_cuda.cu:229-252) is then simply the first act[v] slots of its
+// segment: no compaction, no pointer chasing, and a sweep reads one
+// contiguous run of (nbr, D) per node.
+//
+// Sweep (graph_cuda_grad.hpp:1-84 fused): one kernel computes, per node, the
+// three smallest reduced costs of its active slots from the current F, the
+// gradient g = -(t2+t3)/2 and F' = F + g into the other F buffer, and the
+// block max |g| into a device slot.  No E-sized d array is stored: d is
+// recomputed from F on the fly with the reference's exact expression.
+// LPN lanes cooperate on a node and merge their top-3 lists with xor
+// shuffles; the merge is a selection, so every LPN gives identical bits.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "f2m_internal.h"
+
+namespace f2m {
+namespace {
+
+constexpr int BS = 256;
+constexpr int RED_BLOCKS = 512;
+constexpr unsigned long long INF_BITS = 0x7FF0000000000000ull;
+
+__global__ void s_zero_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void s_zero_i32(int32_t* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void s_zero_u8(uint8_t* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void s_zero_f64(double* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// Block-wide max of a non-negative value into *slot (as bits, atomicMax).
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* slot) {
+  __shared__ double wm[BS / 64];
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wm[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, wm[k]);
+    atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// smallest positive cost, as bits (positive doubles order like uint64)
+__global__ void s_min_pos(const double* __restrict__ c, int64_t m, unsigned long long* out) {
+  unsigned long long best = INF_BITS;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = c[i];
+    if (v > 0) {
+      unsigned long long b = (unsigned long long)__double_as_longlong(v);
+      best = b < best ? b : best;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = (unsigned long long)__shfl_xor((long long)best, off, 64);
+    best = o < best ? o : best;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(out, best);
+}
+
+// working slots: D = cost / d_min (graph_tour.hpp:94; IEEE division)
+__global__ void s_init_slots(int64_t m, const int32_t* __restrict__ nbr, const double* __restrict__ cost,
+                             const unsigned long long* __restrict__ dmin_bits, int32_t* __restrict__ wnbr,
+                             double* __restrict__ wD, int32_t* __restrict__ worig) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const double dmin = __longlong_as_double((long long)*dmin_bits);
+  wnbr[s] = nbr[s];
+  wD[s] = cost[s] / dmin;
+  worig[s] = (int32_t)s;
+}
+__global__ void s_init_nodes(int n, const int32_t* __restrict__ rp, int32_t* __restrict__ deg,
+                             double* __restrict__ F, uint8_t* __restrict__ frozen) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  deg[v] = rp[v + 1] - rp[v];
+  F[v] = 0.0;
+  frozen[v] = 0;
+}
+
+// setvalue2V (graph_cuda.cu:354-361): act = min(deg, k)
+__global__ void s_act_min(int n, const int32_t* __restrict__ deg, int32_t* __restrict__ act, int k) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) act[v] = deg[v] > k ? k : deg[v];
+}
+
+// Fused sweep / gradient.  WRITE_F=false computes g and max|g| only
+// (cuda_get_Nodes_grad_R + max_node_grad); WRITE_F=true also applies
+// F' = F + g (cuda_update_Nodes_F[_BLOCK]); the reduced costs of the next
+// sweep are recomputed from F' (cuda_recompute_Edges).
+template <int LPN, bool FROZEN, bool WRITE_F>
+__global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__ rp,
+                                              const int32_t* __restrict__ len,
+                                              const int32_t* __restrict__ nbr, const double* __restrict__ D,
+                                              const double* __restrict__ Fin, double* __restrict__ Fout,
+                                              double* __restrict__ gout, const uint8_t* __restrict__ frozen,
+                                              unsigned long long* __restrict__ maxslot) {
+  const int gt = blockIdx.x * BS + threadIdx.x;
+  const int v = gt / LPN, lane = gt % LPN;
+  Top3 t;
+  double Fv = 0.0;
+  const bool ok = v < n;
+  if (ok) {
+    const int b = rp[v], L = len[v];
+    Fv = Fin[v];
+    for (int k = lane; k < L; k += LPN) {
+      const int w = nbr[b + k];
+      t.push(dval(D[b + k], Fv, Fin[w], v, w));
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < LPN; off <<= 1) {
+    const double a2 = __shfl_xor(t.a, off, 64);
+    const double b2 = __shfl_xor(t.b, off, 64);
+    const double c2 = __shfl_xor(t.c, off, 64);
+    t.push(a2);
+    t.push(b2);
+    t.push(c2);
+  }
+  double gabs = 0.0;
+  if (ok && lane == 0) {
+    const double g = t.grad();
+    if (gout) gout[v] = g;
+    if (WRITE_F) Fout[v] = (FROZEN && frozen[v]) ? Fv : Fv + g;
+    gabs = fabs(g);
+  }
+  if (maxslot) block_max_to(gabs, maxslot);
+}
+
+// cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
+__global__ void s_part_v4(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
+                          int32_t* __restrict__ act, int kk, int32_t* __restrict__ nbr, double* __restrict__ D,
+                          int32_t* __restrict__ orig, const double* __restrict__ F) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int dg = deg[v];
+  if (dg < kk) { act[v] = dg; return; }
+  const int b = rp[v];
+  const double Fv = F[v];
+  double thr = dval(D[b], Fv, F[nbr[b]], v, nbr[b]);
+  for (int i = 1; i < kk; ++i) {
+    const int w = nbr[b + i];
+    const double d = dval(D[b + i], Fv, F[w], v, w);
+    thr = d > thr ? d : thr;
+  }
+  act[v] = qpart_EF(v, b, dg, thr, nbr, D, orig, F);
+}
+
+// cuda_partition_v2 (graph_cuda_partition.hpp:57-69)
+__global__ void s_part_v2(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
+                          int32_t* __restrict__ act, double thr, int32_t* __restrict__ nbr,
+                          double* __restrict__ D, int32_t* __restrict__ orig, const double* __restrict__ F) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int dg = deg[v];
+  act[v] = dg < 10 ? dg : qpart_EF(v, rp[v], dg, thr, nbr, D, orig, F);
+}
+
+// cuda_partition_v3 with threshold 0.25 over the active prefix
+// (graph_cuda_comb.hpp:301) + class counts: A = #{d < -eps} (x = 1),
+// T = #{-eps <= d <= eps} (tight) over that prefix (graph_cuda_grad.hpp:18-39).
+__global__ void s_classify(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
+                           int32_t* __restrict__ clen, int32_t* __restrict__ A, int32_t* __restrict__ T,
+                           double eps, int32_t* __restrict__ nbr, double* __restrict__ D,
+                           int32_t* __restrict__ orig, const double* __restrict__ F) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v];
+  const int cl = qpart_B(v, b, act[v], 0.25, nbr, D, orig, F);
+  clen[v] = cl;
+  const double Fv = F[v];
+  int a = 0, t = 0;
+  for (int k = 0; k < cl; ++k) {
+    const int w = nbr[b + k];
+    const double d = dval(D[b + k], Fv, F[w], v, w);
+    if (d < -eps) ++a;
+    else if (d <= eps) ++t;
+  }
+  A[v] = a;
+  T[v] = t;
+}
+
+// ---- union-find over tight edges; par[x] <= x always, so roots are minima --
+__device__ __forceinline__ int uf_find(int* par, int x) {
+  volatile int* vp = par;
+  int p = vp[x];
+  while (p != x) {
+    const int gp = vp[p];
+    if (gp != p) vp[x] = gp;  // path halving; any ancestor is a valid parent
+    x = p;
+    p = vp[x];
+  }
+  return x;
+}
+__global__ void s_uf_init(int n, int32_t* par) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) par[v] = v;
+}
+__global__ void s_uf_union(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ clen, double eps,
+                           const int32_t* __restrict__ nbr, const double* __restrict__ D,
+                           const double* __restrict__ F, int32_t* par) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v];
+  const double Fv = F[v];
+  for (int k = 0; k < clen[v]; ++k) {
+    const int w = nbr[b + k];
+    if (w <= v) continue;
+    const double d = dval(D[b + k], Fv, F[w], v, w);
+    if (!(d >= -eps && d <= eps)) continue;
+    int a = v, c = w;
+    for (;;) {
+      a = uf_find(par, a);
+      c = uf_find(par, c);
+      if (a == c) break;
+      if (a < c) { int t = a; a = c; c = t; }  // hook the larger root under the smaller
+      if (atomicCAS(&par[a], a, c) == a) break;
+    }
+  }
+}
+__global__ void s_uf_root(int n, const int32_t* __restrict__ T, int32_t* par, int32_t* __restrict__ root) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) root[v] = T[v] > 0 ? uf_find(par, v) : -1;
+}
+
+// comb regularity (comb::solve_simple, comb_thread.cpp:180-195): irregular if
+// some node has 2A + T != 4.
+__global__ void s_irr_mark(int n, const int32_t* __restrict__ root, const int32_t* __restrict__ A,
+                           const int32_t* __restrict__ T, int32_t* __restrict__ irr) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int r = root[v];
+  if (r >= 0 && 4 - 2 * A[v] - T[v] != 0) irr[r] = 1;
+}
+
+// x per working slot: 1 on d < -eps (node::get_eA_X), 1/2 on tight slots of
+// regular combs; tight slots of irregular combs are set by the least squares.
+// flag = slot is in the reference's X map (A or tight).
+__global__ void s_primal(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
+                         const int32_t* __restrict__ clen, double eps, const int32_t* __restrict__ nbr,
+                         const double* __restrict__ D, const double* __restrict__ F,
+                         const int32_t* __restrict__ root, const int32_t* __restrict__ irr,
+                         double* __restrict__ xw, uint8_t* __restrict__ flag) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v], cl = clen[v], dg = deg[v];
+  const double Fv = F[v];
+  const int r = root[v];
+  const bool reg = r >= 0 && irr[r] == 0;
+  for (int k = 0; k < dg; ++k) {
+    double x = 0.0;
+    uint8_t f = 0;
+    if (k < cl) {
+      const int w = nbr[b + k];
+      const double d = dval(D[b + k], Fv, F[w], v, w);
+      if (d < -eps) { x = 1.0; f = 1; }
+      else if (d <= eps) { x = reg ? 0.5 : 0.0; f = 1; }
+    }
+    xw[b + k] = x;
+    flag[b + k] = f;
+  }
+}
+
+__global__ void s_to_canon(int64_t m, const int32_t* __restrict__ orig, const double* __restrict__ xw,
+                           const uint8_t* __restrict__ fw, double* __restrict__ xc, uint8_t* __restrict__ fc) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const int o = orig[s];
+  xc[o] = xw[s];
+  fc[o] = fw[s];
+}
+
+// ---- irregular comb node lists --------------------------------------------
+__global__ void s_irr_count(int n, const int32_t* __restrict__ root, const int32_t* __restrict__ irr,
+                            int32_t* __restrict__ cnt) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int r = root[v];
+  if (r >= 0 && irr[r]) atomicAdd(&cnt[r], 1);
+}
+__global__ void s_flag_pos(int n, const int32_t* __restrict__ cnt, int32_t* __restrict__ f) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) f[v] = cnt[v] > 0 ? 1 : 0;
+}
+__global__ void s_root_list(int n, const int32_t* __restrict__ cnt, const int32_t* __restrict__ ridx,
+                            int32_t* __restrict__ roots) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n && cnt[v] > 0) roots[ridx[v]] = v;
+}
+__global__ void s_irr_fill(int n, const int32_t* __restrict__ root, const int32_t* __restrict__ irr,
+                           const int32_t* __restrict__ off, int32_t* __restrict__ cur, int32_t* __restrict__ list) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int r = root[v];
+  if (r >= 0 && irr[r]) list[off[r] + atomicAdd(&cur[r], 1)] = v;
+}
+// Ascending sort of each comb's node segment: one block per comb, bitonic
+// network with implicit +inf padding (mirror-compare variant, so padding
+// stays at the tail and out-of-range pairs are skipped).
+__global__ void s_seg_sort(const int32_t* __restrict__ roots, const int32_t* __restrict__ off,
+                           const int32_t* __restrict__ cnt, int32_t* __restrict__ list) {
+  const int r = roots[blockIdx.x];
+  int32_t* a = list + off[r];
+  const int m = cnt[r];
+  int p2 = 1;
+  while (p2 < m) p2 <<= 1;
+  for (int size = 2; size <= p2; size <<= 1) {
+    const int half = size >> 1;
+    for (int idx = threadIdx.x; idx < p2 / 2; idx += blockDim.x) {
+      const int blk = idx / half, pos = idx % half;
+      const int i = blk * size + pos, j = blk * size + size - 1 - pos;
+      if (j < m && a[i] > a[j]) { int t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    __syncthreads();
+    for (int k = half >> 1; k >= 1; k >>= 1) {
+      for (int idx = threadIdx.x; idx < p2 / 2; idx += blockDim.x) {
+        const int blk = idx / k, pos = idx % k;
+        const int i = blk * 2 * k + pos, j = i + k;
+        if (j < m && a[i] > a[j]) { int t = a[i]; a[i] = a[j]; a[j] = t; }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Ax_b::solve (Ax_b.cpp:139-158) for one irregular comb per block.
+// Equations: nodes v of the comb (ascending), sum_{tight slots s of v} x_s
+// = 2 - A(v), row norm sqrt(T(v)).  Variables: tight edges; both slots of an
+// edge hold the same value because every update of a slot uses the same
+// expression in the canonical (lo, hi) order.  x starts at 0; steepest
+// descent with exact line search; stop when max |r_j| < 1e-8; at most 999
+// iterations.  Results rounded to float (XB is float, F2M.h:202).
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  return s;
+}
+__device__ __forceinline__ double block_maxd(double v, double* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = sh[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, sh[k]);
+  return s;
+}
+
+__global__ void __launch_bounds__(BS) s_axb(const int32_t* __restrict__ roots, const int32_t* __restrict__ off,
+                                            const int32_t* __restrict__ cnt, const int32_t* __restrict__ list,
+                                            const int32_t* __restrict__ rp, const int32_t* __restrict__ clen,
+                                            const int32_t* __restrict__ A, const int32_t* __restrict__ T,
+                                            double eps, const int32_t* __restrict__ nbr, const double* __restrict__ D,
+                                            const double* __restrict__ F, double* __restrict__ xw,
+                                            double* __restrict__ xg, double* __restrict__ rn,
+                                            int32_t* __restrict__ status) {
+  __shared__ double sh[BS / 64];
+  const int r = roots[blockIdx.x];
+  const int32_t* nodes = list + off[r];
+  const int m = cnt[r];
+  // tight slots start at 0 (x0 = 0)
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int v = nodes[i], b = rp[v];
+    const double Fv = F[v];
+    for (int k = 0; k < clen[v]; ++k) {
+      const int w = nbr[b + k];
+      const double d = dval(D[b + k], Fv, F[w], v, w);
+      if (d >= -eps && d <= eps) xw[b + k] = 0.0;
+    }
+  }
+  __syncthreads();
+  int go_max = 0;
+  for (int go = 1; go < 1000; ++go) {
+    double dmax = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i], b = rp[v];
+      const double Fv = F[v];
+      double s = 0.0;
+      for (int k = 0; k < clen[v]; ++k) {
+        const int w = nbr[b + k];
+        const double d = dval(D[b + k], Fv, F[w], v, w);
+        if (d >= -eps && d <= eps) s += xw[b + k];
+      }
+      const double rv = (s - (double)(2 - A[v])) / sqrt((double)T[v]);
+      rn[v] = rv;
+      dmax = fmax(dmax, fabs(rv));
+    }
+    dmax = block_maxd(dmax, sh);  // includes the barrier publishing rn
+    if (dmax < 1e-8) { go_max = go; break; }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i], b = rp[v];
+      const double Fv = F[v];
+      for (int k = 0; k < clen[v]; ++k) {
+        const int w = nbr[b + k];
+        const double d = dval(D[b + k], Fv, F[w], v, w);
+        if (d >= -eps && d <= eps) {
+          const int lo = v < w ? v : w, hi = v < w ? w : v;
+          xg[b + k] = rn[lo] / sqrt((double)T[lo]) + rn[hi] / sqrt((double)T[hi]);
+        }
+      }
+    }
+    __syncthreads();
+    double gg = 0.0, gd = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i], b = rp[v];
+      const double Fv = F[v];
+      double bg = 0.0;
+      for (int k = 0; k < clen[v]; ++k) {
+        const int w = nbr[b + k];
+        const double d = dval(D[b + k], Fv, F[w], v, w);
+        if (d >= -eps && d <= eps) bg += xg[b + k];
+      }
+      gg += bg * bg;
+      gd += rn[v] * bg;
+    }
+    gg = block_sum(gg, sh);
+    gd = block_sum(gd, sh);
+    const double alfa = -gd / gg;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i], b = rp[v];
+      const double Fv = F[v];
+      for (int k = 0; k < clen[v]; ++k) {
+        const int w = nbr[b + k];
+        const double d = dval(D[b + k], Fv, F[w], v, w);
+        if (d >= -eps && d <= eps) xw[b + k] += xg[b + k] * alfa;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int v = nodes[i], b = rp[v];
+    const double Fv = F[v];
+    for (int k = 0; k < clen[v]; ++k) {
+      const int w = nbr[b + k];
+      const double d = dval(D[b + k], Fv, F[w], v, w);
+      if (d >= -eps && d <= eps) xw[b + k] = (double)(float)xw[b + k];
+    }
+  }
+  if (threadIdx.x == 0) status[blockIdx.x] = go_max;
+}
+
+// shift_comb_IRREGULAR_B (graph_cuda_comb.hpp:365-403): one thread per
+// IRREGULAR_B comb; "first violating node" = smallest id.
+__global__ void s_shift(int nb, const int32_t* __restrict__ broots, const int32_t* __restrict__ off,
+                        const int32_t* __restrict__ cnt, const int32_t* __restrict__ list,
+                        const int32_t* __restrict__ A, const int32_t* __restrict__ T, double sv, int type,
+                        double* __restrict__ F, uint8_t* __restrict__ frozen) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  const int r = broots[c];
+  for (int i = 0; i < cnt[r]; ++i) {
+    const int v = list[off[r] + i];
+    const int s = 4 - 2 * A[v] - T[v];
+    if (s != 0) {
+      F[v] += sv * (s > 0 ? 1.0 : -1.0);
+      frozen[v] = 1;
+      if (type == 1) break;
+    }
+  }
+}
+
+// ---- deterministic reductions (fixed grid, fixed per-thread order, fixed tree)
+__device__ __forceinline__ double block_sum_det(double v) {
+  __shared__ double wsum[BS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += wsum[k];
+  return s;
+}
+// partials: [0] objective, [1] sum F, [2] sum min(0,d)
+__global__ void __launch_bounds__(BS) s_sums(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ gnbr,
+                                             const double* __restrict__ gcost, const double* __restrict__ xc,
+                                             const int32_t* __restrict__ wnbr, const double* __restrict__ wD,
+                                             const double* __restrict__ F, double* __restrict__ part) {
+  double obj = 0.0, sf = 0.0, sneg = 0.0;
+  for (int v = blockIdx.x * BS + threadIdx.x; v < n; v += gridDim.x * BS) {
+    const int b = rp[v], e = rp[v + 1];
+    const double Fv = F[v];
+    sf += Fv;
+    for (int s = b; s < e; ++s) {
+      if (gnbr[s] > v) obj += xc[s] * gcost[s];
+      const int w = wnbr[s];
+      if (w > v) sneg += fmin(0.0, dval(wD[s], Fv, F[w], v, w));
+    }
+  }
+  obj = block_sum_det(obj);
+  __syncthreads();
+  sf = block_sum_det(sf);
+  __syncthreads();
+  sneg = block_sum_det(sneg);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = obj;
+    part[RED_BLOCKS + blockIdx.x] = sf;
+    part[2 * RED_BLOCKS + blockIdx.x] = sneg;
+  }
+}
+__global__ void s_sums_final(const double* __restrict__ part, double* __restrict__ out) {
+  // one block of BS threads, RED_BLOCKS partials per quantity, fixed order
+  for (int q = 0; q < 3; ++q) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += BS) v += part[q * RED_BLOCKS + i];
+    v = block_sum_det(v);
+    if (threadIdx.x == 0) out[q] = v;
+    __syncthreads();
+  }
+}
+
+// MAX edge rank (graph_tour.hpp:169-180): over edges in X, min of the two
+// endpoint ranks (0-based position in the node's cost-sorted list).
+__global__ void s_maxrank(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ gnbr,
+                          const uint8_t* __restrict__ fc, int32_t* __restrict__ out) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int best = 0;
+  for (int s = rp[v]; s < rp[v + 1]; ++s) {
+    const int w = gnbr[s];
+    if (w <= v || !fc[s]) continue;
+    int r = s - rp[v];
+    for (int q = rp[w]; q < rp[w + 1]; ++q)
+      if (gnbr[q] == v) { r = min(r, q - rp[w]); break; }
+    best = max(best, r);
+  }
+  if (best > 0) atomicMax(out, best);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host driver
+// ===========================================================================
+namespace {
+
+int env_int(const char* name, int dflt) {
+  const char* s = std::getenv(name);
+  return s ? std::atoi(s) : dflt;
+}
+
+struct Runner {
+  f2m_ctx* c;
+  SolveWs& w;
+  f2m_opts o;
+  int n;
+  int64_t m;
+  const int32_t* rp;    // canonical rowptr (dev)
+  const int32_t* gnbr;  // canonical nbr (dev)
+  const double* gcost;  // canonical cost (dev)
+  double dmin = 0, eps = 0;
+  int64_t sweeps = 0;
+  int lpn = 4;
+  unsigned long long* maxs = nullptr;  // device max slots
+  double* part = nullptr;              // reduction partials
+  double* sums = nullptr;              // final sums [3]
+  int32_t* ibuf = nullptr;             // misc ints
+  // irregular-comb lists
+  int32_t nc = 0;
+  std::vector<int32_t> broots_h;
+
+  double* F() const { return (w.cur ? w.F1 : w.F0).as<double>(); }
+  double* Fo() const { return (w.cur ? w.F0 : w.F1).as<double>(); }
+  int32_t* I(DevBuf& b) const { return b.as<int32_t>(); }
+
+  f2m_status alloc() {
+    F2M_HIP(c, w.nbr.reserve(m * 4));
+    F2M_HIP(c, w.D.reserve(m * 8));
+    F2M_HIP(c, w.orig.reserve(m * 4));
+    F2M_HIP(c, w.xw.reserve(m * 8));
+    F2M_HIP(c, w.xc.reserve(m * 8));
+    F2M_HIP(c, w.sortk.reserve(m * 2));  // flags per working + canonical slot
+    F2M_HIP(c, w.F0.reserve(n * 8));
+    F2M_HIP(c, w.F1.reserve(n * 8));
+    F2M_HIP(c, w.g.reserve(n * 8));
+    F2M_HIP(c, w.act.reserve(n * 4));
+    F2M_HIP(c, w.clen.reserve(n * 4));
+    F2M_HIP(c, w.deg.reserve(n * 4));
+    F2M_HIP(c, w.A.reserve(n * 4));
+    F2M_HIP(c, w.T.reserve(n * 4));
+    F2M_HIP(c, w.root.reserve(n * 4));
+    F2M_HIP(c, w.irr.reserve((n + 1) * 4));
+    F2M_HIP(c, w.frozen.reserve(n));
+    F2M_HIP(c, w.sortv.reserve((int64_t)(n + 1) * 4 * 6));  // cnt, off, cur, ridx, roots, list
+    F2M_HIP(c, w.axb.reserve(m * 8 + (int64_t)n * 8 + (int64_t)(n + 1) * 4));
+    F2M_HIP(c, w.red.reserve(64 * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
+    maxs = w.red.as<unsigned long long>();
+    part = reinterpret_cast<double*>(maxs + 64);
+    sums = part + 3 * RED_BLOCKS;
+    ibuf = reinterpret_cast<int32_t*>(sums + 8);
+    return F2M_OK;
+  }
+
+  f2m_status setup() {
+    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs, (int64_t)64, INF_BITS);
+    int nb = grid_for(m, BS);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(s_min_pos, dim3(nb), dim3(BS), 0, c->stream, gcost, m, maxs);
+    F2M_LAUNCH_CHECK(c);
+    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+    F2M_HIP(c, hipMemcpyAsync(hp, maxs, 8, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    if (hp[0] == INF_BITS) return fail(c, F2M_ERR_DEGENERATE, "no positive edge cost");
+    std::memcpy(&dmin, hp, 8);
+    hipLaunchKernelGGL(s_init_slots, dim3(grid_for(m, BS)), dim3(BS), 0, c->stream, m, gnbr, gcost, maxs,
+                       I(w.nbr), w.D.as<double>(), I(w.orig));
+    w.cur = 0;
+    hipLaunchKernelGGL(s_init_nodes, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg),
+                       w.F0.as<double>(), w.frozen.as<uint8_t>());
+    F2M_LAUNCH_CHECK(c);
+    eps = o.err_grad * o.err_comb_mult;
+    lpn = env_int("F2M_LPN", 4);
+    return F2M_OK;
+  }
+
+  template <int LPN>
+  void launch_sweep_t(bool write, bool frozen, const int32_t* len, unsigned long long* mslot) {
+    dim3 grid(grid_for((int64_t)n * LPN, BS)), block(BS);
+    double* fin = F();
+    double* fout = Fo();
+    const uint8_t* fz = w.frozen.as<uint8_t>();
+    if (!write)
+      hipLaunchKernelGGL((s_sweep<LPN, false, false>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
+                         w.D.as<double>(), fin, nullptr, w.g.as<double>(), fz, mslot);
+    else if (frozen)
+      hipLaunchKernelGGL((s_sweep<LPN, true, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
+                         w.D.as<double>(), fin, fout, nullptr, fz, mslot);
+    else
+      hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
+                         w.D.as<double>(), fin, fout, nullptr, fz, mslot);
+  }
+  f2m_status launch_sweep(bool write, bool frozen, const int32_t* len, unsigned long long* mslot) {
+    switch (lpn) {
+      case 1: launch_sweep_t<1>(write, frozen, len, mslot); break;
+      case 2: launch_sweep_t<2>(write, frozen, len, mslot); break;
+      case 8: launch_sweep_t<8>(write, frozen, len, mslot); break;
+      case 16: launch_sweep_t<16>(write, frozen, len, mslot); break;
+      default: launch_sweep_t<4>(write, frozen, len, mslot); break;
+    }
+    F2M_LAUNCH_CHECK(c);
+    if (write) {
+      w.cur ^= 1;
+      ++sweeps;
+    }
+    return F2M_OK;
+  }
+  f2m_status read_max(int slot, double& out) {
+    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+    F2M_HIP(c, hipMemcpyAsync(hp, maxs + slot, 8, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&out, hp, 8);
+    return F2M_OK;
+  }
+  f2m_status zero_slot(int slot) {
+    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs + slot, (int64_t)1, 0ull);
+    F2M_LAUNCH_CHECK(c);
+    return F2M_OK;
+  }
+  // gradient over act + max |g| (host round trip)
+  f2m_status grad_max(double& M) {
+    f2m_status s;
+    if ((s = zero_slot(0))) return s;
+    if ((s = launch_sweep(false, false, I(w.act), maxs))) return s;
+    return read_max(0, M);
+  }
+  f2m_status sweep_n(int count, bool frozen) {
+    for (int i = 0; i < count; ++i) {
+      f2m_status s = launch_sweep(true, frozen, I(w.act), nullptr);
+      if (s) return s;
+    }
+    return F2M_OK;
+  }
+
+  // block_GLOB (graph_cuda.cu:363-413)
+  f2m_status block_glob(int32_t& rounds) {
+    hipLaunchKernelGGL(s_act_min, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, I(w.deg), I(w.act), o.glob_k);
+    F2M_LAUNCH_CHECK(c);
+    int go = 0;
+    f2m_status s;
+    for (; go < o.glob_rounds; ++go) {
+      if ((s = sweep_n(o.glob_sweeps, false))) return s;
+      hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
+                         o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
+      F2M_LAUNCH_CHECK(c);
+      double M;
+      if ((s = grad_max(M))) return s;
+      if (M < o.glob_exit) { ++go; break; }
+    }
+    rounds = go;
+    return F2M_OK;
+  }
+  // check_OPT + update_EF (graph_cuda.cu:254-271, 229-252)
+  f2m_status check_opt(double& M) {
+    hipLaunchKernelGGL(s_part_v2, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act), 1.0,
+                       I(w.nbr), w.D.as<double>(), I(w.orig), F());
+    F2M_LAUNCH_CHECK(c);
+    return grad_max(M);
+  }
+  // block_EF (graph_cuda.cu:308-338)
+  f2m_status block_ef(int N, bool frozen) {
+    double MA;
+    f2m_status s;
+    if ((s = grad_max(MA))) return s;
+    if (!o.early_exit) {
+      int cnt = (MA < o.err_grad) ? std::min(N, o.check_every) : N;
+      return sweep_n(cnt, frozen);
+    }
+    int done = 0;
+    while (done < N) {
+      int chunk = std::min(o.check_every, N - done);
+      if ((s = sweep_n(chunk - 1, frozen))) return s;
+      if (chunk == o.check_every) {
+        if ((s = zero_slot(1))) return s;
+        if ((s = launch_sweep(true, frozen, I(w.act), maxs + 1))) return s;
+        double MB;
+        if ((s = read_max(1, MB))) return s;
+        done += chunk;
+        if (MB < o.err_grad) break;
+      } else {
+        if ((s = launch_sweep(true, frozen, I(w.act), nullptr))) return s;
+        done += chunk;
+      }
+    }
+    return F2M_OK;
+  }
+  // run_start_cuda (graph_cuda.cu:163-199)
+  f2m_status run_start(double& M, int32_t& rounds) {
+    f2m_status s;
+    if ((s = check_opt(M))) return s;
+    int factor = o.factor0, go = 1;
+    for (; go <= o.max_outer; ++go) {
+      if ((s = block_ef(factor, false))) return s;
+      if ((s = check_opt(M))) return s;
+      if (factor < o.factor_max) factor += o.factor_step;
+      if (M < o.err_grad) break;
+    }
+    rounds = go;
+    return F2M_OK;
+  }
+  // run_fix_cuda (graph_cuda.cu:204-226)
+  f2m_status run_fix(double& M) {
+    f2m_status s;
+    if ((s = check_opt(M))) return s;
+    for (int go = 1; go <= o.fix_rounds; ++go) {
+      if ((s = block_ef(o.fix_factor, false))) return s;
+      if ((s = check_opt(M))) return s;
+      if (M < o.err_grad) break;
+    }
+    return F2M_OK;
+  }
+
+  int32_t* cnt() const { return w.sortv.as<int32_t>(); }
+  int32_t* off() const { return cnt() + (n + 1); }
+  int32_t* cur() const { return off() + (n + 1); }
+  int32_t* ridx() const { return cur() + (n + 1); }
+  int32_t* roots() const { return ridx() + (n + 1); }
+  int32_t* list() const { return roots() + (n + 1); }
+  double* xg() const { return w.axb.as<double>(); }
+  double* rn() const { return xg() + m; }
+  int32_t* status() const { return reinterpret_cast<int32_t*>(rn() + n); }
+  uint8_t* fw() const { return w.sortk.as<uint8_t>(); }
+  uint8_t* fc() const { return fw() + m; }
+
+  // get_combs + host_combs_check (graph_cuda_comb.hpp:2-28, 296-356):
+  // returns the number of IRREGULAR_B combs; fills n_combs / n_irregular_a.
+  f2m_status combs_check(int32_t& ncombs, int32_t& irr_a, int32_t& irr_b) {
+    const dim3 gn(grid_for(n, BS)), bs(BS);
+    double* f = F();
+    hipLaunchKernelGGL(s_classify, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.clen), I(w.A), I(w.T), eps,
+                       I(w.nbr), w.D.as<double>(), I(w.orig), f);
+    int32_t* par = I(w.irr);  // reuse as parent array, then as irregular flags
+    hipLaunchKernelGGL(s_uf_init, gn, bs, 0, c->stream, n, par);
+    hipLaunchKernelGGL(s_uf_union, gn, bs, 0, c->stream, n, rp, I(w.clen), eps, I(w.nbr), w.D.as<double>(), f,
+                       par);
+    hipLaunchKernelGGL(s_uf_root, gn, bs, 0, c->stream, n, I(w.T), par, I(w.root));
+    hipLaunchKernelGGL(s_zero_i32, gn, bs, 0, c->stream, par, (int64_t)n);
+    hipLaunchKernelGGL(s_irr_mark, gn, bs, 0, c->stream, n, I(w.root), I(w.A), I(w.T), par);
+    hipLaunchKernelGGL(s_primal, gn, bs, 0, c->stream, n, rp, I(w.deg), I(w.clen), eps, I(w.nbr),
+                       w.D.as<double>(), f, I(w.root), par, w.xw.as<double>(), fw());
+    F2M_LAUNCH_CHECK(c);
+    // number of combs: roots are nodes with root[v] == v; count via flags
+    hipLaunchKernelGGL(s_zero_i32, gn, bs, 0, c->stream, cnt(), (int64_t)n);
+    hipLaunchKernelGGL(s_irr_count, gn, bs, 0, c->stream, n, I(w.root), par, cnt());
+    hipLaunchKernelGGL(s_flag_pos, gn, bs, 0, c->stream, n, cnt(), cur());
+    F2M_LAUNCH_CHECK(c);
+    int32_t nirr = 0;
+    F2M_HIP(c, scan_i32_to_i32(c, cur(), ridx(), n, &nirr));
+    nc = nirr;
+    irr_a = nirr;
+    irr_b = 0;
+    broots_h.clear();
+    if (nirr > 0) {
+      int32_t tot = 0;
+      F2M_HIP(c, scan_i32_to_i32(c, cnt(), off(), n, &tot));
+      hipLaunchKernelGGL(s_root_list, gn, bs, 0, c->stream, n, cnt(), ridx(), roots());
+      hipLaunchKernelGGL(s_zero_i32, gn, bs, 0, c->stream, cur(), (int64_t)n);
+      hipLaunchKernelGGL(s_irr_fill, gn, bs, 0, c->stream, n, I(w.root), par, off(), cur(), list());
+      hipLaunchKernelGGL(s_seg_sort, dim3(nirr), bs, 0, c->stream, roots(), off(), cnt(), list());
+      hipLaunchKernelGGL(s_axb, dim3(nirr), bs, 0, c->stream, roots(), off(), cnt(), list(), rp, I(w.clen), I(w.A),
+                         I(w.T), eps, I(w.nbr), w.D.as<double>(), f, w.xw.as<double>(), xg(), rn(), status());
+      F2M_LAUNCH_CHECK(c);
+      std::vector<int32_t> st(nirr), rts(nirr);
+      F2M_HIP(c, hipMemcpyAsync(st.data(), status(), nirr * 4, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipMemcpyAsync(rts.data(), roots(), nirr * 4, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipStreamSynchronize(c->stream));
+      for (int i = 0; i < nirr; ++i)
+        if (st[i] == 0) broots_h.push_back(rts[i]);
+      irr_b = (int32_t)broots_h.size();
+    }
+    // total combs = distinct roots: count nodes with root[v] == v
+    ncombs = 0;
+    {
+      hipLaunchKernelGGL(s_zero_i32, dim3(1), dim3(64), 0, c->stream, ibuf, (int64_t)4);
+      F2M_LAUNCH_CHECK(c);
+      count_roots(ncombs);
+    }
+    return F2M_OK;
+  }
+  f2m_status count_roots(int32_t& out);
+
+  f2m_status repair(int32_t& irr_b, int32_t& n_repairs, double& M, int32_t& ncombs, int32_t& irr_a) {
+    double shiftV = o.shift0;
+    f2m_status s;
+    DevBuf& bro = c->stage[5];
+    auto attempt = [&](double sv, int type) -> f2m_status {
+      ++n_repairs;
+      hipLaunchKernelGGL(s_zero_u8, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, w.frozen.as<uint8_t>(),
+                         (int64_t)n);
+      F2M_HIP(c, bro.reserve(broots_h.size() * 4 + 4));
+      F2M_HIP(c, hipMemcpyAsync(bro.p, broots_h.data(), broots_h.size() * 4, hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(s_shift, dim3(grid_for((int64_t)broots_h.size(), 64)), dim3(64), 0, c->stream,
+                         (int)broots_h.size(), bro.as<int32_t>(), off(), cnt(), list(), I(w.A), I(w.T), sv, type,
+                         F(), w.frozen.as<uint8_t>());
+      F2M_LAUNCH_CHECK(c);
+      f2m_status s2;
+      if ((s2 = sweep_n(o.frozen_sweeps, true))) return s2;
+      hipLaunchKernelGGL(s_zero_u8, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, w.frozen.as<uint8_t>(),
+                         (int64_t)n);
+      if ((s2 = run_fix(M))) return s2;
+      return combs_check(ncombs, irr_a, irr_b);
+    };
+    for (;;) {
+      if ((s = attempt(shiftV, 0))) return s;
+      if (irr_b == 0) break;
+      if ((s = attempt(shiftV, 1))) return s;
+      if (irr_b == 0) break;
+      shiftV = -shiftV;
+      if ((s = attempt(shiftV, 0))) return s;
+      if (irr_b == 0) break;
+      shiftV = -shiftV;
+      if ((s = attempt(shiftV, 1))) return s;
+      if (irr_b == 0) break;
+      shiftV = -shiftV;
+      shiftV *= 2;
+      if (std::fabs(shiftV) > o.shift_max) break;
+    }
+    return F2M_OK;
+  }
+
+  f2m_status finish(f2m_stats& z) {
+    hipLaunchKernelGGL(s_to_canon, dim3(grid_for(m, BS)), dim3(BS), 0, c->stream, m, I(w.orig), w.xw.as<double>(),
+                       fw(), w.xc.as<double>(), fc());
+    hipLaunchKernelGGL(s_sums, dim3(RED_BLOCKS), dim3(BS), 0, c->stream, n, rp, gnbr, gcost, w.xc.as<double>(),
+                       I(w.nbr), w.D.as<double>(), F(), part);
+    hipLaunchKernelGGL(s_sums_final, dim3(1), dim3(BS), 0, c->stream, part, sums);
+    hipLaunchKernelGGL(s_zero_i32, dim3(1), dim3(64), 0, c->stream, ibuf, (int64_t)4);
+    hipLaunchKernelGGL(s_maxrank, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, gnbr, fc(), ibuf);
+    F2M_LAUNCH_CHECK(c);
+    double hs[3];
+    int32_t mr = 0;
+    F2M_HIP(c, hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipMemcpyAsync(&mr, ibuf, 4, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    z.objective = hs[0];
+    z.dual = dmin * (-hs[1] + hs[2]);
+    z.max_rank = mr;
+    return F2M_OK;
+  }
+};
+
+__global__ void s_count_roots(int n, const int32_t* __restrict__ root, int32_t* out) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = (v < n && root[v] == v) ? 1 : 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
+  if ((threadIdx.x & 63) == 0 && f) atomicAdd(out, f);
+}
+
+f2m_status Runner::count_roots(int32_t& out) {
+  hipLaunchKernelGGL(s_count_roots, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, I(w.root), ibuf);
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, hipMemcpyAsync(&out, ibuf, 4, hipMemcpyDeviceToHost, c->stream));
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  return F2M_OK;
+}
+
+}  // namespace
+
+f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opts& o, f2m_stats* st) {
+  Runner R{c, w, o, g->n, g->nslots, g->rowptr, g->nbr, g->cost};
+  f2m_status s;
+  w.n = g->n;
+  w.m = g->nslots;
+  if ((s = R.alloc())) return s;
+  if ((s = R.setup())) return s;
+  f2m_stats z;
+  std::memset(&z, 0, sizeof z);
+  z.d_min = R.dmin;
+  if ((s = R.block_glob(z.glob_rounds))) return s;
+  double M = 0;
+  if (o.stop_phase != 1)
+    if ((s = R.run_start(M, z.outer_rounds))) return s;
+  z.max_grad = M;
+  if (o.stop_phase == 1 || o.stop_phase == 2) {
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    z.sweeps = R.sweeps;
+    c->last_n = g->n;
+    c->last_m = g->nslots;
+    if (st) *st = z;
+    return F2M_OK;
+  }
+  int32_t irr_b = 0;
+  if ((s = R.combs_check(z.n_combs, z.n_irregular_a, irr_b))) return s;
+  z.n_irregular_b = irr_b;
+  if (irr_b > 0 && o.repair) {
+    if ((s = R.repair(irr_b, z.n_repairs, M, z.n_combs, z.n_irregular_a))) return s;
+    z.max_grad = M;
+  }
+  z.status = irr_b == 0 ? 0 : 1;
+  if ((s = R.finish(z))) return s;
+  z.sweeps = R.sweeps;
+  c->last_n = g->n;
+  c->last_m = g->nslots;
+  if (st) *st = z;
+  return F2M_OK;
+}
+
+}  // namespace f2m
+
+using namespace f2m;
+
+// Test entry: nsweeps Jacobi sweeps on caller arrays.
+namespace {
+template <int LPN>
+void launch_plain(hipStream_t s, int n, const int32_t* rp, const int32_t* len, const int32_t* nbr, const double* D,
+                  const double* fin, double* fout, double* g, unsigned long long* mx) {
+  hipLaunchKernelGGL((s_sweep<LPN, false, true>), dim3(grid_for((int64_t)n * LPN, BS)), dim3(BS), 0, s, n, rp, len,
+                     nbr, D, fin, fout, g, nullptr, mx);
+}
+}  // namespace
+
+extern "C" {
+
+f2m_status f2m_solve(f2m_ctx* c, const f2m_graph* g, const f2m_opts* o, f2m_stats* st) {
+  if (!c || !g || g->n < 2 || g->nslots < 1) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  f2m_opts oo;
+  if (o) oo = *o; else f2m_default_opts(&oo);
+  return solve_graph(c, c->ws, g, oo, st);
+}
+
+f2m_status f2m_result_copy(f2m_ctx* c, double* F, double* x, int32_t* comb_root) {
+  if (!c || c->last_n <= 0) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  SolveWs& w = c->ws;
+  F2M_HIP(c, copy_out(c, F, (w.cur ? w.F1 : w.F0).p, c->last_n * 8));
+  F2M_HIP(c, copy_out(c, x, w.xc.p, c->last_m * 8));
+  F2M_HIP(c, copy_out(c, comb_root, w.root.p, c->last_n * 4));
+  return F2M_OK;
+}
+
+f2m_status f2m_sweep(f2m_ctx* c, int32_t n, const int32_t* rowptr, const int32_t* nbr, const double* D,
+                     const int32_t* len, const double* F_in, double* F_out, double* g_out, int32_t nsweeps,
+                     double* max_abs_g) {
+  if (!c || n < 1 || !rowptr || !nbr || !D || !len || !F_in || !F_out || nsweeps < 1) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  int32_t m = 0;
+  F2M_HIP(c, hipMemcpy(&m, rowptr + n, 4, hipMemcpyDefault));
+  const int32_t* drp = (const int32_t*)stage_in(c, rowptr, (n + 1) * 4ull, 0);
+  const int32_t* dnb = (const int32_t*)stage_in(c, nbr, m * 4ull, 1);
+  const double* dD = (const double*)stage_in(c, D, m * 8ull, 2);
+  const int32_t* dl = (const int32_t*)stage_in(c, len, n * 4ull, 3);
+  const double* dF = (const double*)stage_in(c, F_in, n * 8ull, 4);
+  if (!drp || !dnb || !dD || !dl || !dF) return F2M_ERR_HIP;
+  SolveWs& w = c->ws;
+  F2M_HIP(c, w.F0.reserve(n * 8ull));
+  F2M_HIP(c, w.F1.reserve(n * 8ull));
+  F2M_HIP(c, w.g.reserve(n * 8ull));
+  F2M_HIP(c, w.red.reserve(4096));
+  F2M_HIP(c, hipMemcpyAsync(w.F0.p, dF, n * 8ull, hipMemcpyDeviceToDevice, c->stream));
+  unsigned long long* mx = w.red.as<unsigned long long>();
+  int lpn = env_int("F2M_LPN", 4);
+  int cur = 0;
+  for (int i = 0; i < nsweeps; ++i) {
+    double* fin = cur ? w.F1.as<double>() : w.F0.as<double>();
+    double* fout = cur ? w.F0.as<double>() : w.F1.as<double>();
+    unsigned long long* ms = (i == nsweeps - 1) ? mx : nullptr;
+    if (ms) hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, mx, (int64_t)1, 0ull);
+    switch (lpn) {
+      case 1: launch_plain<1>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      case 2: launch_plain<2>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      case 8: launch_plain<8>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      case 16: launch_plain<16>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      case 64: launch_plain<64>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      default: launch_plain<4>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+    }
+    F2M_LAUNCH_CHECK(c);
+    cur ^= 1;
+  }
+  F2M_HIP(c, copy_out(c, F_out, cur ? w.F1.p : w.F0.p, n * 8ull));
+  if (g_out) F2M_HIP(c, copy_out(c, g_out, w.g.p, n * 8ull));
+  if (max_abs_g) {
+    unsigned long long b = 0;
+    F2M_HIP(c, copy_out(c, &b, mx, 8));
+    std::memcpy(max_abs_g, &b, 8);
+  }
+  return F2M_OK;
+}
+
+}  // extern "C"

