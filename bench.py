@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""F2M solve benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one plain F2M solve (f2m_solve: dual descent with the reference
+schedule, tight components, primal recovery, FP64 objective) of the
+n=100,000 / K=20 clustered workload: 50-component Gaussian mixture, NINT
+costs, seed 0xF2A2.  The kNN graph is built once before the timed region,
+as the reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).
+Prints one JSON line.  `value` = ms per solve (lower is better).
+
+roofline: the dominant kernel is the fused sweep (s_sweep).  Its algorithmic
+bytes per launch are 12 B per active slot (nbr int32 + D double) + 24 B per
+node (rowptr, list length, F in, F out); the average launch duration comes
+from HIP events the library records on its stream around every 16th sweep
+launch of the timed solves.  traffic is read from profiles/traffic.json
+(rocprofv3 FETCH_SIZE + WRITE_SIZE per sweep launch, scaled by a copy
+kernel of known size) when present.
+
+cpu_baseline: the oracle (the CPU restatement of the reference schedule,
+oracle/f2m_oracle.cpp) solving the same graph with the host's threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "F2M solve ms @n=100k,k=20 (plain vs batched); fwd+bwd ms; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sample-every", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the F2M hot path shards by instance or by spatial region, which is not built here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from f2m_amd import workloads as W
+    from f2m_amd.solver import Context
+
+    x, y, mode, K = W.c2()
+    n = len(x)
+    ctx = Context(0)
+    log("kNN graph build", n, K)
+    t = time.perf_counter()
+    g = ctx.knn_build(x, y, K, mode)
+    ctx.synchronize()
+    knn_ms = (time.perf_counter() - t) * 1e3
+    for i in range(args.warmup):
+        st = ctx.solve(g)
+        log("warmup", i, "sweeps", st["sweeps"], "objective", st["objective"])
+    ctx.set_profile(args.sample_every)
+    prof_tot = {"sweep_launches": 0, "sweep_samples": 0, "sweep_ms_sampled": 0.0, "sweep_alg_bytes": 0.0}
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ctx.solve(g)
+        p = ctx.profile()
+        for k in prof_tot:
+            prof_tot[k] += p[k]
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_profile(0)
+    ms = (t1 - t0) * 1e3 / args.steps
+    log("timed", args.steps, "solves:", ms, "ms/solve")
+
+    launch_ms = prof_tot["sweep_ms_sampled"] / max(1, prof_tot["sweep_samples"])
+    bytes_per_launch = prof_tot["sweep_alg_bytes"] / max(1, prof_tot["sweep_launches"])
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("s_sweep_c2_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "s_sweep", "avg_launch_us": round(launch_ms * 1e3, 3),
+                "alg_bytes_per_launch": round(bytes_per_launch, 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        rp, nb, c = ctx.graph_csr(g)
+        log("cpu baseline: oracle solve with", threads, "threads")
+        t = time.perf_counter()
+        r = O.solve(rp, nb, c, threads=threads)
+        cpu_ms = (time.perf_counter() - t) * 1e3
+        ok = abs(r["stats"]["objective"] - st["objective"]) <= 1e-9 * abs(st["objective"])
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": f"one full solve of the same n={n} K={K} graph ({r['stats']['sweeps']} sweeps), "
+                         f"objective agrees: {ok}",
+               "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1)}
+
+    out = {
+        "metric": METRIC, "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "plain F2M solve, n=100000 50-component Gaussian mixture, NINT costs, K=20",
+                   "n": n, "k": K, "nslots": int(g.nslots), "sweeps": int(st["sweeps"]),
+                   "schedule": "reference", "parallelism": "single GPU"},
+        "solves_per_s": round(1e3 / ms, 3), "objective": st["objective"], "knn_build_ms": round(knn_ms, 3),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

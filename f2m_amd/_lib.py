@@ -54,12 +54,22 @@ class Batch(ctypes.Structure):
     ]
 
 
+class Profile(ctypes.Structure):
+    _fields_ = [
+        ("sweep_launches", c_i64), ("sweep_samples", c_i64), ("sweep_ms_sampled", c_f64),
+        ("sweep_alg_bytes", c_f64), ("active_slots", c_i64), ("solve_ms", c_f64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every symbol declared in include/f2m.h
 EXPORTS = [
     "f2m_default_opts", "f2m_strerror", "f2m_last_error", "f2m_version", "f2m_ctx_create", "f2m_ctx_destroy",
-    "f2m_ctx_synchronize", "f2m_tsplib_read", "f2m_points_count", "f2m_points_duplicates", "f2m_points_copy",
+    "f2m_ctx_synchronize", "f2m_ctx_set_profile", "f2m_get_profile", "f2m_tsplib_read", "f2m_points_count", "f2m_points_duplicates", "f2m_points_copy",
     "f2m_points_write_duplicates", "f2m_points_free", "f2m_knn_build", "f2m_knn_directed", "f2m_graph_from_csr",
-    "f2m_graph_copy", "f2m_solve", "f2m_result_copy", "f2m_sweep", "f2m_knn_build_batched", "f2m_solve_batched",
+    "f2m_graph_copy", "f2m_solve", "f2m_result_copy", "f2m_sweep", "f2m_knn_build_batched", "f2m_solve_batched", "f2m_batch_autotune", "f2m_batch_copy",
     "f2m_batch_result_copy", "f2m_bound_backward",
 ]
 
@@ -71,6 +81,8 @@ _sig = {
     "f2m_ctx_create": (c_i32, [c_i32, c_p, ctypes.POINTER(c_p)]),
     "f2m_ctx_destroy": (None, [c_p]),
     "f2m_ctx_synchronize": (c_i32, [c_p]),
+    "f2m_ctx_set_profile": (c_i32, [c_p, c_i32]),
+    "f2m_get_profile": (c_i32, [c_p, ctypes.POINTER(Profile)]),
     "f2m_tsplib_read": (c_i32, [ctypes.c_char_p, ctypes.POINTER(c_p)]),
     "f2m_points_count": (c_i64, [c_p]),
     "f2m_points_duplicates": (c_i64, [c_p]),
@@ -86,6 +98,8 @@ _sig = {
     "f2m_sweep": (c_i32, [c_p, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i32, c_p]),
     "f2m_knn_build_batched": (c_i32, [c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, ctypes.POINTER(Batch)]),
     "f2m_solve_batched": (c_i32, [c_p, ctypes.POINTER(Batch), ctypes.POINTER(Opts), c_p]),
+    "f2m_batch_autotune": (c_i32, [c_p, ctypes.POINTER(Batch), ctypes.POINTER(Opts), c_i32, ctypes.c_char_p]),
+    "f2m_batch_copy": (c_i32, [c_p, ctypes.POINTER(Batch), c_p, c_p, c_p, c_p]),
     "f2m_batch_result_copy": (c_i32, [c_p, c_p, c_p, c_p]),
     "f2m_bound_backward": (c_i32, [c_p, ctypes.POINTER(Batch), c_p, c_p, c_p, c_p, c_p, c_p]),
 }

@@ -1,0 +1,60 @@
+"""Differentiable F2M bound: torch.autograd.Function over the C ABI.
+
+    V = F2MBound.apply(px, py, node_off, k, solver, opts)
+
+px, py: float32 CUDA tensors holding the points of a batch of instances
+packed back to back (instance i = rows node_off[i]:node_off[i+1]).  Forward
+builds the symmetrised kNN graph of each instance with RAW_FP32 costs
+(float-rounded Euclidean distances), solves all instances with one batched
+persistent launch and returns the optimum V_i of each instance (float64).
+Backward runs the hand-written kernel f2m_bound_backward:
+    dV_i/dp_u = sum_{e=(u,w)} x*_e (p_u - p_w) / |p_u - p_w|
+(envelope theorem; valid where x* is unique and the kNN graph is locally
+constant), scaled by the upstream gradient of the instance.
+
+The solver context's batch buffers are reused by the next forward on the
+same context: call backward before the next forward, or use one context per
+pending graph.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .solver import COST_RAW_FP32, Context
+
+
+class F2MBound(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, px, py, node_off, k, solver: Context, opts=None):
+        if not (px.is_cuda and py.is_cuda and px.dtype == torch.float32 and py.dtype == torch.float32):
+            raise TypeError("px, py must be float32 CUDA tensors")
+        node_off = np.ascontiguousarray(np.asarray(node_off), np.int64)
+        x64 = px.detach().to(torch.float64).contiguous()
+        y64 = py.detach().to(torch.float64).contiguous()
+        torch.cuda.current_stream().synchronize()
+        b = solver.knn_build_batched((x64, node_off), y64, k, COST_RAW_FP32)
+        solver.solve_batched(b, opts)
+        xs = torch.empty(b.nslots, dtype=torch.float64, device=px.device)
+        res = solver.batch_result(b, F=np.zeros(b.n_total), x=xs)
+        ctx.save_for_backward(px.detach().contiguous(), py.detach().contiguous(), xs)
+        ctx.solver = solver
+        ctx.batch = b
+        return torch.from_numpy(res["objective"]).to(px.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        px, py, xs = ctx.saved_tensors
+        scale = gout.detach().to(torch.float32).contiguous()
+        gx = torch.empty_like(px)
+        gy = torch.empty_like(py)
+        torch.cuda.current_stream().synchronize()
+        ctx.solver.bound_backward(ctx.batch, xs, px, py, scale, gx, gy)
+        ctx.solver.synchronize()
+        return gx, gy, None, None, None, None
+
+
+def f2m_bound(px, py, node_off, k=20, solver: Context | None = None, opts=None):
+    """Convenience wrapper; creates a context on px's device when none is given."""
+    solver = solver or Context(px.device.index or 0)
+    return F2MBound.apply(px, py, node_off, k, solver, opts)

@@ -1,0 +1,879 @@
+// f2m_batch.hip -- batched F2M: thousands of independent instances in one
+// persistent launch (new; the reference solves one instance per process).
+//
+// Each workgroup pulls instances from a queue ordered by decreasing size
+// (longest-processing-time first) and runs the complete plain schedule of
+// graph::RUN for that instance -- warm start, active-set descent, tight
+// components, least squares, dual repair, objective -- with workgroup
+// barriers where the single-graph path has kernel boundaries.  Every step
+// uses the same device routines and the same arithmetic as f2m_solve, so an
+// instance's potentials, components and x equal those of a plain solve of
+// the same graph bit for bit.
+//
+// Instances are grouped into size buckets; each bucket is one persistent
+// launch with its own (lanes per node) schedule.  The whole sequence (queue
+// reset kernels + one launch per bucket) is captured once into a hipGraph
+// and replayed on later calls with the same batch and options.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "f2m_device.h"
+
+namespace f2m {
+namespace {
+
+constexpr int BB = 256;  // threads per workgroup (one instance at a time)
+
+struct BArgs {
+  int n_inst;
+  const int64_t* node_off;
+  const int64_t* rowptr;
+  const int32_t* gnbr;
+  const double* gcost;
+  int32_t* nbr;
+  double* D;
+  int32_t* orig;
+  double* F0;
+  double* F1;
+  int32_t *act, *deg, *clen, *A, *T, *root, *irr, *aux, *list;
+  uint8_t* frozen;
+  double *xw, *xc, *xg, *rn;
+  uint8_t *fw, *fc;
+  const int32_t* order;  // instance ids of this bucket, decreasing size
+  int32_t n_order;
+  int32_t* queue;
+  f2m_opts o;
+  f2m_stats* stats;
+  int32_t* cur_out;
+  long long* clk;  // per-instance elapsed wall_clock64 ticks (autotuning), or null
+};
+
+// Instance view: local node ids, pointers offset to the instance.
+struct Inst {
+  int n;
+  int64_t nb, sb, m;
+  const int64_t* rp;  // global slot offsets of local nodes (rowptr + nb)
+  int32_t* nbr;
+  double* D;
+  int32_t* orig;
+  const int32_t* gnbr;
+  const double* gcost;
+  double* F[2];
+  int32_t *act, *deg, *clen, *A, *T, *root, *irr, *aux, *list;
+  uint8_t* frozen;
+  double *xw, *xc, *xg, *rn;
+  uint8_t *fw, *fc;
+  __device__ __forceinline__ int64_t b(int v) const { return rp[v] - sb; }
+};
+
+struct LocalRP {
+  const int64_t* rp;
+  int64_t sb;
+  __device__ __forceinline__ int64_t operator()(int v) const { return rp[v] - sb; }
+};
+
+// One pass over all nodes: gradient over the first len[v] slots from F[cur];
+// if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns max |g|
+// to every thread (barriers inside).
+template <int LPN>
+__device__ double wg_pass(const Inst& I, const int32_t* len, int cur, bool write, bool use_frozen, double* sh) {
+  const double* Fin = I.F[cur];
+  double* Fout = I.F[cur ^ 1];
+  double gmax = 0.0;
+  const int total = I.n * LPN;
+  for (int base = 0; base < total; base += BB) {
+    const int gt = base + threadIdx.x;
+    const int v = gt / LPN, lane = gt % LPN;
+    Top3 t;
+    double Fv = 0.0;
+    const bool ok = v < I.n;
+    if (ok) {
+      const int64_t b = I.b(v);
+      const int L = len[v];
+      Fv = Fin[v];
+      for (int k = lane; k < L; k += LPN) {
+        const int w = I.nbr[b + k];
+        t.push(dval(I.D[b + k], Fv, Fin[w], v, w));
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < LPN; off <<= 1) {
+      const double a2 = __shfl_xor(t.a, off, 64);
+      const double b2 = __shfl_xor(t.b, off, 64);
+      const double c2 = __shfl_xor(t.c, off, 64);
+      t.push(a2);
+      t.push(b2);
+      t.push(c2);
+    }
+    if (ok && lane == 0) {
+      const double g = t.grad();
+      if (write) Fout[v] = (use_frozen && I.frozen[v]) ? Fv : Fv + g;
+      gmax = fmax(gmax, fabs(g));
+    }
+  }
+  return block_max_all(gmax, sh);
+}
+
+// Ascending ids v in [0,n) with pred(v) into out; returns the count.
+template <class P>
+__device__ int wg_compact(int n, P pred, int32_t* out, int* shi) {
+  int total = 0;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += BB) {
+    const int v = base + threadIdx.x;
+    const int f = (v < n && pred(v)) ? 1 : 0;
+    const unsigned long long bal = __ballot(f);
+    const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (lane == 0) shi[wid] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < BB / 64; ++k) {
+      if (k < wid) off += shi[k];
+      tot += shi[k];
+    }
+    if (f) out[total + off + wpre] = v;
+    total += tot;
+  }
+  __syncthreads();
+  return total;
+}
+
+template <int LPN>
+struct Sched {
+  const Inst& I;
+  const f2m_opts& o;
+  double* sh;
+  int* shi;
+  int cur = 0;
+  int64_t sweeps = 0;
+  double eps;
+
+  __device__ double grad_act() { return wg_pass<LPN>(I, I.act, cur, false, false, sh); }
+  __device__ double sweep(bool frozen) {
+    const double m = wg_pass<LPN>(I, I.act, cur, true, frozen, sh);
+    cur ^= 1;
+    ++sweeps;
+    return m;
+  }
+  __device__ int block_glob() {
+    for (int v = threadIdx.x; v < I.n; v += BB) I.act[v] = I.deg[v] > o.glob_k ? o.glob_k : I.deg[v];
+    __syncthreads();
+    int go = 0;
+    for (; go < o.glob_rounds; ++go) {
+      for (int s = 0; s < o.glob_sweeps; ++s) sweep(false);
+      const double* F = I.F[cur];
+      for (int v = threadIdx.x; v < I.n; v += BB)
+        I.act[v] = part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
+      __syncthreads();
+      const double M = grad_act();
+      if (M < o.glob_exit) { ++go; break; }
+    }
+    return go;
+  }
+  __device__ double check_opt() {
+    const double* F = I.F[cur];
+    for (int v = threadIdx.x; v < I.n; v += BB)
+      I.act[v] = part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
+    __syncthreads();
+    return grad_act();
+  }
+  __device__ void block_ef(int N, bool frozen) {
+    const double MA = grad_act();
+    if (!o.early_exit) {
+      const int cnt = (MA < o.err_grad) ? min(N, o.check_every) : N;
+      for (int i = 0; i < cnt; ++i) sweep(frozen);
+      return;
+    }
+    int done = 0;
+    while (done < N) {
+      const int chunk = min(o.check_every, N - done);
+      double MB = 0.0;
+      for (int i = 0; i < chunk; ++i) MB = sweep(frozen);
+      done += chunk;
+      if (chunk == o.check_every && MB < o.err_grad) break;
+    }
+  }
+  __device__ int run_start(double& M) {
+    M = check_opt();
+    int factor = o.factor0, go = 1;
+    for (; go <= o.max_outer; ++go) {
+      block_ef(factor, false);
+      M = check_opt();
+      if (factor < o.factor_max) factor += o.factor_step;
+      if (M < o.err_grad) break;
+    }
+    return go;
+  }
+  __device__ double run_fix() {
+    double M = check_opt();
+    for (int go = 1; go <= o.fix_rounds; ++go) {
+      block_ef(o.fix_factor, false);
+      M = check_opt();
+      if (M < o.err_grad) break;
+    }
+    return M;
+  }
+  // classes, tight components, primal, least squares; returns #IRREGULAR_B
+  __device__ int combs_check(int& ncombs, int& irr_a) {
+    const double* F = I.F[cur];
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      int cl, a, t;
+      classify_node(v, I.b(v), I.act[v], eps, I.nbr, I.D, I.orig, F, cl, a, t);
+      I.clen[v] = cl;
+      I.A[v] = a;
+      I.T[v] = t;
+      I.aux[v] = v;  // union-find parents
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      const int64_t b = I.b(v);
+      const double Fv = F[v];
+      for (int k = 0; k < I.clen[v]; ++k) {
+        const int w = I.nbr[b + k];
+        if (w <= v) continue;
+        const double d = dval(I.D[b + k], Fv, F[w], v, w);
+        if (d >= -eps && d <= eps) uf_unite(I.aux, v, w);
+      }
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < I.n; v += BB) I.root[v] = I.T[v] > 0 ? uf_find(I.aux, v) : -1;
+    __syncthreads();
+    for (int v = threadIdx.x; v < I.n; v += BB) I.irr[v] = 0;
+    __syncthreads();
+    double nc = 0.0;
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      const int r = I.root[v];
+      if (r >= 0 && 4 - 2 * I.A[v] - I.T[v] != 0) I.irr[r] = 1;
+      if (r == v) nc += 1.0;
+    }
+    ncombs = (int)block_sum_all(nc, sh);
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      const int r = I.root[v];
+      primal_node(v, I.b(v), I.deg[v], I.clen[v], eps, I.nbr, I.D, F, r >= 0 && I.irr[r] == 0, I.xw, I.fw);
+    }
+    __syncthreads();
+    const Inst& II = I;
+    const int nirr = wg_compact(I.n, [&](int v) { return II.root[v] == v && II.irr[v] != 0; }, I.aux, shi);
+    irr_a = nirr;
+    int irr_b = 0;
+    LocalRP rpf{I.rp, I.sb};
+    for (int i = 0; i < nirr; ++i) {
+      const int r = I.aux[i];
+      const int m = wg_compact(I.n, [&](int v) { return II.root[v] == r; }, I.list, shi);
+      const int go = axb_comb(I.list, m, rpf, I.clen, I.A, I.T, eps, I.nbr, I.D, F, I.xw, I.xg, I.rn, sh);
+      if (go == 0) {
+        if (threadIdx.x == 0) I.irr[r] = 2;
+        ++irr_b;
+      }
+      __syncthreads();
+    }
+    return irr_b;
+  }
+  // shift_comb_IRREGULAR_B (graph_cuda_comb.hpp:365-403) on every comb with
+  // irr == 2; type 1 shifts only the smallest violating node of each comb.
+  __device__ void shift(double sv, int type) {
+    double* F = I.F[cur];
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      I.frozen[v] = 0;
+      I.list[v] = INT_MAX;
+    }
+    __syncthreads();
+    if (type == 1) {
+      for (int v = threadIdx.x; v < I.n; v += BB) {
+        const int r = I.root[v];
+        if (r >= 0 && I.irr[r] == 2 && 4 - 2 * I.A[v] - I.T[v] != 0) atomicMin(&I.list[r], v);
+      }
+      __syncthreads();
+    }
+    for (int v = threadIdx.x; v < I.n; v += BB) {
+      const int r = I.root[v];
+      const int s = 4 - 2 * I.A[v] - I.T[v];
+      if (r >= 0 && I.irr[r] == 2 && s != 0 && (type == 0 || I.list[r] == v)) {
+        F[v] += sv * (s > 0 ? 1.0 : -1.0);
+        I.frozen[v] = 1;
+      }
+    }
+    __syncthreads();
+  }
+};
+
+template <int LPN>
+__device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
+  Inst I;
+  I.nb = a.node_off[inst];
+  I.n = (int)(a.node_off[inst + 1] - I.nb);
+  I.rp = a.rowptr + I.nb;
+  I.sb = I.rp[0];
+  I.m = I.rp[I.n] - I.sb;
+  I.nbr = a.nbr + I.sb;
+  I.D = a.D + I.sb;
+  I.orig = a.orig + I.sb;
+  I.gnbr = a.gnbr + I.sb;
+  I.gcost = a.gcost + I.sb;
+  I.F[0] = a.F0 + I.nb;
+  I.F[1] = a.F1 + I.nb;
+  I.act = a.act + I.nb;
+  I.deg = a.deg + I.nb;
+  I.clen = a.clen + I.nb;
+  I.A = a.A + I.nb;
+  I.T = a.T + I.nb;
+  I.root = a.root + I.nb;
+  I.irr = a.irr + I.nb;
+  I.aux = a.aux + I.nb;
+  I.list = a.list + I.nb;
+  I.frozen = a.frozen + I.nb;
+  I.xw = a.xw + I.sb;
+  I.xc = a.xc + I.sb;
+  I.xg = a.xg + I.sb;
+  I.rn = a.rn + I.nb;
+  I.fw = a.fw + I.sb;
+  I.fc = a.fc + I.sb;
+  const f2m_opts& o = a.o;
+  f2m_stats z;
+  memset(&z, 0, sizeof z);
+
+  // d_min: smallest positive cost (graph_tour.hpp:94)
+  double mn = 1.0 / 0.0;
+  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+    const double c = I.gcost[s];
+    if (c > 0 && c < mn) mn = c;
+  }
+  const double dmin = -block_max_all(-mn, sh);
+  if (!(dmin > 0) || isinf(dmin)) {
+    if (threadIdx.x == 0) {
+      z.status = -1;
+      a.stats[inst] = z;
+    }
+    return;
+  }
+  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+    I.nbr[s] = I.gnbr[s];
+    I.D[s] = I.gcost[s] / dmin;
+    I.orig[s] = (int32_t)s;
+  }
+  for (int v = threadIdx.x; v < I.n; v += BB) {
+    I.deg[v] = (int)(I.rp[v + 1] - I.rp[v]);
+    I.F[0][v] = 0.0;
+    I.frozen[v] = 0;
+  }
+  __syncthreads();
+  z.d_min = dmin;
+  Sched<LPN> S{I, o, sh, shi};
+  S.eps = o.err_grad * o.err_comb_mult;
+  z.glob_rounds = S.block_glob();
+  double M = 0.0;
+  if (o.stop_phase != 1) z.outer_rounds = S.run_start(M);
+  z.max_grad = M;
+  if (o.stop_phase == 1 || o.stop_phase == 2) {
+    z.sweeps = S.sweeps;
+    if (threadIdx.x == 0) {
+      a.stats[inst] = z;
+      a.cur_out[inst] = S.cur;
+    }
+    __syncthreads();
+    return;
+  }
+  int irr_b = S.combs_check(z.n_combs, z.n_irregular_a);
+  z.n_irregular_b = irr_b;
+  if (irr_b > 0 && o.repair) {
+    double shiftV = o.shift0;
+    auto attempt = [&](double sv, int type) {
+      ++z.n_repairs;
+      S.shift(sv, type);
+      for (int k = 0; k < o.frozen_sweeps; ++k) S.sweep(true);
+      for (int v = threadIdx.x; v < I.n; v += BB) I.frozen[v] = 0;
+      __syncthreads();
+      M = S.run_fix();
+      return S.combs_check(z.n_combs, z.n_irregular_a);
+    };
+    for (;;) {
+      if ((irr_b = attempt(shiftV, 0)) == 0) break;
+      if ((irr_b = attempt(shiftV, 1)) == 0) break;
+      shiftV = -shiftV;
+      if ((irr_b = attempt(shiftV, 0)) == 0) break;
+      shiftV = -shiftV;
+      if ((irr_b = attempt(shiftV, 1)) == 0) break;
+      shiftV = -shiftV;
+      shiftV *= 2;
+      if (fabs(shiftV) > o.shift_max) break;
+    }
+    z.max_grad = M;
+  }
+  z.status = irr_b == 0 ? 0 : 1;
+  // x to canonical slots, objective, dual, MAX edge rank
+  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+    const int oo = I.orig[s];
+    I.xc[oo] = I.xw[s];
+    I.fc[oo] = I.fw[s];
+  }
+  __syncthreads();
+  const double* F = I.F[S.cur];
+  double obj = 0.0, sf = 0.0, sneg = 0.0, mr = 0.0;
+  for (int v = threadIdx.x; v < I.n; v += BB) {
+    const int64_t b = I.b(v), e = I.b(v + 1);
+    const double Fv = F[v];
+    sf += Fv;
+    for (int64_t s = b; s < e; ++s) {
+      const int w = I.gnbr[s];
+      if (w > v) {
+        obj += I.xc[s] * I.gcost[s];
+        if (I.fc[s]) {
+          int r = (int)(s - b);
+          for (int64_t q = I.b(w); q < I.b(w + 1); ++q)
+            if (I.gnbr[q] == v) { r = min(r, (int)(q - I.b(w))); break; }
+          mr = fmax(mr, (double)r);
+        }
+      }
+      const int ww = I.nbr[s];
+      if (ww > v) sneg += fmin(0.0, dval(I.D[s], Fv, F[ww], v, ww));
+    }
+  }
+  obj = block_sum_all(obj, sh);
+  sf = block_sum_all(sf, sh);
+  sneg = block_sum_all(sneg, sh);
+  mr = block_max_all(mr, sh);
+  z.objective = obj;
+  z.dual = dmin * (-sf + sneg);
+  z.max_rank = (int)mr;
+  z.sweeps = S.sweeps;
+  if (threadIdx.x == 0) {
+    a.stats[inst] = z;
+    a.cur_out[inst] = S.cur;
+  }
+  __syncthreads();
+}
+
+template <int LPN>
+__global__ void __launch_bounds__(BB) b_persistent(BArgs a) {
+  __shared__ double sh[BB / 64];
+  __shared__ int shi[BB / 64 + 1];
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) shi[BB / 64] = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int idx = shi[BB / 64];
+    if (idx >= a.n_order) break;
+    const long long t0 = wall_clock64();
+    solve_instance<LPN>(a, a.order[idx], sh, shi);
+    if (a.clk && threadIdx.x == 0) a.clk[a.order[idx]] = wall_clock64() - t0;
+  }
+}
+
+__global__ void b_zero_i32(int32_t* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---- batched kNN helpers ---------------------------------------------------
+__global__ void b_localize(int64_t m, int n_inst, const int64_t* __restrict__ node_off,
+                           const int64_t* __restrict__ rowptr, int32_t* __restrict__ nbr) {
+  // nbr holds global node ids; rewrite as instance-local ids.  Slot s belongs
+  // to the node whose global id is nbr's own instance: binary search node_off.
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const int64_t w = nbr[s];
+  int lo = 0, hi = n_inst;  // largest i with node_off[i] <= w
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (node_off[mid] <= w) lo = mid; else hi = mid;
+  }
+  nbr[s] = (int32_t)(w - node_off[lo]);
+}
+
+}  // namespace
+
+// defined in f2m_knn.hip
+f2m_status knn_directed_into(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode,
+                             int32_t* out_id, double* out_c, int32_t id_offset);
+f2m_status symmetrise_i64(f2m_ctx* c, int64_t n, int K, const int32_t* knn, const double* kc, DevBuf& rpb,
+                          DevBuf& nbb, DevBuf& cb, int64_t* nslots_out);
+
+namespace {
+
+struct BatchPlan {
+  std::vector<int32_t> order;  // all instances, bucketed, decreasing size
+  std::vector<int> bucket_start, bucket_lpn;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<unsigned char> key;
+};
+
+// Lanes per node for an instance of n nodes: F2M_BATCH_LPN, else the
+// context's autotuned table (f2m_batch_autotune), else 2 below 2000 nodes
+// and 4 above.
+int batch_lpn_for(const f2m_ctx* c, int64_t n) {
+  const char* e = std::getenv("F2M_BATCH_LPN");
+  if (e) return std::atoi(e);
+  for (const auto& t : c->b_tune)
+    if (n <= t.first) return t.second;
+  return n >= 2000 ? 4 : 2;
+}
+
+}  // namespace
+
+}  // namespace f2m
+
+namespace f2m {
+void release_batch_plan(f2m_ctx* c) {
+  BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
+  if (!P) return;
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  delete P;
+  c->b_plan = nullptr;
+}
+}  // namespace f2m
+
+using namespace f2m;
+
+extern "C" {
+
+f2m_status f2m_knn_build_batched(f2m_ctx* c, int32_t n_inst, const int64_t* node_off, const double* x,
+                                 const double* y, int32_t k, int32_t cost_mode, f2m_batch* out) {
+  if (!c || n_inst < 1 || !node_off || !x || !y || !out || k < 1 || cost_mode < 0 || cost_mode > 2)
+    return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  if (k > 80) return fail(c, F2M_ERR_LIMIT, "k > 80");
+  const int64_t n_total = node_off[n_inst];
+  for (int i = 0; i < n_inst; ++i) {
+    const int64_t ni = node_off[i + 1] - node_off[i];
+    if (ni < 2 || ni - 1 < k) return fail(c, F2M_ERR_LIMIT, "every instance needs more than k points");
+    if (ni > (1 << 30)) return fail(c, F2M_ERR_LIMIT, "instance too large");
+  }
+  const double* dx = (const double*)stage_in(c, x, n_total * 8ull, 0);
+  const double* dy = (const double*)stage_in(c, y, n_total * 8ull, 1);
+  if (!dx || !dy) return F2M_ERR_HIP;
+  F2M_HIP(c, c->b_knn.reserve(n_total * k * 4ull));
+  F2M_HIP(c, c->b_kc.reserve(n_total * k * 8ull));
+  int32_t* knn = c->b_knn.as<int32_t>();
+  double* kc = c->b_kc.as<double>();
+  for (int i = 0; i < n_inst; ++i) {
+    const int64_t o = node_off[i], ni = node_off[i + 1] - o;
+    f2m_status s = knn_directed_into(c, ni, dx + o, dy + o, k, cost_mode, knn + o * k, kc + o * k, (int32_t)o);
+    if (s) return s;
+  }
+  int64_t m = 0;
+  f2m_status s = symmetrise_i64(c, n_total, k, knn, kc, c->b_rowptr, c->b_nbr, c->b_cost, &m);
+  if (s) return s;
+  F2M_HIP(c, c->b_node_off.reserve((n_inst + 1) * 8ull));
+  F2M_HIP(c, hipMemcpyAsync(c->b_node_off.p, node_off, (n_inst + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(b_localize, dim3(grid_for(m, 256)), dim3(256), 0, c->stream, m, n_inst,
+                     c->b_node_off.as<int64_t>(), c->b_rowptr.as<int64_t>(), c->b_nbr.as<int32_t>());
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  c->b_n_inst = n_inst;
+  c->b_n_total = n_total;
+  c->b_nslots = m;
+  out->n_inst = n_inst;
+  out->k = k;
+  out->cost_mode = cost_mode;
+  out->reserved = 0;
+  out->n_total = n_total;
+  out->nslots = m;
+  out->node_off = c->b_node_off.as<int64_t>();
+  out->rowptr = c->b_rowptr.as<int64_t>();
+  out->nbr = c->b_nbr.as<int32_t>();
+  out->cost = c->b_cost.as<double>();
+  return F2M_OK;
+}
+
+f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, f2m_stats* st) {
+  if (!c || !b || b->n_inst < 1) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  f2m_opts oo;
+  if (o) oo = *o; else f2m_default_opts(&oo);
+  const int64_t N = b->n_total, M = b->nslots;
+  const int n_inst = b->n_inst;
+  SolveWs& w = c->bws;
+  F2M_HIP(c, w.nbr.reserve(M * 4));
+  F2M_HIP(c, w.D.reserve(M * 8));
+  F2M_HIP(c, w.orig.reserve(M * 4));
+  F2M_HIP(c, w.xw.reserve(M * 8));
+  F2M_HIP(c, w.xc.reserve(M * 8));
+  F2M_HIP(c, w.axb.reserve(M * 8 + N * 8));
+  F2M_HIP(c, w.sortk.reserve(M * 2));
+  F2M_HIP(c, w.F0.reserve(N * 8));
+  F2M_HIP(c, w.F1.reserve(N * 8));
+  F2M_HIP(c, w.act.reserve(N * 4));
+  F2M_HIP(c, w.clen.reserve(N * 4));
+  F2M_HIP(c, w.deg.reserve(N * 4));
+  F2M_HIP(c, w.A.reserve(N * 4));
+  F2M_HIP(c, w.T.reserve(N * 4));
+  F2M_HIP(c, w.root.reserve(N * 4));
+  F2M_HIP(c, w.irr.reserve(N * 4));
+  F2M_HIP(c, w.sortv.reserve(N * 8));  // aux + list
+  F2M_HIP(c, w.frozen.reserve(N));
+  F2M_HIP(c, c->b_stats.reserve(n_inst * sizeof(f2m_stats) + 64 * 4));
+  F2M_HIP(c, c->b_order.reserve((n_inst + 64) * 4ull));
+  F2M_HIP(c, c->b_cur.reserve((n_inst + 64) * 4ull));
+
+  // host copy of node offsets -> size buckets -> queue order
+  std::vector<int64_t> off(n_inst + 1);
+  F2M_HIP(c, hipMemcpyAsync(off.data(), b->node_off, (n_inst + 1) * 8ull, hipMemcpyDefault, c->stream));
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  // key: batch identity + options; a new key rebuilds the captured graph
+  std::vector<unsigned char> key(sizeof(f2m_batch) + sizeof(f2m_opts));
+  std::memcpy(key.data(), b, sizeof(f2m_batch));
+  std::memcpy(key.data() + sizeof(f2m_batch), &oo, sizeof(f2m_opts));
+  const char* lpn_env = std::getenv("F2M_BATCH_LPN");
+  if (lpn_env) key.insert(key.end(), lpn_env, lpn_env + std::strlen(lpn_env));
+  for (const auto& t : c->b_tune) {
+    const unsigned char* q = reinterpret_cast<const unsigned char*>(&t);
+    key.insert(key.end(), q, q + sizeof t);
+  }
+  key.push_back(c->b_clk_on ? 1 : 0);
+  if (!c->b_plan) c->b_plan = new BatchPlan;
+  BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
+  const bool rebuild = P->key != key || !P->exec;
+  if (rebuild) {
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr;
+    P->graph = nullptr;
+    P->key = key;
+    std::vector<int32_t> ids(n_inst);
+    std::iota(ids.begin(), ids.end(), 0);
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b2) {
+      return off[a + 1] - off[a] > off[b2 + 1] - off[b2];
+    });
+    // buckets by lanes-per-node schedule (autotuned choice per size class)
+    P->order = ids;
+    P->bucket_start.clear();
+    P->bucket_lpn.clear();
+    int prev = -1;
+    for (int i = 0; i < n_inst; ++i) {
+      const int l = batch_lpn_for(c, off[ids[i] + 1] - off[ids[i]]);
+      if (l != prev) {
+        P->bucket_start.push_back(i);
+        P->bucket_lpn.push_back(l);
+        prev = l;
+      }
+    }
+    P->bucket_start.push_back(n_inst);
+    F2M_HIP(c, hipMemcpyAsync(c->b_order.p, P->order.data(), n_inst * 4ull, hipMemcpyHostToDevice, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  f2m_stats* dstats = c->b_stats.as<f2m_stats>();
+  int32_t* queues = reinterpret_cast<int32_t*>(dstats + n_inst);
+  int32_t* curs = c->b_cur.as<int32_t>();
+
+  BArgs a;
+  a.n_inst = n_inst;
+  a.node_off = b->node_off;
+  a.rowptr = b->rowptr;
+  a.gnbr = b->nbr;
+  a.gcost = b->cost;
+  a.nbr = w.nbr.as<int32_t>();
+  a.D = w.D.as<double>();
+  a.orig = w.orig.as<int32_t>();
+  a.F0 = w.F0.as<double>();
+  a.F1 = w.F1.as<double>();
+  a.act = w.act.as<int32_t>();
+  a.deg = w.deg.as<int32_t>();
+  a.clen = w.clen.as<int32_t>();
+  a.A = w.A.as<int32_t>();
+  a.T = w.T.as<int32_t>();
+  a.root = w.root.as<int32_t>();
+  a.irr = w.irr.as<int32_t>();
+  a.aux = w.sortv.as<int32_t>();
+  a.list = a.aux + N;
+  a.frozen = w.frozen.as<uint8_t>();
+  a.xw = w.xw.as<double>();
+  a.xc = w.xc.as<double>();
+  a.xg = w.axb.as<double>();
+  a.rn = a.xg + M;
+  a.fw = w.sortk.as<uint8_t>();
+  a.fc = a.fw + M;
+  a.o = oo;
+  a.stats = dstats;
+  a.cur_out = curs;
+  a.clk = nullptr;
+  if (c->b_clk_on) {
+    F2M_HIP(c, c->b_clk.reserve((n_inst + 64) * 8ull));
+    a.clk = c->b_clk.as<long long>();
+  }
+
+  int ncu = 0;
+  F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  auto enqueue = [&]() -> f2m_status {
+    const int nb = (int)P->bucket_lpn.size();
+    hipLaunchKernelGGL(b_zero_i32, dim3(1), dim3(64), 0, c->stream, queues, 16);
+    F2M_LAUNCH_CHECK(c);
+    for (int k = 0; k < nb && k < 16; ++k) {
+      BArgs ak = a;
+      ak.order = c->b_order.as<int32_t>() + P->bucket_start[k];
+      ak.n_order = P->bucket_start[k + 1] - P->bucket_start[k];
+      ak.queue = queues + k;
+      int per_cu = 0;
+      int grid;
+      switch (P->bucket_lpn[k]) {
+        case 1:
+          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<1>, BB, 0));
+          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
+          hipLaunchKernelGGL(b_persistent<1>, dim3(grid), dim3(BB), 0, c->stream, ak);
+          break;
+        case 2:
+          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<2>, BB, 0));
+          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
+          hipLaunchKernelGGL(b_persistent<2>, dim3(grid), dim3(BB), 0, c->stream, ak);
+          break;
+        case 8:
+          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<8>, BB, 0));
+          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
+          hipLaunchKernelGGL(b_persistent<8>, dim3(grid), dim3(BB), 0, c->stream, ak);
+          break;
+        default:
+          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<4>, BB, 0));
+          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
+          hipLaunchKernelGGL(b_persistent<4>, dim3(grid), dim3(BB), 0, c->stream, ak);
+          break;
+      }
+      F2M_LAUNCH_CHECK(c);
+    }
+    return F2M_OK;
+  };
+  const bool use_graph = std::getenv("F2M_NO_GRAPH") == nullptr;
+  if (use_graph) {
+    if (rebuild) {
+      F2M_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      f2m_status s = enqueue();
+      hipGraph_t gph = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &gph);
+      if (s) return s;
+      F2M_HIP(c, e);
+      P->graph = gph;
+      F2M_HIP(c, hipGraphInstantiate(&P->exec, gph, nullptr, nullptr, 0));
+    }
+    F2M_HIP(c, hipGraphLaunch(P->exec, c->stream));
+  } else {
+    f2m_status s = enqueue();
+    if (s) return s;
+  }
+  F2M_HIP(c, hipStreamSynchronize(c->stream));
+  if (st) F2M_HIP(c, hipMemcpy(st, dstats, n_inst * sizeof(f2m_stats), hipMemcpyDeviceToHost));
+  return F2M_OK;
+}
+
+f2m_status f2m_batch_result_copy(f2m_ctx* c, double* F, double* x, double* objective) {
+  if (!c || c->b_n_inst <= 0) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  SolveWs& w = c->bws;
+  const int n_inst = c->b_n_inst;
+  if (F) {
+    // per instance the current buffer may be F0 or F1
+    std::vector<int32_t> cur(n_inst);
+    std::vector<int64_t> off(n_inst + 1);
+    F2M_HIP(c, hipMemcpy(cur.data(), c->b_cur.p, n_inst * 4ull, hipMemcpyDeviceToHost));
+    F2M_HIP(c, hipMemcpy(off.data(), c->b_node_off.p, (n_inst + 1) * 8ull, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n_inst; ++i) {
+      const double* src = (cur[i] ? w.F1 : w.F0).as<double>() + off[i];
+      F2M_HIP(c, hipMemcpyAsync(F + off[i], src, (off[i + 1] - off[i]) * 8ull, hipMemcpyDefault, c->stream));
+    }
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  F2M_HIP(c, copy_out(c, x, w.xc.p, c->b_nslots * 8ull));
+  if (objective) {
+    std::vector<f2m_stats> st(n_inst);
+    F2M_HIP(c, hipMemcpy(st.data(), c->b_stats.p, n_inst * sizeof(f2m_stats), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n_inst; ++i) objective[i] = st[i].objective;
+  }
+  return F2M_OK;
+}
+
+// Autotuner: picks lanes per node for each instance-size class by timing
+// every candidate on the first `sample` instances of the batch (each
+// workgroup records its instance's wall_clock64 ticks) and keeping, per
+// class, the candidate with the smallest summed time.  All candidates give
+// bit-identical results; only speed differs.  The table is kept on the
+// context and, when cache_path is given, written there and reused on later
+// calls (one line per class: "<max instance size> <lanes per node>").
+f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, int32_t sample,
+                              const char* cache_path) {
+  if (!c || !b || b->n_inst < 1) return F2M_ERR_ARG;
+  static const int64_t bounds[] = {500, 1000, 2000, 4000, 8000, 16000, INT64_MAX};
+  constexpr int NC = sizeof(bounds) / sizeof(bounds[0]);
+  static const int cand[] = {1, 2, 4, 8};
+  if (cache_path) {
+    FILE* f = std::fopen(cache_path, "r");
+    if (f) {
+      std::vector<std::pair<int64_t, int32_t>> t;
+      long long mx;
+      int l;
+      while (std::fscanf(f, "%lld %d", &mx, &l) == 2) t.push_back({(int64_t)mx, (int32_t)l});
+      std::fclose(f);
+      if (!t.empty()) {
+        c->b_tune = t;
+        return F2M_OK;
+      }
+    }
+  }
+  const int ns = std::max(1, std::min<int>(sample, b->n_inst));
+  f2m_batch sub = *b;
+  sub.n_inst = ns;  // node_off / rowptr hold absolute offsets: a prefix is a batch
+  std::vector<int64_t> off(ns + 1);
+  F2M_HIP(c, hipMemcpy(off.data(), b->node_off, (ns + 1) * 8ull, hipMemcpyDefault));
+  double best_t[NC];
+  int best_l[NC];
+  for (int k = 0; k < NC; ++k) { best_t[k] = -1; best_l[k] = 0; }
+  const auto saved = c->b_tune;
+  c->b_clk_on = true;
+  std::vector<long long> clk(ns);
+  f2m_status st = F2M_OK;
+  for (int l : cand) {
+    c->b_tune = {{INT64_MAX, l}};
+    st = f2m_solve_batched(c, &sub, o, nullptr);
+    if (st) break;
+    if (hipMemcpy(clk.data(), c->b_clk.p, ns * 8ull, hipMemcpyDeviceToHost) != hipSuccess) { st = F2M_ERR_HIP; break; }
+    double tc[NC] = {0};
+    int cnt[NC] = {0};
+    for (int i = 0; i < ns; ++i) {
+      const int64_t n = off[i + 1] - off[i];
+      int k = 0;
+      while (n > bounds[k]) ++k;
+      tc[k] += (double)clk[i];
+      ++cnt[k];
+    }
+    for (int k = 0; k < NC; ++k)
+      if (cnt[k] && (best_t[k] < 0 || tc[k] < best_t[k])) { best_t[k] = tc[k]; best_l[k] = l; }
+  }
+  c->b_clk_on = false;
+  if (st) { c->b_tune = saved; return st; }
+  // classes without samples take the nearest sampled class below, else above
+  std::vector<std::pair<int64_t, int32_t>> t;
+  int last = 0;
+  for (int k = 0; k < NC; ++k) if (best_l[k]) { last = best_l[k]; break; }
+  for (int k = 0; k < NC; ++k) {
+    if (best_l[k]) last = best_l[k];
+    t.push_back({bounds[k], last ? last : 4});
+  }
+  c->b_tune = t;
+  if (cache_path) {
+    FILE* f = std::fopen(cache_path, "w");
+    if (f) {
+      for (auto& e : t) std::fprintf(f, "%lld %d\n", (long long)e.first, e.second);
+      std::fclose(f);
+    }
+  }
+  return F2M_OK;
+}
+
+f2m_status f2m_batch_copy(f2m_ctx* c, const f2m_batch* b, int64_t* node_off, int64_t* rowptr, int32_t* nbr,
+                          double* cost) {
+  if (!c || !b) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  F2M_HIP(c, copy_out(c, node_off, b->node_off, (b->n_inst + 1) * 8ull));
+  F2M_HIP(c, copy_out(c, rowptr, b->rowptr, (b->n_total + 1) * 8ull));
+  F2M_HIP(c, copy_out(c, nbr, b->nbr, b->nslots * 4ull));
+  F2M_HIP(c, copy_out(c, cost, b->cost, b->nslots * 8ull));
+  return F2M_OK;
+}
+
+}  // extern "C"

@@ -247,15 +247,17 @@ void f2m_ctx_destroy(f2m_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  f2m::release_batch_plan(c);
   f2m::DevBuf* all[] = {&c->g_rowptr, &c->g_nbr, &c->g_cost, &c->k_x, &c->k_y, &c->k_cell,
                         &c->k_cstart, &c->k_order, &c->k_sx, &c->k_sy, &c->k_knn, &c->k_kcost,
                         &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->b_node_off,
-                        &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_inst_of, &c->b_stats,
-                        &c->b_obj, &c->b_work};
+                        &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_order, &c->b_stats,
+                        &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk};
   for (auto* b : all) b->release();
   for (auto& s : c->stage) s.release();
   release_ws(c->ws);
   release_ws(c->bws);
+  for (auto e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

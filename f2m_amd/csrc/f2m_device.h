@@ -1,0 +1,218 @@
+// f2m_device.h -- device routines shared by the single-graph solve
+// (f2m_solve.hip) and the batched persistent solve (f2m_batch.hip).
+#pragma once
+#include "f2m_internal.h"
+
+namespace f2m {
+
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block reductions over blockDim.x threads; every thread gets the result.
+// sh: at least blockDim.x/64 doubles of shared memory.  Barriers inside.
+__device__ __forceinline__ double block_sum_all(double v, double* sh) {
+  v = wave_sum_d(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  return s;
+}
+__device__ __forceinline__ double block_max_all(double v, double* sh) {
+  v = wave_max_d(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = sh[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, sh[k]);
+  return s;
+}
+
+// ---- per-node steps (one thread per node), shared by both solve paths ------
+// cuda_partition_v4 (graph_cuda_partition.hpp:89-115): threshold = max d of
+// the first kk slots, partition the whole list (>= 3 below).
+__device__ inline int part_v4_node(int v, int64_t b, int dg, int kk, int32_t* __restrict__ nbr,
+                                   double* __restrict__ D, int32_t* __restrict__ orig,
+                                   const double* __restrict__ F) {
+  if (dg < kk) return dg;
+  const double Fv = F[v];
+  double thr = dval(D[b], Fv, F[nbr[b]], v, nbr[b]);
+  for (int i = 1; i < kk; ++i) {
+    const int w = nbr[b + i];
+    const double d = dval(D[b + i], Fv, F[w], v, w);
+    thr = d > thr ? d : thr;
+  }
+  return qpart_EF(v, b, dg, thr, nbr, D, orig, F);
+}
+// cuda_partition_v2 (graph_cuda_partition.hpp:57-69): lists shorter than 10
+// stay whole.
+__device__ inline int part_v2_node(int v, int64_t b, int dg, double thr, int32_t* __restrict__ nbr,
+                                   double* __restrict__ D, int32_t* __restrict__ orig,
+                                   const double* __restrict__ F) {
+  return dg < 10 ? dg : qpart_EF(v, b, dg, thr, nbr, D, orig, F);
+}
+// cuda_partition_v3 at 0.25 over the active prefix (graph_cuda_comb.hpp:301)
+// and the class counts over that prefix (graph_cuda_grad.hpp:18-39):
+// A = #{d < -eps} (x = 1), T = #{-eps <= d <= eps} (tight).
+__device__ inline void classify_node(int v, int64_t b, int act, double eps, int32_t* __restrict__ nbr,
+                                     double* __restrict__ D, int32_t* __restrict__ orig,
+                                     const double* __restrict__ F, int& cl, int& A, int& T) {
+  cl = qpart_B(v, b, act, 0.25, nbr, D, orig, F);
+  const double Fv = F[v];
+  int a = 0, t = 0;
+  for (int k = 0; k < cl; ++k) {
+    const int w = nbr[b + k];
+    const double d = dval(D[b + k], Fv, F[w], v, w);
+    if (d < -eps) ++a;
+    else if (d <= eps) ++t;
+  }
+  A = a;
+  T = t;
+}
+// x per working slot: 1 on d < -eps (node::get_eA_X, node.cpp:61-71), 1/2 on
+// the tight slots of a regular component (comb::solve_simple,
+// comb_thread.cpp:180-195), 0 elsewhere (irregular components are filled by
+// the least squares).  flag marks the slots of the reference's X map.
+__device__ inline void primal_node(int v, int64_t b, int dg, int cl, double eps, const int32_t* __restrict__ nbr,
+                                   const double* __restrict__ D, const double* __restrict__ F, bool regular,
+                                   double* __restrict__ xw, uint8_t* __restrict__ flag) {
+  const double Fv = F[v];
+  for (int k = 0; k < dg; ++k) {
+    double x = 0.0;
+    uint8_t f = 0;
+    if (k < cl) {
+      const int w = nbr[b + k];
+      const double d = dval(D[b + k], Fv, F[w], v, w);
+      if (d < -eps) { x = 1.0; f = 1; }
+      else if (d <= eps) { x = regular ? 0.5 : 0.0; f = 1; }
+    }
+    xw[b + k] = x;
+    flag[b + k] = f;
+  }
+}
+
+// Union-find with par[x] <= x: hooking always puts the larger root under the
+// smaller, so each root is the smallest id of its component.
+__device__ __forceinline__ int uf_find(int* par, int x) {
+  volatile int* vp = par;
+  int p = vp[x];
+  while (p != x) {
+    const int gp = vp[p];
+    if (gp != p) vp[x] = gp;  // path halving; any ancestor is a valid parent
+    x = p;
+    p = vp[x];
+  }
+  return x;
+}
+__device__ __forceinline__ void uf_unite(int* par, int a, int c) {
+  for (;;) {
+    a = uf_find(par, a);
+    c = uf_find(par, c);
+    if (a == c) return;
+    if (a < c) { int t = a; a = c; c = t; }
+    if (atomicCAS(&par[a], a, c) == a) return;
+  }
+}
+
+// Ax_b::solve (Ax_b.cpp:139-158) on one irregular tight component, executed
+// by the whole block.  nodes: the component's nodes in ascending id order.
+// Equation of node v: sum over its tight slots s of x_s = 2 - A(v), row
+// norm sqrt(T(v)).  Variables are tight edges; the two slots of an edge hold
+// the same value because each update uses the same expression in the
+// canonical (lo, hi) order.  x0 = 0; steepest descent with exact line
+// search; stop when max |r_j| < 1e-8; at most 999 iterations; x stored
+// rounded to float (XB is float, F2M.h:202).  Returns go_max (0 = failed).
+// rp64/base: slot offsets of node v are [rp(v), rp(v)+clen[v]).
+template <class RP>
+__device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const int32_t* __restrict__ clen,
+                        const int32_t* __restrict__ A, const int32_t* __restrict__ T, double eps,
+                        const int32_t* __restrict__ nbr, const double* __restrict__ D,
+                        const double* __restrict__ F, double* __restrict__ xw, double* __restrict__ xg,
+                        double* __restrict__ rn, double* sh) {
+  auto tight = [&](int v, int64_t s, double Fv) {
+    const int w = nbr[s];
+    const double d = dval(D[s], Fv, F[w], v, w);
+    return d >= -eps && d <= eps;
+  };
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int v = nodes[i];
+    const int64_t b = rp(v);
+    const double Fv = F[v];
+    for (int k = 0; k < clen[v]; ++k)
+      if (tight(v, b + k, Fv)) xw[b + k] = 0.0;
+  }
+  __syncthreads();
+  int go_max = 0;
+  for (int go = 1; go < 1000; ++go) {
+    double dmax = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i];
+      const int64_t b = rp(v);
+      const double Fv = F[v];
+      double s = 0.0;
+      for (int k = 0; k < clen[v]; ++k)
+        if (tight(v, b + k, Fv)) s += xw[b + k];
+      const double r = (s - (double)(2 - A[v])) / sqrt((double)T[v]);
+      rn[v] = r;
+      dmax = fmax(dmax, fabs(r));
+    }
+    dmax = block_max_all(dmax, sh);  // barriers publish rn
+    if (dmax < 1e-8) { go_max = go; break; }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i];
+      const int64_t b = rp(v);
+      const double Fv = F[v];
+      for (int k = 0; k < clen[v]; ++k) {
+        if (!tight(v, b + k, Fv)) continue;
+        const int w = nbr[b + k];
+        const int lo = v < w ? v : w, hi = v < w ? w : v;
+        xg[b + k] = rn[lo] / sqrt((double)T[lo]) + rn[hi] / sqrt((double)T[hi]);
+      }
+    }
+    __syncthreads();
+    double gg = 0.0, gd = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i];
+      const int64_t b = rp(v);
+      const double Fv = F[v];
+      double bg = 0.0;
+      for (int k = 0; k < clen[v]; ++k)
+        if (tight(v, b + k, Fv)) bg += xg[b + k];
+      gg += bg * bg;
+      gd += rn[v] * bg;
+    }
+    gg = block_sum_all(gg, sh);
+    gd = block_sum_all(gd, sh);
+    const double alfa = -gd / gg;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = nodes[i];
+      const int64_t b = rp(v);
+      const double Fv = F[v];
+      for (int k = 0; k < clen[v]; ++k)
+        if (tight(v, b + k, Fv)) xw[b + k] += xg[b + k] * alfa;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int v = nodes[i];
+    const int64_t b = rp(v);
+    const double Fv = F[v];
+    for (int k = 0; k < clen[v]; ++k)
+      if (tight(v, b + k, Fv)) xw[b + k] = (double)(float)xw[b + k];
+  }
+  __syncthreads();
+  return go_max;
+}
+
+}  // namespace f2m

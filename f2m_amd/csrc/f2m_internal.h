@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "f2m.h"
@@ -63,14 +64,22 @@ struct f2m_ctx {
   f2m::DevBuf stage[6];
   f2m::SolveWs ws;
   // batched storage
-  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_inst_of, b_stats, b_obj, b_work;
+  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc;
   f2m::SolveWs bws;
+  void* b_plan = nullptr;                          // captured launch plan (f2m_batch.hip)
+  std::vector<std::pair<int64_t, int32_t>> b_tune;  // (max instance size, lanes per node)
+  f2m::DevBuf b_clk;                               // per-instance ticks while autotuning
+  bool b_clk_on = false;
   int64_t b_n_total = 0, b_nslots = 0;
   int32_t b_n_inst = 0;
   // pinned host scalars for small read-backs
   void* pinned = nullptr;
   // last solved graph (for f2m_result_copy)
   int64_t last_n = 0, last_m = 0;
+  // instrumentation
+  int32_t prof_every = 0;
+  std::vector<hipEvent_t> prof_ev;  // pool of start/end pairs
+  f2m_profile prof{};
 };
 
 namespace f2m {
@@ -89,6 +98,9 @@ hipError_t copy_out(f2m_ctx* c, void* dst, const void* src, size_t bytes);
 // out[n] = total); total_host may be null.  tmp is scratch.
 hipError_t scan_i32_to_i32(f2m_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* total_host);
 hipError_t scan_i32_to_i64(f2m_ctx* c, const int32_t* in, int64_t* out, int64_t n, int64_t* total_host);
+
+// Destroys the captured batched-launch plan of a context (f2m_batch.hip).
+void release_batch_plan(f2m_ctx* c);
 
 inline int grid_for(int64_t threads, int block) { return (int)((threads + block - 1) / block); }
 

@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom 
                                            const int32_t* __restrict__ order,
                                            const double* __restrict__ sx, const double* __restrict__ sy,
                                            const int32_t* __restrict__ cell,
-                                           int32_t* __restrict__ out_id, double* __restrict__ out_c) {
+                                           int32_t* __restrict__ out_id, double* __restrict__ out_c,
+                                           int32_t id_offset) {
   extern __shared__ unsigned char smem[];
   double* bc = reinterpret_cast<double*>(smem);
   int32_t* bw = reinterpret_cast<int32_t*>(bc + (size_t)K * T);
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom 
     }
   }
   for (int j = 0; j < K; ++j) {
-    out_id[(int64_t)v * K + j] = bw[j * T + t];
+    out_id[(int64_t)v * K + j] = bw[j * T + t] + id_offset;
     out_c[(int64_t)v * K + j] = bc[j * T + t];
   }
 }
@@ -236,8 +237,9 @@ __global__ void k_sort_seg(int64_t n, const RP* __restrict__ rp, int32_t* __rest
   }
 }
 
-// Directed kNN of one instance into ctx->k_knn / k_kcost.
-static f2m_status knn_directed_dev(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode) {
+// Directed kNN of one instance into (out_id, out_c) [n*K]; ids + id_offset.
+f2m_status knn_directed_into(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode,
+                             int32_t* out_id, double* out_c, int32_t id_offset) {
   F2M_HIP(c, c->k_red.reserve(64));
   unsigned long long* bb = c->k_red.as<unsigned long long>();
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, bb, (int64_t)4);
@@ -272,8 +274,6 @@ static f2m_status knn_directed_dev(f2m_ctx* c, int64_t n, const double* x, const
   F2M_HIP(c, c->k_order.reserve(n * 4));
   F2M_HIP(c, c->k_sx.reserve(n * 8));
   F2M_HIP(c, c->k_sy.reserve(n * 8));
-  F2M_HIP(c, c->k_knn.reserve(n * K * 4));
-  F2M_HIP(c, c->k_kcost.reserve(n * K * 8));
   int32_t* cnt = c->k_cur.as<int32_t>();
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, cnt, ncell);
   hipLaunchKernelGGL(k_cell, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, y, n, g,
@@ -290,30 +290,33 @@ static f2m_status knn_directed_dev(f2m_ctx* c, int64_t n, const double* x, const
     lds = (size_t)K * 128 * 12;
     hipLaunchKernelGGL(k_knn<128>, dim3(grid_for(n, 128)), dim3(128), lds, c->stream, n, K, mode, g,
                        c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(), c->k_sx.as<double>(),
-                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), c->k_knn.as<int32_t>(),
-                       c->k_kcost.as<double>());
+                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), out_id, out_c, id_offset);
   } else {
     lds = (size_t)K * 64 * 12;
     hipLaunchKernelGGL(k_knn<64>, dim3(grid_for(n, 64)), dim3(64), lds, c->stream, n, K, mode, g,
                        c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(), c->k_sx.as<double>(),
-                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), c->k_knn.as<int32_t>(),
-                       c->k_kcost.as<double>());
+                       c->k_sy.as<double>(), c->k_cell.as<int32_t>(), out_id, out_c, id_offset);
   }
   F2M_LAUNCH_CHECK(c);
   return F2M_OK;
 }
 
-// Symmetrise ctx->k_knn into a CSR at (rowptr, nbr, cost); RP = int32/int64.
+static f2m_status knn_directed_dev(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode) {
+  F2M_HIP(c, c->k_knn.reserve(n * K * 4));
+  F2M_HIP(c, c->k_kcost.reserve(n * K * 8));
+  return knn_directed_into(c, n, x, y, K, mode, c->k_knn.as<int32_t>(), c->k_kcost.as<double>(), 0);
+}
+
+// Symmetrise a directed kNN (global ids) into a CSR at (rowptr, nbr, cost);
+// RP = int32/int64.
 template <class RP>
-static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, DevBuf& rpb, DevBuf& nbb, DevBuf& cb,
-                             int64_t rp_base_unused, int64_t* nslots_out) {
-  (void)rp_base_unused;
+static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, const int32_t* knn, const double* kc, DevBuf& rpb,
+                             DevBuf& nbb, DevBuf& cb, int64_t* nslots_out) {
   F2M_HIP(c, c->k_rev.reserve(n * 4));
   F2M_HIP(c, c->k_deg.reserve(n * 4));
   int32_t* rev = c->k_rev.as<int32_t>();
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
-  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K,
-                     c->k_knn.as<int32_t>(), rev);
+  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K, knn, rev);
   hipLaunchKernelGGL(k_deg, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, K, rev, c->k_deg.as<int32_t>());
   F2M_LAUNCH_CHECK(c);
   F2M_HIP(c, rpb.reserve((n + 1) * sizeof(RP)));
@@ -326,14 +329,18 @@ static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, DevBuf& rpb, DevBuf& 
   F2M_HIP(c, nbb.reserve(m * 4));
   F2M_HIP(c, cb.reserve(m * 8));
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
-  hipLaunchKernelGGL(k_fill<RP>, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K,
-                     c->k_knn.as<int32_t>(), c->k_kcost.as<double>(), (const RP*)rpb.p, rev, nbb.as<int32_t>(),
-                     cb.as<double>());
+  hipLaunchKernelGGL(k_fill<RP>, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K, knn, kc,
+                     (const RP*)rpb.p, rev, nbb.as<int32_t>(), cb.as<double>());
   hipLaunchKernelGGL(k_sort_seg<RP>, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, n, (const RP*)rpb.p,
                      nbb.as<int32_t>(), cb.as<double>());
   F2M_LAUNCH_CHECK(c);
   *nslots_out = m;
   return F2M_OK;
+}
+
+f2m_status symmetrise_i64(f2m_ctx* c, int64_t n, int K, const int32_t* knn, const double* kc, DevBuf& rpb,
+                          DevBuf& nbb, DevBuf& cb, int64_t* nslots_out) {
+  return symmetrise<int64_t>(c, n, K, knn, kc, rpb, nbb, cb, nslots_out);
 }
 
 }  // namespace f2m
@@ -370,7 +377,8 @@ f2m_status f2m_knn_build(f2m_ctx* c, int32_t n, const double* x, const double* y
   f2m_status s = knn_directed_dev(c, n, dx, dy, K, cost_mode);
   if (s) return s;
   int64_t m = 0;
-  s = symmetrise<int32_t>(c, n, K, c->g_rowptr, c->g_nbr, c->g_cost, 0, &m);
+  s = symmetrise<int32_t>(c, n, K, c->k_knn.as<int32_t>(), c->k_kcost.as<double>(), c->g_rowptr, c->g_nbr,
+                         c->g_cost, &m);
   if (s) return s;
   F2M_HIP(c, hipStreamSynchronize(c->stream));
   out->n = n;

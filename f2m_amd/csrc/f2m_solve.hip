@@ -19,13 +19,14 @@
 // LPN lanes cooperate on a node and merge their top-3 lists with xor
 // shuffles; the merge is a selection, so every LPN gives identical bits.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
-#include "f2m_internal.h"
+#include "f2m_device.h"
 
 namespace f2m {
 namespace {
@@ -124,7 +125,8 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
                                               const int32_t* __restrict__ nbr, const double* __restrict__ D,
                                               const double* __restrict__ Fin, double* __restrict__ Fout,
                                               double* __restrict__ gout, const uint8_t* __restrict__ frozen,
-                                              unsigned long long* __restrict__ maxslot) {
+                                              unsigned long long* __restrict__ maxslot,
+                                              unsigned long long* __restrict__ actsum) {
   const int gt = blockIdx.x * BS + threadIdx.x;
   const int v = gt / LPN, lane = gt % LPN;
   Top3 t;
@@ -155,6 +157,12 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
     gabs = fabs(g);
   }
   if (maxslot) block_max_to(gabs, maxslot);
+  if (actsum) {  // sum of list lengths (instrumentation: slots per sweep)
+    unsigned long long l = (ok && lane == 0) ? (unsigned long long)len[v] : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) l += (unsigned long long)__shfl_xor((long long)l, off, 64);
+    if ((threadIdx.x & 63) == 0 && l) atomicAdd(actsum, l);
+  }
 }
 
 // cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
@@ -162,18 +170,7 @@ __global__ void s_part_v4(int n, const int32_t* __restrict__ rp, const int32_t* 
                           int32_t* __restrict__ act, int kk, int32_t* __restrict__ nbr, double* __restrict__ D,
                           int32_t* __restrict__ orig, const double* __restrict__ F) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int dg = deg[v];
-  if (dg < kk) { act[v] = dg; return; }
-  const int b = rp[v];
-  const double Fv = F[v];
-  double thr = dval(D[b], Fv, F[nbr[b]], v, nbr[b]);
-  for (int i = 1; i < kk; ++i) {
-    const int w = nbr[b + i];
-    const double d = dval(D[b + i], Fv, F[w], v, w);
-    thr = d > thr ? d : thr;
-  }
-  act[v] = qpart_EF(v, b, dg, thr, nbr, D, orig, F);
+  if (v < n) act[v] = part_v4_node(v, rp[v], deg[v], kk, nbr, D, orig, F);
 }
 
 // cuda_partition_v2 (graph_cuda_partition.hpp:57-69)
@@ -181,47 +178,24 @@ __global__ void s_part_v2(int n, const int32_t* __restrict__ rp, const int32_t* 
                           int32_t* __restrict__ act, double thr, int32_t* __restrict__ nbr,
                           double* __restrict__ D, int32_t* __restrict__ orig, const double* __restrict__ F) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int dg = deg[v];
-  act[v] = dg < 10 ? dg : qpart_EF(v, rp[v], dg, thr, nbr, D, orig, F);
+  if (v < n) act[v] = part_v2_node(v, rp[v], deg[v], thr, nbr, D, orig, F);
 }
 
-// cuda_partition_v3 with threshold 0.25 over the active prefix
-// (graph_cuda_comb.hpp:301) + class counts: A = #{d < -eps} (x = 1),
-// T = #{-eps <= d <= eps} (tight) over that prefix (graph_cuda_grad.hpp:18-39).
+// cuda_partition_v3 with threshold 0.25 over the active prefix + class counts
 __global__ void s_classify(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                            int32_t* __restrict__ clen, int32_t* __restrict__ A, int32_t* __restrict__ T,
                            double eps, int32_t* __restrict__ nbr, double* __restrict__ D,
                            int32_t* __restrict__ orig, const double* __restrict__ F) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  const int b = rp[v];
-  const int cl = qpart_B(v, b, act[v], 0.25, nbr, D, orig, F);
+  int cl, a, t;
+  classify_node(v, rp[v], act[v], eps, nbr, D, orig, F, cl, a, t);
   clen[v] = cl;
-  const double Fv = F[v];
-  int a = 0, t = 0;
-  for (int k = 0; k < cl; ++k) {
-    const int w = nbr[b + k];
-    const double d = dval(D[b + k], Fv, F[w], v, w);
-    if (d < -eps) ++a;
-    else if (d <= eps) ++t;
-  }
   A[v] = a;
   T[v] = t;
 }
 
-// ---- union-find over tight edges; par[x] <= x always, so roots are minima --
-__device__ __forceinline__ int uf_find(int* par, int x) {
-  volatile int* vp = par;
-  int p = vp[x];
-  while (p != x) {
-    const int gp = vp[p];
-    if (gp != p) vp[x] = gp;  // path halving; any ancestor is a valid parent
-    x = p;
-    p = vp[x];
-  }
-  return x;
-}
+// ---- union-find over tight edges (roots are component minima) ----------
 __global__ void s_uf_init(int n, int32_t* par) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) par[v] = v;
@@ -237,15 +211,7 @@ __global__ void s_uf_union(int n, const int32_t* __restrict__ rp, const int32_t*
     const int w = nbr[b + k];
     if (w <= v) continue;
     const double d = dval(D[b + k], Fv, F[w], v, w);
-    if (!(d >= -eps && d <= eps)) continue;
-    int a = v, c = w;
-    for (;;) {
-      a = uf_find(par, a);
-      c = uf_find(par, c);
-      if (a == c) break;
-      if (a < c) { int t = a; a = c; c = t; }  // hook the larger root under the smaller
-      if (atomicCAS(&par[a], a, c) == a) break;
-    }
+    if (d >= -eps && d <= eps) uf_unite(par, v, w);
   }
 }
 __global__ void s_uf_root(int n, const int32_t* __restrict__ T, int32_t* par, int32_t* __restrict__ root) {
@@ -263,9 +229,7 @@ __global__ void s_irr_mark(int n, const int32_t* __restrict__ root, const int32_
   if (r >= 0 && 4 - 2 * A[v] - T[v] != 0) irr[r] = 1;
 }
 
-// x per working slot: 1 on d < -eps (node::get_eA_X), 1/2 on tight slots of
-// regular combs; tight slots of irregular combs are set by the least squares.
-// flag = slot is in the reference's X map (A or tight).
+// x per working slot (primal_node)
 __global__ void s_primal(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
                          const int32_t* __restrict__ clen, double eps, const int32_t* __restrict__ nbr,
                          const double* __restrict__ D, const double* __restrict__ F,
@@ -273,22 +237,8 @@ __global__ void s_primal(int n, const int32_t* __restrict__ rp, const int32_t* _
                          double* __restrict__ xw, uint8_t* __restrict__ flag) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  const int b = rp[v], cl = clen[v], dg = deg[v];
-  const double Fv = F[v];
   const int r = root[v];
-  const bool reg = r >= 0 && irr[r] == 0;
-  for (int k = 0; k < dg; ++k) {
-    double x = 0.0;
-    uint8_t f = 0;
-    if (k < cl) {
-      const int w = nbr[b + k];
-      const double d = dval(D[b + k], Fv, F[w], v, w);
-      if (d < -eps) { x = 1.0; f = 1; }
-      else if (d <= eps) { x = reg ? 0.5 : 0.0; f = 1; }
-    }
-    xw[b + k] = x;
-    flag[b + k] = f;
-  }
+  primal_node(v, rp[v], deg[v], clen[v], eps, nbr, D, F, r >= 0 && irr[r] == 0, xw, flag);
 }
 
 __global__ void s_to_canon(int64_t m, const int32_t* __restrict__ orig, const double* __restrict__ xw,
@@ -353,35 +303,11 @@ __global__ void s_seg_sort(const int32_t* __restrict__ roots, const int32_t* __r
   }
 }
 
-// Ax_b::solve (Ax_b.cpp:139-158) for one irregular comb per block.
-// Equations: nodes v of the comb (ascending), sum_{tight slots s of v} x_s
-// = 2 - A(v), row norm sqrt(T(v)).  Variables: tight edges; both slots of an
-// edge hold the same value because every update of a slot uses the same
-// expression in the canonical (lo, hi) order.  x starts at 0; steepest
-// descent with exact line search; stop when max |r_j| < 1e-8; at most 999
-// iterations.  Results rounded to float (XB is float, F2M.h:202).
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
-  return s;
-}
-__device__ __forceinline__ double block_maxd(double v, double* sh) {
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double s = sh[0];
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, sh[k]);
-  return s;
-}
-
+// least squares (axb_comb) for one irregular comb per block
+struct RP32 {
+  const int32_t* rp;
+  __device__ __forceinline__ int64_t operator()(int v) const { return rp[v]; }
+};
 __global__ void __launch_bounds__(BS) s_axb(const int32_t* __restrict__ roots, const int32_t* __restrict__ off,
                                             const int32_t* __restrict__ cnt, const int32_t* __restrict__ list,
                                             const int32_t* __restrict__ rp, const int32_t* __restrict__ clen,
@@ -392,87 +318,8 @@ __global__ void __launch_bounds__(BS) s_axb(const int32_t* __restrict__ roots, c
                                             int32_t* __restrict__ status) {
   __shared__ double sh[BS / 64];
   const int r = roots[blockIdx.x];
-  const int32_t* nodes = list + off[r];
-  const int m = cnt[r];
-  // tight slots start at 0 (x0 = 0)
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const int v = nodes[i], b = rp[v];
-    const double Fv = F[v];
-    for (int k = 0; k < clen[v]; ++k) {
-      const int w = nbr[b + k];
-      const double d = dval(D[b + k], Fv, F[w], v, w);
-      if (d >= -eps && d <= eps) xw[b + k] = 0.0;
-    }
-  }
-  __syncthreads();
-  int go_max = 0;
-  for (int go = 1; go < 1000; ++go) {
-    double dmax = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      const int v = nodes[i], b = rp[v];
-      const double Fv = F[v];
-      double s = 0.0;
-      for (int k = 0; k < clen[v]; ++k) {
-        const int w = nbr[b + k];
-        const double d = dval(D[b + k], Fv, F[w], v, w);
-        if (d >= -eps && d <= eps) s += xw[b + k];
-      }
-      const double rv = (s - (double)(2 - A[v])) / sqrt((double)T[v]);
-      rn[v] = rv;
-      dmax = fmax(dmax, fabs(rv));
-    }
-    dmax = block_maxd(dmax, sh);  // includes the barrier publishing rn
-    if (dmax < 1e-8) { go_max = go; break; }
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      const int v = nodes[i], b = rp[v];
-      const double Fv = F[v];
-      for (int k = 0; k < clen[v]; ++k) {
-        const int w = nbr[b + k];
-        const double d = dval(D[b + k], Fv, F[w], v, w);
-        if (d >= -eps && d <= eps) {
-          const int lo = v < w ? v : w, hi = v < w ? w : v;
-          xg[b + k] = rn[lo] / sqrt((double)T[lo]) + rn[hi] / sqrt((double)T[hi]);
-        }
-      }
-    }
-    __syncthreads();
-    double gg = 0.0, gd = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      const int v = nodes[i], b = rp[v];
-      const double Fv = F[v];
-      double bg = 0.0;
-      for (int k = 0; k < clen[v]; ++k) {
-        const int w = nbr[b + k];
-        const double d = dval(D[b + k], Fv, F[w], v, w);
-        if (d >= -eps && d <= eps) bg += xg[b + k];
-      }
-      gg += bg * bg;
-      gd += rn[v] * bg;
-    }
-    gg = block_sum(gg, sh);
-    gd = block_sum(gd, sh);
-    const double alfa = -gd / gg;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      const int v = nodes[i], b = rp[v];
-      const double Fv = F[v];
-      for (int k = 0; k < clen[v]; ++k) {
-        const int w = nbr[b + k];
-        const double d = dval(D[b + k], Fv, F[w], v, w);
-        if (d >= -eps && d <= eps) xw[b + k] += xg[b + k] * alfa;
-      }
-    }
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const int v = nodes[i], b = rp[v];
-    const double Fv = F[v];
-    for (int k = 0; k < clen[v]; ++k) {
-      const int w = nbr[b + k];
-      const double d = dval(D[b + k], Fv, F[w], v, w);
-      if (d >= -eps && d <= eps) xw[b + k] = (double)(float)xw[b + k];
-    }
-  }
-  if (threadIdx.x == 0) status[blockIdx.x] = go_max;
+  const int go = axb_comb(list + off[r], cnt[r], RP32{rp}, clen, A, T, eps, nbr, D, F, xw, xg, rn, sh);
+  if (threadIdx.x == 0) status[blockIdx.x] = go;
 }
 
 // shift_comb_IRREGULAR_B (graph_cuda_comb.hpp:365-403): one thread per
@@ -588,6 +435,11 @@ struct Runner {
   double dmin = 0, eps = 0;
   int64_t sweeps = 0;
   int lpn = 4;
+  // instrumentation: active slots of the current lists, totals
+  unsigned long long cur_actsum = 0;
+  int64_t active_slots = 0;
+  double alg_bytes = 0;
+  size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
   double* part = nullptr;              // reduction partials
   double* sums = nullptr;              // final sums [3]
@@ -651,34 +503,66 @@ struct Runner {
   }
 
   template <int LPN>
-  void launch_sweep_t(bool write, bool frozen, const int32_t* len, unsigned long long* mslot) {
+  void launch_sweep_t(bool write, bool frozen, const int32_t* len, unsigned long long* mslot,
+                      unsigned long long* asum) {
     dim3 grid(grid_for((int64_t)n * LPN, BS)), block(BS);
     double* fin = F();
     double* fout = Fo();
     const uint8_t* fz = w.frozen.as<uint8_t>();
     if (!write)
       hipLaunchKernelGGL((s_sweep<LPN, false, false>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, nullptr, w.g.as<double>(), fz, mslot);
+                         w.D.as<double>(), fin, nullptr, w.g.as<double>(), fz, mslot, asum);
     else if (frozen)
       hipLaunchKernelGGL((s_sweep<LPN, true, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, fout, nullptr, fz, mslot);
+                         w.D.as<double>(), fin, fout, nullptr, fz, mslot, nullptr);
     else
       hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, fout, nullptr, fz, mslot);
+                         w.D.as<double>(), fin, fout, nullptr, fz, mslot, nullptr);
   }
-  f2m_status launch_sweep(bool write, bool frozen, const int32_t* len, unsigned long long* mslot) {
+  f2m_status launch_sweep(bool write, bool frozen, const int32_t* len, unsigned long long* mslot,
+                          unsigned long long* asum = nullptr) {
+    const bool timed = write && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
+    if (timed) {
+      while (c->prof_ev.size() < 2 * (prof_used + 1)) {
+        hipEvent_t e;
+        F2M_HIP(c, hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+      }
+      F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
+    }
     switch (lpn) {
-      case 1: launch_sweep_t<1>(write, frozen, len, mslot); break;
-      case 2: launch_sweep_t<2>(write, frozen, len, mslot); break;
-      case 8: launch_sweep_t<8>(write, frozen, len, mslot); break;
-      case 16: launch_sweep_t<16>(write, frozen, len, mslot); break;
-      default: launch_sweep_t<4>(write, frozen, len, mslot); break;
+      case 1: launch_sweep_t<1>(write, frozen, len, mslot, asum); break;
+      case 2: launch_sweep_t<2>(write, frozen, len, mslot, asum); break;
+      case 8: launch_sweep_t<8>(write, frozen, len, mslot, asum); break;
+      case 16: launch_sweep_t<16>(write, frozen, len, mslot, asum); break;
+      default: launch_sweep_t<4>(write, frozen, len, mslot, asum); break;
     }
     F2M_LAUNCH_CHECK(c);
+    if (timed) {
+      F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
+      ++prof_used;
+    }
     if (write) {
       w.cur ^= 1;
       ++sweeps;
+      active_slots += cur_actsum;
+      alg_bytes += 12.0 * (double)cur_actsum + 24.0 * (double)n;
     }
+    return F2M_OK;
+  }
+  // harvest sampled sweep timings into ctx->prof (after a stream sync)
+  f2m_status harvest() {
+    double ms = 0;
+    for (size_t i = 0; i < prof_used; ++i) {
+      float t = 0;
+      F2M_HIP(c, hipEventElapsedTime(&t, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+      ms += t;
+    }
+    c->prof.sweep_launches = sweeps;
+    c->prof.sweep_samples = (int64_t)prof_used;
+    c->prof.sweep_ms_sampled = ms;
+    c->prof.sweep_alg_bytes = alg_bytes;
+    c->prof.active_slots = active_slots;
     return F2M_OK;
   }
   f2m_status read_max(int slot, double& out) {
@@ -695,10 +579,15 @@ struct Runner {
   }
   // gradient over act + max |g| (host round trip)
   f2m_status grad_max(double& M) {
+    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs, (int64_t)3, 0ull);
     f2m_status s;
-    if ((s = zero_slot(0))) return s;
-    if ((s = launch_sweep(false, false, I(w.act), maxs))) return s;
-    return read_max(0, M);
+    if ((s = launch_sweep(false, false, I(w.act), maxs, maxs + 2))) return s;
+    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+    F2M_HIP(c, hipMemcpyAsync(hp, maxs, 24, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&M, hp, 8);
+    cur_actsum = hp[2];
+    return F2M_OK;
   }
   f2m_status sweep_n(int count, bool frozen) {
     for (int i = 0; i < count; ++i) {
@@ -714,6 +603,8 @@ struct Runner {
     F2M_LAUNCH_CHECK(c);
     int go = 0;
     f2m_status s;
+    double M0;
+    if ((s = grad_max(M0))) return s;  // list lengths for the byte count; no state change
     for (; go < o.glob_rounds; ++go) {
       if ((s = sweep_n(o.glob_sweeps, false))) return s;
       hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
@@ -950,6 +841,7 @@ f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opt
   z.max_grad = M;
   if (o.stop_phase == 1 || o.stop_phase == 2) {
     F2M_HIP(c, hipStreamSynchronize(c->stream));
+    if ((s = R.harvest())) return s;
     z.sweeps = R.sweeps;
     c->last_n = g->n;
     c->last_m = g->nslots;
@@ -965,6 +857,7 @@ f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opt
   }
   z.status = irr_b == 0 ? 0 : 1;
   if ((s = R.finish(z))) return s;
+  if ((s = R.harvest())) return s;
   z.sweeps = R.sweeps;
   c->last_n = g->n;
   c->last_m = g->nslots;
@@ -982,7 +875,7 @@ template <int LPN>
 void launch_plain(hipStream_t s, int n, const int32_t* rp, const int32_t* len, const int32_t* nbr, const double* D,
                   const double* fin, double* fout, double* g, unsigned long long* mx) {
   hipLaunchKernelGGL((s_sweep<LPN, false, true>), dim3(grid_for((int64_t)n * LPN, BS)), dim3(BS), 0, s, n, rp, len,
-                     nbr, D, fin, fout, g, nullptr, mx);
+                     nbr, D, fin, fout, g, nullptr, mx, nullptr);
 }
 }  // namespace
 
@@ -993,7 +886,22 @@ f2m_status f2m_solve(f2m_ctx* c, const f2m_graph* g, const f2m_opts* o, f2m_stat
   F2M_HIP(c, hipSetDevice(c->device));
   f2m_opts oo;
   if (o) oo = *o; else f2m_default_opts(&oo);
-  return solve_graph(c, c->ws, g, oo, st);
+  auto t0 = std::chrono::steady_clock::now();
+  f2m_status s = solve_graph(c, c->ws, g, oo, st);
+  c->prof.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return s;
+}
+
+f2m_status f2m_ctx_set_profile(f2m_ctx* c, int32_t sample_every) {
+  if (!c || sample_every < 0) return F2M_ERR_ARG;
+  c->prof_every = sample_every;
+  return F2M_OK;
+}
+
+f2m_status f2m_get_profile(f2m_ctx* c, f2m_profile* out) {
+  if (!c || !out) return F2M_ERR_ARG;
+  *out = c->prof;
+  return F2M_OK;
 }
 
 f2m_status f2m_result_copy(f2m_ctx* c, double* F, double* x, int32_t* comb_root) {

@@ -79,6 +79,7 @@ class Context:
         check(lib.f2m_ctx_create(device, stream, ctypes.byref(h)))
         self.h = h
         self.graph = None
+        self.batch = None
 
     def close(self):
         if self.h:
@@ -90,6 +91,14 @@ class Context:
 
     def _check(self, st):
         check(st, self.h)
+
+    def set_profile(self, sample_every: int):
+        self._check(lib.f2m_ctx_set_profile(self.h, sample_every))
+
+    def profile(self) -> dict:
+        p = L.Profile()
+        self._check(lib.f2m_get_profile(self.h, ctypes.byref(p)))
+        return p.as_dict()
 
     def synchronize(self):
         self._check(lib.f2m_ctx_synchronize(self.h))
@@ -160,3 +169,62 @@ class Context:
         self._check(lib.f2m_sweep(self.h, n, _ptr(rowptr), _ptr(nbr), _ptr(D), _ptr(length), _ptr(F), _ptr(Fo),
                                   _ptr(g), nsweeps, ctypes.byref(mx)))
         return Fo, g, mx.value
+
+    # ---- batched form ----------------------------------------------------
+    def knn_build_batched(self, xs, ys, k: int, mode: int) -> L.Batch:
+        """xs, ys: lists of per-instance coordinate arrays (or concatenated arrays
+        with `node_off` given as xs=(x, node_off))."""
+        if isinstance(xs, tuple):
+            x, node_off = xs
+            y = ys
+        else:
+            node_off = np.zeros(len(xs) + 1, np.int64)
+            node_off[1:] = np.cumsum([len(a) for a in xs])
+            x = np.concatenate(xs)
+            y = np.concatenate(ys)
+        if not hasattr(x, "data_ptr"):
+            x = np.ascontiguousarray(x, np.float64)
+            y = np.ascontiguousarray(y, np.float64)
+        node_off = np.ascontiguousarray(node_off, np.int64)
+        b = L.Batch()
+        self._check(lib.f2m_knn_build_batched(self.h, len(node_off) - 1, _ptr(node_off), _ptr(x), _ptr(y), k, mode,
+                                              ctypes.byref(b)))
+        self.batch = b
+        self.batch_node_off = node_off
+        return b
+
+    def batch_csr(self, b: L.Batch | None = None):
+        b = b or self.batch
+        node_off = np.zeros(b.n_inst + 1, np.int64)
+        rowptr = np.zeros(b.n_total + 1, np.int64)
+        nbr = np.zeros(b.nslots, np.int32)
+        cost = np.zeros(b.nslots, np.float64)
+        self._check(lib.f2m_batch_copy(self.h, ctypes.byref(b), _ptr(node_off), _ptr(rowptr), _ptr(nbr), _ptr(cost)))
+        return node_off, rowptr, nbr, cost
+
+    def solve_batched(self, b: L.Batch | None = None, opts: L.Opts | None = None) -> list:
+        b = b or self.batch
+        st = (L.Stats * b.n_inst)()
+        self._check(lib.f2m_solve_batched(self.h, ctypes.byref(b), ctypes.byref(opts) if opts else None, st))
+        return [s.as_dict() for s in st]
+
+    def batch_autotune(self, b: L.Batch | None = None, opts: L.Opts | None = None, sample: int = 256,
+                       cache_path: str | None = None):
+        b = b or self.batch
+        self._check(lib.f2m_batch_autotune(self.h, ctypes.byref(b), ctypes.byref(opts) if opts else None, sample,
+                                           os.fsencode(cache_path) if cache_path else None))
+
+    def batch_result(self, b: L.Batch | None = None, F=None, x=None, objective=None):
+        """Copies results; pass torch tensors to keep them on the device."""
+        b = b or self.batch
+        F = np.zeros(b.n_total) if F is None else F
+        x = np.zeros(b.nslots) if x is None else x
+        objective = np.zeros(b.n_inst) if objective is None else objective
+        self._check(lib.f2m_batch_result_copy(self.h, _ptr(F), _ptr(x), _ptr(objective)))
+        return {"F": F, "x": x, "objective": objective}
+
+    def bound_backward(self, b, x_slot, px, py, grad_scale, gx, gy):
+        """Device tensors; b=None uses the context's single graph."""
+        self._check(lib.f2m_bound_backward(self.h, ctypes.byref(b) if b is not None else None, _ptr(x_slot),
+                                           _ptr(px), _ptr(py), _ptr(grad_scale), _ptr(gx), _ptr(gy)))
+

@@ -132,6 +132,24 @@ f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out);
 void f2m_ctx_destroy(f2m_ctx* ctx);
 f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
 
+/* ---- instrumentation (new; the reference prints gettimeofday deltas,
+ * graph_cuda_define.hpp:29-34) -------------------------------------------------
+ * With sample_every > 0, every sample_every-th sweep launch of f2m_solve is
+ * bracketed by HIP events on the context stream; f2m_get_profile reports the
+ * sampled average launch duration and the algorithmic bytes of all sweeps:
+ * 12 B per active slot (nbr int32 + D double) + 24 B per node (rowptr,
+ * length, F in, F out). */
+typedef struct f2m_profile {
+  int64_t sweep_launches;   /* sweeps executed by the last solve               */
+  int64_t sweep_samples;    /* launches timed with events                      */
+  double sweep_ms_sampled;  /* summed duration of the timed launches (ms)      */
+  double sweep_alg_bytes;   /* algorithmic bytes of all sweeps of the solve    */
+  int64_t active_slots;     /* sum over sweeps of the active slots processed   */
+  double solve_ms;          /* wall time of the last f2m_solve (host clock)    */
+} f2m_profile;
+f2m_status f2m_ctx_set_profile(f2m_ctx* ctx, int32_t sample_every);
+f2m_status f2m_get_profile(f2m_ctx* ctx, f2m_profile* out);
+
 /* ---- TSPLIB input (replaces get_size + FXY::get_KXY, FXY.hpp:186-309) ------
  * Reads NODE_COORD_SECTION up to EOF, rounds coordinates to 1e-3, drops exact
  * duplicate points (kept label -> dropped labels).  Host-side. */
@@ -188,6 +206,18 @@ f2m_status f2m_knn_build_batched(f2m_ctx* ctx, int32_t n_inst, const int64_t* no
 /* Solves every instance with the plain schedule, all in one persistent launch
  * per phase.  st [host] n_inst stats (may be NULL). */
 f2m_status f2m_solve_batched(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, f2m_stats* st);
+/* Autotuner (new): times lanes-per-node schedules {1,2,4,8} on the first
+ * `sample` instances (per-instance device clocks), keeps the fastest per size
+ * class (<=500, 1000, 2000, 4000, 8000, 16000, larger) for later
+ * f2m_solve_batched calls on this context.  With cache_path, an existing file
+ * is loaded instead of tuning, else the chosen table is written there.  The
+ * schedules differ in speed only: results are bit-identical. */
+f2m_status f2m_batch_autotune(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, int32_t sample,
+                              const char* cache_path);
+/* Copy a batch's arrays to caller arrays [dev|host]: node_off (n_inst+1),
+ * rowptr (n_total+1), nbr and cost (nslots); any may be NULL. */
+f2m_status f2m_batch_copy(f2m_ctx* ctx, const f2m_batch* b, int64_t* node_off, int64_t* rowptr, int32_t* nbr,
+                          double* cost);
 /* x [dev|host] nslots per canonical slot, F [dev|host] n_total. */
 f2m_status f2m_batch_result_copy(f2m_ctx* ctx, double* F, double* x, double* objective);
 

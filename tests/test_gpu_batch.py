@@ -1,0 +1,185 @@
+"""GPU tests of the batched persistent solve and the differentiable bound.
+
+Bar: every instance of a batch gives the same graph, potentials, components,
+x and schedule counts as a plain f2m_solve of that instance, bit for bit
+(objective within 1e-12 relative: the per-instance sums are reduced in a
+different order); every lanes-per-node schedule gives identical bits; graph
+replays equal the eager result.  Backward: within 1e-4 relative of the FP64
+CPU gradient and of central finite differences (eps = 1e-4) on instances
+checked to be nondegenerate (same kNN graph and same x at +-eps).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from f2m_amd import workloads as W
+from tests.conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_instances(seed=5, count=24):
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(100), np.log(3000))))
+        p = (rng.random((n, 2), dtype=np.float32) * np.float32(1e6)).astype(np.float32)
+        x, y = W.dedup(p[:, 0], p[:, 1])
+        xs.append(x)
+        ys.append(y)
+    return xs, ys
+
+
+@pytest.fixture(scope="module")
+def batch_case():
+    return mixed_instances()
+
+
+def _slices(node_off, rowptr, i):
+    a, b = node_off[i], node_off[i + 1]
+    return a, b, rowptr[a], rowptr[b]
+
+
+@pytest.mark.parametrize("mode", [W.COST_NINT, W.COST_RAW_FP64])
+def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
+    from f2m_amd.solver import Context
+    xs, ys = batch_case
+    say("batch build", len(xs), mode)
+    b = ctx.knn_build_batched(xs, ys, 16, mode)
+    node_off, rowptr, nbr, cost = ctx.batch_csr(b)
+    say("batch solve")
+    sts = ctx.solve_batched(b)
+    res = ctx.batch_result(b)
+    plain = Context(0)
+    for i in range(len(xs)):
+        a, e, sa, se = _slices(node_off, rowptr, i)
+        g = plain.knn_build(xs[i], ys[i], 16, mode)
+        rp, nb, c = plain.graph_csr(g)
+        assert np.array_equal(rowptr[a:e + 1] - sa, rp) and np.array_equal(nbr[sa:se], nb)
+        assert np.array_equal(cost[sa:se], c)
+        say("plain solve", i, len(xs[i]))
+        st = plain.solve(g)
+        r = plain.result()
+        sb = sts[i]
+        for key in ("sweeps", "glob_rounds", "outer_rounds", "n_combs", "n_irregular_a", "n_irregular_b",
+                    "status", "max_rank", "n_repairs"):
+            assert sb[key] == st[key], (i, key, sb[key], st[key])
+        assert np.array_equal(res["F"][a:e], r["F"]), i
+        if st["n_irregular_a"] == 0:
+            assert np.array_equal(res["x"][sa:se], r["x"]), i
+        else:
+            assert np.allclose(res["x"][sa:se], r["x"], atol=1e-6), i
+        assert abs(sb["objective"] - st["objective"]) <= 1e-12 * abs(st["objective"])
+        assert abs(res["objective"][i] - st["objective"]) <= 1e-12 * abs(st["objective"])
+    plain.close()
+
+
+def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch):
+    xs, ys = batch_case
+    b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
+    ref = None
+    for lpn in ("1", "2", "4", "8"):
+        monkeypatch.setenv("F2M_BATCH_LPN", lpn)
+        say("batch lpn", lpn)
+        sts = ctx.solve_batched(b)
+        res = ctx.batch_result(b)
+        if ref is None:
+            ref = (sts, res)
+        else:
+            assert sts == ref[0]
+            assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    monkeypatch.delenv("F2M_BATCH_LPN")
+    # captured graph: first call captures, later calls replay; eager work between
+    outs = []
+    for rep in range(3):
+        say("graph replay", rep)
+        sts = ctx.solve_batched(b)
+        res = ctx.batch_result(b)
+        outs.append((sts, res))
+        _ = ctx.knn_directed(xs[0], ys[0], 5, W.COST_NINT)  # unrelated eager work on the stream
+    for sts, res in outs:
+        assert sts == ref[0] and np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    monkeypatch.setenv("F2M_NO_GRAPH", "1")
+    sts = ctx.solve_batched(b)
+    assert sts == ref[0]
+
+
+def test_batch_autotune_keeps_results(ctx, batch_case, tmp_path):
+    xs, ys = batch_case
+    b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
+    ref = ctx.solve_batched(b)
+    cache = str(tmp_path / "tune.txt")
+    say("autotune")
+    ctx.batch_autotune(b, sample=len(xs), cache_path=cache)
+    lines = open(cache).read().split()
+    assert len(lines) == 14 and set(lines[1::2]) <= {"1", "2", "4", "8"}
+    assert ctx.solve_batched(b) == ref
+    ctx.batch_autotune(b, sample=4, cache_path=cache)  # loads the cache
+
+
+def _nondegenerate_fd(orc, x, y, K, u, h):
+    """Central FD of the F2M optimum (FP64 oracle, RAW_FP64 costs) at coordinate u,
+    or None if the kNN graph or x changes within +-h."""
+    vals = []
+    ref = None
+    for sgn in (+1, -1):
+        xx = x.copy()
+        xx[u] += sgn * h
+        rp, nb, c = orc.graph_build(xx, y, K, W.COST_RAW_FP64)
+        r = orc.solve(rp, nb, c)
+        key = (rp.tobytes(), nb.tobytes(), r["x"].tobytes())
+        if ref is None:
+            ref = key
+        elif key != ref or r["stats"]["n_irregular_a"]:
+            return None
+        vals.append(r["stats"]["objective"])
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+def test_bound_backward_fp64_and_fd(orc):
+    import torch
+
+    from f2m_amd.autograd import F2MBound
+    from f2m_amd.solver import Context
+    solver = Context(0, stream=torch.cuda.current_stream().cuda_stream)
+    insts, _, K = W.c5(count=4, n=400)
+    xs = [a for a, _ in insts]
+    ys = [b for _, b in insts]
+    node_off = np.zeros(len(xs) + 1, np.int64)
+    node_off[1:] = np.cumsum([len(a) for a in xs])
+    px = torch.tensor(np.concatenate(xs), dtype=torch.float32, device="cuda", requires_grad=True)
+    py = torch.tensor(np.concatenate(ys), dtype=torch.float32, device="cuda", requires_grad=True)
+    say("autograd forward")
+    V = F2MBound.apply(px, py, node_off, K, solver, None)
+    w = torch.arange(1, len(xs) + 1, dtype=torch.float64, device="cuda")
+    say("autograd backward")
+    (V * w).sum().backward()
+    gx = px.grad.cpu().numpy()
+    gy = py.grad.cpu().numpy()
+    # FP64 CPU gradient with the GPU's x on the same graph
+    node_off_b, rowptr, nbr, cost = solver.batch_csr()
+    res = solver.batch_result()
+    for i in range(len(xs)):
+        a, e = node_off[i], node_off[i + 1]
+        sa, se = rowptr[a], rowptr[e]
+        rp = (rowptr[a:e + 1] - sa).astype(np.int32)
+        ex, ey = orc.bound_backward(rp, nbr[sa:se], res["x"][sa:se], xs[i], ys[i])
+        scale = float(i + 1)
+        gref = np.concatenate([ex, ey]) * scale
+        gget = np.concatenate([gx[a:e], gy[a:e]])
+        assert np.linalg.norm(gget - gref) <= 1e-4 * np.linalg.norm(gref), i
+        # objective agrees with an FP64 oracle solve of the same RAW_FP32 graph
+        r = orc.solve(rp, nbr[sa:se], cost[sa:se])
+        assert abs(r["stats"]["objective"] - V[i].item()) <= 1e-9 * abs(r["stats"]["objective"])
+    # central finite differences on the first instance (FP64 costs), eps = 1e-4
+    x0, y0 = xs[0].astype(np.float64), ys[0].astype(np.float64)
+    checked = 0
+    for u in range(0, len(x0), 37):
+        fd = _nondegenerate_fd(orc, x0, y0, K, u, 1e-4)
+        if fd is None:
+            continue
+        checked += 1
+        assert abs(fd - gx[u]) <= 1e-4 * max(1.0, abs(fd)), (u, fd, gx[u])
+    assert checked >= 3
+    solver.close()
