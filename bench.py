@@ -11,8 +11,8 @@ as the reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).
 Prints one JSON line.  `value` = ms per solve (lower is better).
 
 roofline: the dominant kernel is the fused sweep (s_sweep).  Its algorithmic
-bytes per launch are 12 B per active slot (nbr int32 + D double) + 24 B per
-node (rowptr, list length, F in, F out); the average launch duration comes
+bytes per launch are 12 B per active slot (nbr int32 + D double) + 20 B per
+node (list offset, F in, F out); the average launch duration comes
 from HIP events the library records on its stream around every 16th sweep
 launch of the timed solves.  traffic is read from profiles/traffic.json
 (rocprofv3 FETCH_SIZE + WRITE_SIZE per sweep launch, scaled by a copy

@@ -14,6 +14,14 @@ LIB_PATH = os.path.join(_PKG, "libf2m_hip.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"f2m_amd: HIP library not built ({LIB_PATH}); run `python -m f2m_amd.build`")
 
+# One HIP runtime per process: PyTorch ships its own libamdhip64 with the same
+# soname as /opt/rocm's.  Loading torch first makes the dynamic loader bind this
+# library to torch's runtime, so tensors and f2m calls share device state.
+try:  # pragma: no cover - torch is optional
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = ctypes.CDLL(LIB_PATH)
 
 c_i32, c_i64, c_f64, c_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p

@@ -44,6 +44,7 @@ struct SolveWs {
   DevBuf red;                   // reduction scratch (doubles / max bits)
   DevBuf axb;                   // least-squares scratch
   DevBuf sortk, sortv, sortk2, sortv2, sorttmp;  // comb node sort
+  DevBuf ef, efoff;             // compacted active lists (16-byte slots) + offsets
   int cur = 0;                  // which F buffer holds the current potentials
   int64_t n = 0, m = 0;
 };

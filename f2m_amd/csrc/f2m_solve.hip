@@ -115,29 +115,48 @@ __global__ void s_act_min(int n, const int32_t* __restrict__ deg, int32_t* __res
   if (v < n) act[v] = deg[v] > k ? k : deg[v];
 }
 
-// Fused sweep / gradient.  WRITE_F=false computes g and max|g| only
-// (cuda_get_Nodes_grad_R + max_node_grad); WRITE_F=true also applies
-// F' = F + g (cuda_update_Nodes_F[_BLOCK]); the reduced costs of the next
-// sweep are recomputed from F' (cuda_recompute_Edges).
+// Active-list slot: the reference's EF array (graph_cuda.cu:229-252,
+// get_cuda_EF graph_cuda_grad.hpp:96-111) holds pointers to the active edges;
+// here the (D, nbr) pair itself is copied, 16 bytes, one vector load.
+struct ESlot {
+  double D;
+  int32_t nbr;
+  int32_t pad;
+};
+
+// Compact the first act[v] working slots of every node into ef[efoff[v]..].
+__global__ void s_ef_fill(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
+                          const int32_t* __restrict__ efoff, const int32_t* __restrict__ nbr,
+                          const double* __restrict__ D, ESlot* __restrict__ ef) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v], o = efoff[v], L = act[v];
+  for (int k = 0; k < L; ++k) ef[o + k] = ESlot{D[b + k], nbr[b + k], 0};
+}
+
+// Fused sweep / gradient over the compacted active lists.  WRITE_F=false
+// computes g and max|g| only (cuda_get_Nodes_grad_R + max_node_grad);
+// WRITE_F=true also applies F' = F + g (cuda_update_Nodes_F[_BLOCK]); the
+// reduced costs of the next sweep are recomputed from F' (cuda_recompute_Edges).
+// The block max goes to one of 32 slots 128 bytes apart (blockIdx % 32) so the
+// atomics of different blocks do not queue on one address.
 template <int LPN, bool FROZEN, bool WRITE_F>
-__global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__ rp,
-                                              const int32_t* __restrict__ len,
-                                              const int32_t* __restrict__ nbr, const double* __restrict__ D,
-                                              const double* __restrict__ Fin, double* __restrict__ Fout,
-                                              double* __restrict__ gout, const uint8_t* __restrict__ frozen,
-                                              unsigned long long* __restrict__ maxslot,
-                                              unsigned long long* __restrict__ actsum) {
+__global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__ efoff,
+                                              const ESlot* __restrict__ ef, const double* __restrict__ Fin,
+                                              double* __restrict__ Fout, double* __restrict__ gout,
+                                              const uint8_t* __restrict__ frozen,
+                                              unsigned long long* __restrict__ maxslots) {
   const int gt = blockIdx.x * BS + threadIdx.x;
   const int v = gt / LPN, lane = gt % LPN;
   Top3 t;
   double Fv = 0.0;
   const bool ok = v < n;
   if (ok) {
-    const int b = rp[v], L = len[v];
+    const int b = efoff[v], e = efoff[v + 1];
     Fv = Fin[v];
-    for (int k = lane; k < L; k += LPN) {
-      const int w = nbr[b + k];
-      t.push(dval(D[b + k], Fv, Fin[w], v, w));
+    for (int k = b + lane; k < e; k += LPN) {
+      const ESlot sl = ef[k];
+      t.push(dval(sl.D, Fv, Fin[sl.nbr], v, sl.nbr));
     }
   }
 #pragma unroll
@@ -156,13 +175,7 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
     if (WRITE_F) Fout[v] = (FROZEN && frozen[v]) ? Fv : Fv + g;
     gabs = fabs(g);
   }
-  if (maxslot) block_max_to(gabs, maxslot);
-  if (actsum) {  // sum of list lengths (instrumentation: slots per sweep)
-    unsigned long long l = (ok && lane == 0) ? (unsigned long long)len[v] : 0ull;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) l += (unsigned long long)__shfl_xor((long long)l, off, 64);
-    if ((threadIdx.x & 63) == 0 && l) atomicAdd(actsum, l);
-  }
+  if (maxslots) block_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
 }
 
 // cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
@@ -423,6 +436,10 @@ int env_int(const char* name, int dflt) {
   return s ? std::atoi(s) : dflt;
 }
 
+// device max slots: [0, 512) gradient max (32 slots x 16), [512, 1024)
+// early-exit max of the last sweep of a chunk
+constexpr int MAXS = 1024;
+
 struct Runner {
   f2m_ctx* c;
   SolveWs& w;
@@ -472,16 +489,18 @@ struct Runner {
     F2M_HIP(c, w.frozen.reserve(n));
     F2M_HIP(c, w.sortv.reserve((int64_t)(n + 1) * 4 * 6));  // cnt, off, cur, ridx, roots, list
     F2M_HIP(c, w.axb.reserve(m * 8 + (int64_t)n * 8 + (int64_t)(n + 1) * 4));
-    F2M_HIP(c, w.red.reserve(64 * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
+    F2M_HIP(c, w.ef.reserve(m * 16));
+    F2M_HIP(c, w.efoff.reserve((n + 1) * 4));
+    F2M_HIP(c, w.red.reserve(MAXS * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
     maxs = w.red.as<unsigned long long>();
-    part = reinterpret_cast<double*>(maxs + 64);
+    part = reinterpret_cast<double*>(maxs + MAXS);
     sums = part + 3 * RED_BLOCKS;
     ibuf = reinterpret_cast<int32_t*>(sums + 8);
     return F2M_OK;
   }
 
   f2m_status setup() {
-    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs, (int64_t)64, INF_BITS);
+    hipLaunchKernelGGL(s_zero_u64, dim3(4), dim3(256), 0, c->stream, maxs, (int64_t)MAXS, INF_BITS);
     int nb = grid_for(m, BS);
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(s_min_pos, dim3(nb), dim3(BS), 0, c->stream, gcost, m, maxs);
@@ -503,24 +522,24 @@ struct Runner {
   }
 
   template <int LPN>
-  void launch_sweep_t(bool write, bool frozen, const int32_t* len, unsigned long long* mslot,
-                      unsigned long long* asum) {
+  void launch_sweep_t(bool write, bool frozen, unsigned long long* mslots) {
     dim3 grid(grid_for((int64_t)n * LPN, BS)), block(BS);
     double* fin = F();
     double* fout = Fo();
     const uint8_t* fz = w.frozen.as<uint8_t>();
+    const int32_t* eo = I(w.efoff);
+    const ESlot* ef = w.ef.as<ESlot>();
     if (!write)
-      hipLaunchKernelGGL((s_sweep<LPN, false, false>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, nullptr, w.g.as<double>(), fz, mslot, asum);
+      hipLaunchKernelGGL((s_sweep<LPN, false, false>), grid, block, 0, c->stream, n, eo, ef, fin, nullptr,
+                         w.g.as<double>(), fz, mslots);
     else if (frozen)
-      hipLaunchKernelGGL((s_sweep<LPN, true, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, fout, nullptr, fz, mslot, nullptr);
+      hipLaunchKernelGGL((s_sweep<LPN, true, true>), grid, block, 0, c->stream, n, eo, ef, fin, fout, nullptr, fz,
+                         mslots);
     else
-      hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, rp, len, I(w.nbr),
-                         w.D.as<double>(), fin, fout, nullptr, fz, mslot, nullptr);
+      hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, eo, ef, fin, fout, nullptr, fz,
+                         mslots);
   }
-  f2m_status launch_sweep(bool write, bool frozen, const int32_t* len, unsigned long long* mslot,
-                          unsigned long long* asum = nullptr) {
+  f2m_status launch_sweep(bool write, bool frozen, unsigned long long* mslots) {
     const bool timed = write && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
     if (timed) {
       while (c->prof_ev.size() < 2 * (prof_used + 1)) {
@@ -531,11 +550,11 @@ struct Runner {
       F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
     }
     switch (lpn) {
-      case 1: launch_sweep_t<1>(write, frozen, len, mslot, asum); break;
-      case 2: launch_sweep_t<2>(write, frozen, len, mslot, asum); break;
-      case 8: launch_sweep_t<8>(write, frozen, len, mslot, asum); break;
-      case 16: launch_sweep_t<16>(write, frozen, len, mslot, asum); break;
-      default: launch_sweep_t<4>(write, frozen, len, mslot, asum); break;
+      case 1: launch_sweep_t<1>(write, frozen, mslots); break;
+      case 2: launch_sweep_t<2>(write, frozen, mslots); break;
+      case 8: launch_sweep_t<8>(write, frozen, mslots); break;
+      case 16: launch_sweep_t<16>(write, frozen, mslots); break;
+      default: launch_sweep_t<4>(write, frozen, mslots); break;
     }
     F2M_LAUNCH_CHECK(c);
     if (timed) {
@@ -546,7 +565,7 @@ struct Runner {
       w.cur ^= 1;
       ++sweeps;
       active_slots += cur_actsum;
-      alg_bytes += 12.0 * (double)cur_actsum + 24.0 * (double)n;
+      alg_bytes += 12.0 * (double)cur_actsum + 20.0 * (double)n;
     }
     return F2M_OK;
   }
@@ -565,33 +584,46 @@ struct Runner {
     c->prof.active_slots = active_slots;
     return F2M_OK;
   }
-  f2m_status read_max(int slot, double& out) {
-    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
-    F2M_HIP(c, hipMemcpyAsync(hp, maxs + slot, 8, hipMemcpyDeviceToHost, c->stream));
-    F2M_HIP(c, hipStreamSynchronize(c->stream));
-    std::memcpy(&out, hp, 8);
+  // update_EF (graph_cuda.cu:229-252): offsets = scan(act), copy the active
+  // prefixes.  With profiling on, the total is read back for the byte count.
+  f2m_status ef_rebuild() {
+    F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
+    hipLaunchKernelGGL(s_ef_fill, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.act), I(w.efoff),
+                       I(w.nbr), w.D.as<double>(), w.ef.as<ESlot>());
+    F2M_LAUNCH_CHECK(c);
+    if (c->prof_every > 0) {
+      int32_t* hp = static_cast<int32_t*>(c->pinned);
+      F2M_HIP(c, hipMemcpyAsync(hp, I(w.efoff) + n, 4, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipStreamSynchronize(c->stream));
+      cur_actsum = (unsigned long long)hp[0];
+    }
     return F2M_OK;
   }
-  f2m_status zero_slot(int slot) {
-    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs + slot, (int64_t)1, 0ull);
+  // max over the 32 strided slots of region r (0: gradient, 1: early exit)
+  f2m_status read_max(int r, double& out) {
+    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+    F2M_HIP(c, hipMemcpyAsync(hp, maxs + 512 * r, 512 * 8, hipMemcpyDeviceToHost, c->stream));
+    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    unsigned long long b = 0;
+    for (int k = 0; k < 32; ++k) b = hp[16 * k] > b ? hp[16 * k] : b;
+    std::memcpy(&out, &b, 8);
+    return F2M_OK;
+  }
+  f2m_status zero_region(int r) {
+    hipLaunchKernelGGL(s_zero_u64, dim3(2), dim3(256), 0, c->stream, maxs + 512 * r, (int64_t)512, 0ull);
     F2M_LAUNCH_CHECK(c);
     return F2M_OK;
   }
-  // gradient over act + max |g| (host round trip)
+  // gradient over the active lists + max |g| (host round trip)
   f2m_status grad_max(double& M) {
-    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, maxs, (int64_t)3, 0ull);
     f2m_status s;
-    if ((s = launch_sweep(false, false, I(w.act), maxs, maxs + 2))) return s;
-    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
-    F2M_HIP(c, hipMemcpyAsync(hp, maxs, 24, hipMemcpyDeviceToHost, c->stream));
-    F2M_HIP(c, hipStreamSynchronize(c->stream));
-    std::memcpy(&M, hp, 8);
-    cur_actsum = hp[2];
-    return F2M_OK;
+    if ((s = zero_region(0))) return s;
+    if ((s = launch_sweep(false, false, maxs))) return s;
+    return read_max(0, M);
   }
   f2m_status sweep_n(int count, bool frozen) {
     for (int i = 0; i < count; ++i) {
-      f2m_status s = launch_sweep(true, frozen, I(w.act), nullptr);
+      f2m_status s = launch_sweep(true, frozen, nullptr);
       if (s) return s;
     }
     return F2M_OK;
@@ -603,13 +635,13 @@ struct Runner {
     F2M_LAUNCH_CHECK(c);
     int go = 0;
     f2m_status s;
-    double M0;
-    if ((s = grad_max(M0))) return s;  // list lengths for the byte count; no state change
+    if ((s = ef_rebuild())) return s;
     for (; go < o.glob_rounds; ++go) {
       if ((s = sweep_n(o.glob_sweeps, false))) return s;
       hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
                          o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
       F2M_LAUNCH_CHECK(c);
+      if ((s = ef_rebuild())) return s;
       double M;
       if ((s = grad_max(M))) return s;
       if (M < o.glob_exit) { ++go; break; }
@@ -622,6 +654,8 @@ struct Runner {
     hipLaunchKernelGGL(s_part_v2, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act), 1.0,
                        I(w.nbr), w.D.as<double>(), I(w.orig), F());
     F2M_LAUNCH_CHECK(c);
+    f2m_status s = ef_rebuild();
+    if (s) return s;
     return grad_max(M);
   }
   // block_EF (graph_cuda.cu:308-338)
@@ -638,14 +672,14 @@ struct Runner {
       int chunk = std::min(o.check_every, N - done);
       if ((s = sweep_n(chunk - 1, frozen))) return s;
       if (chunk == o.check_every) {
-        if ((s = zero_slot(1))) return s;
-        if ((s = launch_sweep(true, frozen, I(w.act), maxs + 1))) return s;
+        if ((s = zero_region(1))) return s;
+        if ((s = launch_sweep(true, frozen, maxs + 512))) return s;
         double MB;
         if ((s = read_max(1, MB))) return s;
         done += chunk;
         if (MB < o.err_grad) break;
       } else {
-        if ((s = launch_sweep(true, frozen, I(w.act), nullptr))) return s;
+        if ((s = launch_sweep(true, frozen, nullptr))) return s;
         done += chunk;
       }
     }
@@ -869,13 +903,14 @@ f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opt
 
 using namespace f2m;
 
-// Test entry: nsweeps Jacobi sweeps on caller arrays.
+// Test entry: nsweeps Jacobi sweeps on caller arrays, through the same
+// compaction (s_ef_fill) and sweep kernel as f2m_solve.
 namespace {
 template <int LPN>
-void launch_plain(hipStream_t s, int n, const int32_t* rp, const int32_t* len, const int32_t* nbr, const double* D,
-                  const double* fin, double* fout, double* g, unsigned long long* mx) {
-  hipLaunchKernelGGL((s_sweep<LPN, false, true>), dim3(grid_for((int64_t)n * LPN, BS)), dim3(BS), 0, s, n, rp, len,
-                     nbr, D, fin, fout, g, nullptr, mx, nullptr);
+void launch_plain(hipStream_t s, int n, const int32_t* efoff, const ESlot* ef, const double* fin, double* fout,
+                  double* g, unsigned long long* mx) {
+  hipLaunchKernelGGL((s_sweep<LPN, false, true>), dim3(grid_for((int64_t)n * LPN, BS)), dim3(BS), 0, s, n, efoff,
+                     ef, fin, fout, g, nullptr, mx);
 }
 }  // namespace
 
@@ -934,20 +969,28 @@ f2m_status f2m_sweep(f2m_ctx* c, int32_t n, const int32_t* rowptr, const int32_t
   F2M_HIP(c, w.red.reserve(4096));
   F2M_HIP(c, hipMemcpyAsync(w.F0.p, dF, n * 8ull, hipMemcpyDeviceToDevice, c->stream));
   unsigned long long* mx = w.red.as<unsigned long long>();
+  F2M_HIP(c, w.efoff.reserve((n + 1) * 4ull));
+  F2M_HIP(c, w.ef.reserve(m * 16ull + 16));
+  F2M_HIP(c, scan_i32_to_i32(c, dl, w.efoff.as<int32_t>(), n, nullptr));
+  hipLaunchKernelGGL(s_ef_fill, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, drp, dl, w.efoff.as<int32_t>(), dnb,
+                     dD, w.ef.as<ESlot>());
+  F2M_LAUNCH_CHECK(c);
+  const int32_t* eo = w.efoff.as<int32_t>();
+  const ESlot* ef = w.ef.as<ESlot>();
   int lpn = env_int("F2M_LPN", 4);
   int cur = 0;
   for (int i = 0; i < nsweeps; ++i) {
     double* fin = cur ? w.F1.as<double>() : w.F0.as<double>();
     double* fout = cur ? w.F0.as<double>() : w.F1.as<double>();
     unsigned long long* ms = (i == nsweeps - 1) ? mx : nullptr;
-    if (ms) hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, mx, (int64_t)1, 0ull);
+    if (ms) hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(512), 0, c->stream, mx, (int64_t)512, 0ull);
     switch (lpn) {
-      case 1: launch_plain<1>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
-      case 2: launch_plain<2>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
-      case 8: launch_plain<8>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
-      case 16: launch_plain<16>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
-      case 64: launch_plain<64>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
-      default: launch_plain<4>(c->stream, n, drp, dl, dnb, dD, fin, fout, w.g.as<double>(), ms); break;
+      case 1: launch_plain<1>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
+      case 2: launch_plain<2>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
+      case 8: launch_plain<8>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
+      case 16: launch_plain<16>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
+      case 64: launch_plain<64>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
+      default: launch_plain<4>(c->stream, n, eo, ef, fin, fout, w.g.as<double>(), ms); break;
     }
     F2M_LAUNCH_CHECK(c);
     cur ^= 1;
@@ -955,8 +998,10 @@ f2m_status f2m_sweep(f2m_ctx* c, int32_t n, const int32_t* rowptr, const int32_t
   F2M_HIP(c, copy_out(c, F_out, cur ? w.F1.p : w.F0.p, n * 8ull));
   if (g_out) F2M_HIP(c, copy_out(c, g_out, w.g.p, n * 8ull));
   if (max_abs_g) {
+    unsigned long long h[512];
+    F2M_HIP(c, copy_out(c, h, mx, sizeof h));
     unsigned long long b = 0;
-    F2M_HIP(c, copy_out(c, &b, mx, 8));
+    for (int k = 0; k < 32; ++k) b = h[16 * k] > b ? h[16 * k] : b;
     std::memcpy(max_abs_g, &b, 8);
   }
   return F2M_OK;
