@@ -194,6 +194,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     }
     gg = block_sum_all(gg, sh);
     gd = block_sum_all(gd, sh);
+    if (gg == 0.0) break;  // the reference would take 0/0 = NaN here; treated as no convergence
     const double alfa = -gd / gg;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
       const int v = nodes[i];

@@ -620,6 +620,11 @@ int orc_axb(int m, const int32_t* row_ptr, const int32_t* col, const int32_t* b,
     }
     double gg = 0, gd = 0;
     for (int j = 0; j < m; ++j) { gg += bg[j] * bg[j]; gd += dj[j] * bg[j]; }
+    // Deviation: the reference divides 0/0 here (an inconsistent system at its
+    // least-squares point), fills x with NaN and then reports convergence
+    // because fabs(NaN) > d_max is false.  Here gg == 0 ends the solve as
+    // not converged.
+    if (gg == 0) break;
     double alfa = -gd / gg;
     for (int i = 0; i < nv; ++i) x[i] += xg[i] * alfa;
   }
