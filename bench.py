@@ -20,6 +20,16 @@ kernel of known size) when present.
 
 cpu_baseline: the oracle (the CPU restatement of the reference schedule,
 oracle/f2m_oracle.cpp) solving the same graph with the host's threads.
+
+batched: the 4096-instance workload (n log-uniform in [100, 20000], fp32
+points x 1e6, NINT costs, K=16) solved by one replay of the captured
+persistent launch (f2m_solve_batched), against a loop of plain GPU solves
+timed on 16 instances spread over the size range (per-instance throughput,
+identical optima checked on those 16).
+
+fwd_bwd: the differentiable bound on 256 instances of n=2000 (fp32 points
+in the unit square, RAW_FP32 costs, K=20): forward (kNN + batched solve) and
+backward (f2m_bound_backward through torch.autograd).
 """
 from __future__ import annotations
 
@@ -47,6 +57,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-every", type=int, default=16)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-fwd-bwd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -111,6 +123,9 @@ def main():
                          f"objective agrees: {ok}",
                "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1)}
 
+    batched = None if args.no_batched else bench_batched(ctx)
+    fwd_bwd = None if args.no_fwd_bwd else bench_fwd_bwd()
+
     out = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
@@ -119,10 +134,87 @@ def main():
                    "n": n, "k": K, "nslots": int(g.nslots), "sweeps": int(st["sweeps"]),
                    "schedule": "reference", "parallelism": "single GPU"},
         "solves_per_s": round(1e3 / ms, 3), "objective": st["objective"], "knn_build_ms": round(knn_ms, 3),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "fwd_bwd": fwd_bwd,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_batched(ctx):
+    import numpy as np
+
+    from f2m_amd import workloads as W
+    log("batched: generating 4096 instances")
+    insts, mode, K = W.c4()
+    xs = [a for a, _ in insts]
+    ys = [b for _, b in insts]
+    t = time.perf_counter()
+    b = ctx.knn_build_batched(xs, ys, K, mode)
+    knn_s = time.perf_counter() - t
+    t = time.perf_counter()
+    st = ctx.solve_batched(b)  # first call captures the hipGraph
+    first_s = time.perf_counter() - t
+    t = time.perf_counter()
+    st = ctx.solve_batched(b)  # replay
+    batched_s = time.perf_counter() - t
+    log("batched solve", batched_s, "s (capture call", first_s, "s)")
+    idx = np.linspace(0, len(xs) - 1, 16).astype(int)
+    loop_s = 0.0
+    same = True
+    for i in idx:
+        g = ctx.knn_build(xs[i], ys[i], K, mode)
+        ctx.solve(g)
+        t = time.perf_counter()
+        s2 = ctx.solve(g)
+        loop_s += time.perf_counter() - t
+        same &= s2["sweeps"] == st[i]["sweeps"] and abs(s2["objective"] - st[i]["objective"]) <= 1e-12 * abs(
+            s2["objective"])
+    plain_ms = loop_s * 1e3 / len(idx)
+    per_inst_ms = batched_s * 1e3 / len(xs)
+    return {"workload": "4096 instances, n log-uniform in [100,20000], fp32 points x1e6, NINT costs, K=16",
+            "instances": len(xs), "sum_n": int(sum(len(a) for a in xs)), "nslots": int(b.nslots),
+            "ms": round(batched_s * 1e3, 1), "per_instance_ms": round(per_inst_ms, 4),
+            "plain_loop_per_instance_ms": round(plain_ms, 3),
+            "speedup_per_instance": round(plain_ms / per_inst_ms, 2), "identical_optima_sampled": bool(same),
+            "all_optimal_found": all(x["status"] == 0 for x in st), "knn_build_s": round(knn_s, 3),
+            "sweeps_max": int(max(x["sweeps"] for x in st))}
+
+
+def bench_fwd_bwd():
+    import numpy as np
+    import torch
+
+    from f2m_amd import workloads as W
+    from f2m_amd.autograd import F2MBound
+    from f2m_amd.solver import Context
+    insts, mode, K = W.c5()
+    xs = [a for a, _ in insts]
+    ys = [b for _, b in insts]
+    node_off = np.zeros(len(xs) + 1, np.int64)
+    node_off[1:] = np.cumsum([len(a) for a in xs])
+    solver = Context(0, stream=torch.cuda.current_stream().cuda_stream)
+    px = torch.tensor(np.concatenate(xs), dtype=torch.float32, device="cuda", requires_grad=True)
+    py = torch.tensor(np.concatenate(ys), dtype=torch.float32, device="cuda", requires_grad=True)
+    best = None
+    for rep in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        V = F2MBound.apply(px, py, node_off, K, solver, None)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        V.sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        px.grad = None
+        py.grad = None
+        if rep and (best is None or t2 - t0 < best[0] + best[1]):
+            best = (t1 - t0, t2 - t1)
+    log("fwd+bwd", best)
+    solver.close()
+    return {"workload": "256 instances of n=2000, fp32 points in [0,1)^2, RAW_FP32 costs, K=20",
+            "fwd_ms": round(best[0] * 1e3, 2), "bwd_ms": round(best[1] * 1e3, 3),
+            "fwd_bwd_ms": round((best[0] + best[1]) * 1e3, 2),
+            "fwd_bwd_over_fwd": round((best[0] + best[1]) / best[0], 4)}
 
 
 if __name__ == "__main__":

@@ -29,7 +29,6 @@
 namespace f2m {
 namespace {
 
-constexpr int BB = 256;  // threads per workgroup (one instance at a time)
 
 struct BArgs {
   int n_inst;
@@ -46,6 +45,8 @@ struct BArgs {
   uint8_t* frozen;
   double *xw, *xc, *xg, *rn;
   uint8_t *fw, *fc;
+  int32_t* efoff;  // per instance n+1 entries at node_off[i] + i
+  ESlot* ef;       // compacted active lists, per instance at its slot base
   const int32_t* order;  // instance ids of this bucket, decreasing size
   int32_t n_order;
   int32_t* queue;
@@ -70,6 +71,8 @@ struct Inst {
   uint8_t* frozen;
   double *xw, *xc, *xg, *rn;
   uint8_t *fw, *fc;
+  int32_t* efoff;
+  ESlot* ef;
   __device__ __forceinline__ int64_t b(int v) const { return rp[v] - sb; }
 };
 
@@ -79,28 +82,27 @@ struct LocalRP {
   __device__ __forceinline__ int64_t operator()(int v) const { return rp[v] - sb; }
 };
 
-// One pass over all nodes: gradient over the first len[v] slots from F[cur];
-// if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns max |g|
-// to every thread (barriers inside).
-template <int LPN>
-__device__ double wg_pass(const Inst& I, const int32_t* len, int cur, bool write, bool use_frozen, double* sh) {
+// One pass over all nodes: gradient over the compacted active lists from
+// F[cur]; if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns
+// max |g| to every thread (barriers inside).
+template <int BS, int LPN>
+__device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, double* sh) {
   const double* Fin = I.F[cur];
   double* Fout = I.F[cur ^ 1];
   double gmax = 0.0;
   const int total = I.n * LPN;
-  for (int base = 0; base < total; base += BB) {
+  for (int base = 0; base < total; base += BS) {
     const int gt = base + threadIdx.x;
     const int v = gt / LPN, lane = gt % LPN;
     Top3 t;
     double Fv = 0.0;
     const bool ok = v < I.n;
     if (ok) {
-      const int64_t b = I.b(v);
-      const int L = len[v];
+      const int b = I.efoff[v], e = I.efoff[v + 1];
       Fv = Fin[v];
-      for (int k = lane; k < L; k += LPN) {
-        const int w = I.nbr[b + k];
-        t.push(dval(I.D[b + k], Fv, Fin[w], v, w));
+      for (int k = b + lane; k < e; k += LPN) {
+        const ESlot sl = I.ef[k];
+        t.push(dval(sl.D, Fv, Fin[sl.nbr], v, sl.nbr));
       }
     }
 #pragma unroll
@@ -121,12 +123,48 @@ __device__ double wg_pass(const Inst& I, const int32_t* len, int cur, bool write
   return block_max_all(gmax, sh);
 }
 
+// update_EF (graph_cuda.cu:229-252) inside the workgroup: efoff = exclusive
+// scan of act, then copy each node's active prefix into ef.
+template <int BS>
+__device__ void wg_ef_rebuild(const Inst& I, int* shi) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < I.n; base += BS) {
+    const int v = base + threadIdx.x;
+    const int a = v < I.n ? I.act[v] : 0;
+    int inc = a;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    __syncthreads();
+    if (lane == 63) shi[wid] = inc;
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int k = 0; k < BS / 64; ++k) {
+      if (k < wid) pre += shi[k];
+      tot += shi[k];
+    }
+    if (v < I.n) I.efoff[v] = carry + pre + inc - a;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) I.efoff[I.n] = carry;
+  __syncthreads();
+  for (int v = threadIdx.x; v < I.n; v += BS) {
+    const int64_t b = I.b(v);
+    const int o = I.efoff[v], L = I.act[v];
+    for (int k = 0; k < L; ++k) I.ef[o + k] = ESlot{I.D[b + k], I.nbr[b + k], 0};
+  }
+  __syncthreads();
+}
+
 // Ascending ids v in [0,n) with pred(v) into out; returns the count.
-template <class P>
+template <int BS, class P>
 __device__ int wg_compact(int n, P pred, int32_t* out, int* shi) {
   int total = 0;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < n; base += BB) {
+  for (int base = 0; base < n; base += BS) {
     const int v = base + threadIdx.x;
     const int f = (v < n && pred(v)) ? 1 : 0;
     const unsigned long long bal = __ballot(f);
@@ -135,7 +173,7 @@ __device__ int wg_compact(int n, P pred, int32_t* out, int* shi) {
     if (lane == 0) shi[wid] = __popcll(bal);
     __syncthreads();
     int off = 0, tot = 0;
-    for (int k = 0; k < BB / 64; ++k) {
+    for (int k = 0; k < BS / 64; ++k) {
       if (k < wid) off += shi[k];
       tot += shi[k];
     }
@@ -146,7 +184,7 @@ __device__ int wg_compact(int n, P pred, int32_t* out, int* shi) {
   return total;
 }
 
-template <int LPN>
+template <int BS, int LPN>
 struct Sched {
   const Inst& I;
   const f2m_opts& o;
@@ -156,23 +194,25 @@ struct Sched {
   int64_t sweeps = 0;
   double eps;
 
-  __device__ double grad_act() { return wg_pass<LPN>(I, I.act, cur, false, false, sh); }
+  __device__ double grad_act() { return wg_pass<BS, LPN>(I, cur, false, false, sh); }
   __device__ double sweep(bool frozen) {
-    const double m = wg_pass<LPN>(I, I.act, cur, true, frozen, sh);
+    const double m = wg_pass<BS, LPN>(I, cur, true, frozen, sh);
     cur ^= 1;
     ++sweeps;
     return m;
   }
   __device__ int block_glob() {
-    for (int v = threadIdx.x; v < I.n; v += BB) I.act[v] = I.deg[v] > o.glob_k ? o.glob_k : I.deg[v];
+    for (int v = threadIdx.x; v < I.n; v += BS) I.act[v] = I.deg[v] > o.glob_k ? o.glob_k : I.deg[v];
     __syncthreads();
+    wg_ef_rebuild<BS>(I, shi);
     int go = 0;
     for (; go < o.glob_rounds; ++go) {
       for (int s = 0; s < o.glob_sweeps; ++s) sweep(false);
       const double* F = I.F[cur];
-      for (int v = threadIdx.x; v < I.n; v += BB)
+      for (int v = threadIdx.x; v < I.n; v += BS)
         I.act[v] = part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
       __syncthreads();
+      wg_ef_rebuild<BS>(I, shi);
       const double M = grad_act();
       if (M < o.glob_exit) { ++go; break; }
     }
@@ -180,9 +220,10 @@ struct Sched {
   }
   __device__ double check_opt() {
     const double* F = I.F[cur];
-    for (int v = threadIdx.x; v < I.n; v += BB)
+    for (int v = threadIdx.x; v < I.n; v += BS)
       I.act[v] = part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
     __syncthreads();
+    wg_ef_rebuild<BS>(I, shi);
     return grad_act();
   }
   __device__ void block_ef(int N, bool frozen) {
@@ -224,7 +265,7 @@ struct Sched {
   // classes, tight components, primal, least squares; returns #IRREGULAR_B
   __device__ int combs_check(int& ncombs, int& irr_a) {
     const double* F = I.F[cur];
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       int cl, a, t;
       classify_node(v, I.b(v), I.act[v], eps, I.nbr, I.D, I.orig, F, cl, a, t);
       I.clen[v] = cl;
@@ -233,7 +274,7 @@ struct Sched {
       I.aux[v] = v;  // union-find parents
     }
     __syncthreads();
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       const int64_t b = I.b(v);
       const double Fv = F[v];
       for (int k = 0; k < I.clen[v]; ++k) {
@@ -244,30 +285,30 @@ struct Sched {
       }
     }
     __syncthreads();
-    for (int v = threadIdx.x; v < I.n; v += BB) I.root[v] = I.T[v] > 0 ? uf_find(I.aux, v) : -1;
+    for (int v = threadIdx.x; v < I.n; v += BS) I.root[v] = I.T[v] > 0 ? uf_find(I.aux, v) : -1;
     __syncthreads();
-    for (int v = threadIdx.x; v < I.n; v += BB) I.irr[v] = 0;
+    for (int v = threadIdx.x; v < I.n; v += BS) I.irr[v] = 0;
     __syncthreads();
     double nc = 0.0;
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       const int r = I.root[v];
       if (r >= 0 && 4 - 2 * I.A[v] - I.T[v] != 0) I.irr[r] = 1;
       if (r == v) nc += 1.0;
     }
     ncombs = (int)block_sum_all(nc, sh);
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       const int r = I.root[v];
       primal_node(v, I.b(v), I.deg[v], I.clen[v], eps, I.nbr, I.D, F, r >= 0 && I.irr[r] == 0, I.xw, I.fw);
     }
     __syncthreads();
     const Inst& II = I;
-    const int nirr = wg_compact(I.n, [&](int v) { return II.root[v] == v && II.irr[v] != 0; }, I.aux, shi);
+    const int nirr = wg_compact<BS>(I.n, [&](int v) { return II.root[v] == v && II.irr[v] != 0; }, I.aux, shi);
     irr_a = nirr;
     int irr_b = 0;
     LocalRP rpf{I.rp, I.sb};
     for (int i = 0; i < nirr; ++i) {
       const int r = I.aux[i];
-      const int m = wg_compact(I.n, [&](int v) { return II.root[v] == r; }, I.list, shi);
+      const int m = wg_compact<BS>(I.n, [&](int v) { return II.root[v] == r; }, I.list, shi);
       const int go = axb_comb(I.list, m, rpf, I.clen, I.A, I.T, eps, I.nbr, I.D, F, I.xw, I.xg, I.rn, sh);
       if (go == 0) {
         if (threadIdx.x == 0) I.irr[r] = 2;
@@ -281,19 +322,19 @@ struct Sched {
   // irr == 2; type 1 shifts only the smallest violating node of each comb.
   __device__ void shift(double sv, int type) {
     double* F = I.F[cur];
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       I.frozen[v] = 0;
       I.list[v] = INT_MAX;
     }
     __syncthreads();
     if (type == 1) {
-      for (int v = threadIdx.x; v < I.n; v += BB) {
+      for (int v = threadIdx.x; v < I.n; v += BS) {
         const int r = I.root[v];
         if (r >= 0 && I.irr[r] == 2 && 4 - 2 * I.A[v] - I.T[v] != 0) atomicMin(&I.list[r], v);
       }
       __syncthreads();
     }
-    for (int v = threadIdx.x; v < I.n; v += BB) {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
       const int r = I.root[v];
       const int s = 4 - 2 * I.A[v] - I.T[v];
       if (r >= 0 && I.irr[r] == 2 && s != 0 && (type == 0 || I.list[r] == v)) {
@@ -305,7 +346,7 @@ struct Sched {
   }
 };
 
-template <int LPN>
+template <int BS, int LPN>
 __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   Inst I;
   I.nb = a.node_off[inst];
@@ -336,13 +377,15 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   I.rn = a.rn + I.nb;
   I.fw = a.fw + I.sb;
   I.fc = a.fc + I.sb;
+  I.efoff = a.efoff + I.nb + inst;
+  I.ef = a.ef + I.sb;
   const f2m_opts& o = a.o;
   f2m_stats z;
   memset(&z, 0, sizeof z);
 
   // d_min: smallest positive cost (graph_tour.hpp:94)
   double mn = 1.0 / 0.0;
-  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+  for (int64_t s = threadIdx.x; s < I.m; s += BS) {
     const double c = I.gcost[s];
     if (c > 0 && c < mn) mn = c;
   }
@@ -354,19 +397,19 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
     }
     return;
   }
-  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+  for (int64_t s = threadIdx.x; s < I.m; s += BS) {
     I.nbr[s] = I.gnbr[s];
     I.D[s] = I.gcost[s] / dmin;
     I.orig[s] = (int32_t)s;
   }
-  for (int v = threadIdx.x; v < I.n; v += BB) {
+  for (int v = threadIdx.x; v < I.n; v += BS) {
     I.deg[v] = (int)(I.rp[v + 1] - I.rp[v]);
     I.F[0][v] = 0.0;
     I.frozen[v] = 0;
   }
   __syncthreads();
   z.d_min = dmin;
-  Sched<LPN> S{I, o, sh, shi};
+  Sched<BS, LPN> S{I, o, sh, shi};
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();
   double M = 0.0;
@@ -389,7 +432,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
       ++z.n_repairs;
       S.shift(sv, type);
       for (int k = 0; k < o.frozen_sweeps; ++k) S.sweep(true);
-      for (int v = threadIdx.x; v < I.n; v += BB) I.frozen[v] = 0;
+      for (int v = threadIdx.x; v < I.n; v += BS) I.frozen[v] = 0;
       __syncthreads();
       M = S.run_fix();
       return S.combs_check(z.n_combs, z.n_irregular_a);
@@ -409,7 +452,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   }
   z.status = irr_b == 0 ? 0 : 1;
   // x to canonical slots, objective, dual, MAX edge rank
-  for (int64_t s = threadIdx.x; s < I.m; s += BB) {
+  for (int64_t s = threadIdx.x; s < I.m; s += BS) {
     const int oo = I.orig[s];
     I.xc[oo] = I.xw[s];
     I.fc[oo] = I.fw[s];
@@ -417,7 +460,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   __syncthreads();
   const double* F = I.F[S.cur];
   double obj = 0.0, sf = 0.0, sneg = 0.0, mr = 0.0;
-  for (int v = threadIdx.x; v < I.n; v += BB) {
+  for (int v = threadIdx.x; v < I.n; v += BS) {
     const int64_t b = I.b(v), e = I.b(v + 1);
     const double Fv = F[v];
     sf += Fv;
@@ -451,18 +494,18 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   __syncthreads();
 }
 
-template <int LPN>
-__global__ void __launch_bounds__(BB) b_persistent(BArgs a) {
-  __shared__ double sh[BB / 64];
-  __shared__ int shi[BB / 64 + 1];
+template <int BS, int LPN>
+__global__ void __launch_bounds__(BS) b_persistent(BArgs a) {
+  __shared__ double sh[BS / 64];
+  __shared__ int shi[BS / 64 + 1];
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) shi[BB / 64] = atomicAdd(a.queue, 1);
+    if (threadIdx.x == 0) shi[BS / 64] = atomicAdd(a.queue, 1);
     __syncthreads();
-    const int idx = shi[BB / 64];
+    const int idx = shi[BS / 64];
     if (idx >= a.n_order) break;
     const long long t0 = wall_clock64();
-    solve_instance<LPN>(a, a.order[idx], sh, shi);
+    solve_instance<BS, LPN>(a, a.order[idx], sh, shi);
     if (a.clk && threadIdx.x == 0) a.clk[a.order[idx]] = wall_clock64() - t0;
   }
 }
@@ -500,21 +543,55 @@ namespace {
 
 struct BatchPlan {
   std::vector<int32_t> order;  // all instances, bucketed, decreasing size
-  std::vector<int> bucket_start, bucket_lpn;
+  std::vector<int> bucket_start, bucket_lpn, bucket_bs;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<unsigned char> key;
 };
 
-// Lanes per node for an instance of n nodes: F2M_BATCH_LPN, else the
-// context's autotuned table (f2m_batch_autotune), else 2 below 2000 nodes
-// and 4 above.
-int batch_lpn_for(const f2m_ctx* c, int64_t n) {
-  const char* e = std::getenv("F2M_BATCH_LPN");
-  if (e) return std::atoi(e);
+// Schedule of an instance of n nodes: (lanes per node, workgroup threads).
+// F2M_BATCH_LPN / F2M_BATCH_BS override; else the context's autotuned table
+// (f2m_batch_autotune); else lanes 2 below 1000 nodes and 4 above, 256
+// threads below 4000 nodes, 512 below 10000, 1024 above -- and when the
+// whole batch fits the GPU at a larger workgroup (n_inst * threads <= CUs *
+// 1024), the larger workgroup, since idle CUs would otherwise wait on the
+// slowest instance.
+void batch_cfg_for(const f2m_ctx* c, int64_t n, int n_inst, int ncu, int& lpn, int& bs) {
+  lpn = n < 1000 ? 2 : 4;
+  bs = n < 4000 ? 256 : (n < 10000 ? 512 : 1024);
+  while (bs < 1024 && (int64_t)n_inst * bs * 2 <= (int64_t)ncu * 1024 && n >= 2 * bs) bs *= 2;
   for (const auto& t : c->b_tune)
-    if (n <= t.first) return t.second;
-  return n >= 2000 ? 4 : 2;
+    if (n <= t.max_n) { lpn = t.lpn; bs = t.bs; break; }
+  if (const char* e = std::getenv("F2M_BATCH_LPN")) lpn = std::atoi(e);
+  if (const char* e = std::getenv("F2M_BATCH_BS")) bs = std::atoi(e);
+  if (lpn != 1 && lpn != 2 && lpn != 8) lpn = 4;
+  if (bs != 512 && bs != 1024) bs = 256;
+}
+
+template <int BS, int LPN>
+hipError_t launch_bucket(const BArgs& ak, int ncu, hipStream_t st) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<BS, LPN>, BS, 0);
+  if (e != hipSuccess) return e;
+  const int grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
+  hipLaunchKernelGGL((b_persistent<BS, LPN>), dim3(grid), dim3(BS), 0, st, ak);
+  return hipGetLastError();
+}
+
+template <int BS>
+hipError_t launch_bucket_bs(const BArgs& ak, int lpn, int ncu, hipStream_t st) {
+  switch (lpn) {
+    case 1: return launch_bucket<BS, 1>(ak, ncu, st);
+    case 2: return launch_bucket<BS, 2>(ak, ncu, st);
+    case 8: return launch_bucket<BS, 8>(ak, ncu, st);
+    default: return launch_bucket<BS, 4>(ak, ncu, st);
+  }
+}
+
+hipError_t launch_bucket_any(const BArgs& ak, int bs, int lpn, int ncu, hipStream_t st) {
+  if (bs == 1024) return launch_bucket_bs<1024>(ak, lpn, ncu, st);
+  if (bs == 512) return launch_bucket_bs<512>(ak, lpn, ncu, st);
+  return launch_bucket_bs<256>(ak, lpn, ncu, st);
 }
 
 }  // namespace
@@ -611,10 +688,14 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   F2M_HIP(c, w.irr.reserve(N * 4));
   F2M_HIP(c, w.sortv.reserve(N * 8));  // aux + list
   F2M_HIP(c, w.frozen.reserve(N));
+  F2M_HIP(c, c->b_ef.reserve(M * 16));
+  F2M_HIP(c, c->b_efoff.reserve((N + n_inst + 1) * 4));
   F2M_HIP(c, c->b_stats.reserve(n_inst * sizeof(f2m_stats) + 64 * 4));
   F2M_HIP(c, c->b_order.reserve((n_inst + 64) * 4ull));
   F2M_HIP(c, c->b_cur.reserve((n_inst + 64) * 4ull));
 
+  int ncu = 0;
+  F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   // host copy of node offsets -> size buckets -> queue order
   std::vector<int64_t> off(n_inst + 1);
   F2M_HIP(c, hipMemcpyAsync(off.data(), b->node_off, (n_inst + 1) * 8ull, hipMemcpyDefault, c->stream));
@@ -629,6 +710,7 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
     const unsigned char* q = reinterpret_cast<const unsigned char*>(&t);
     key.insert(key.end(), q, q + sizeof t);
   }
+  if (const char* e = std::getenv("F2M_BATCH_BS")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
   if (!c->b_plan) c->b_plan = new BatchPlan;
   BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
@@ -648,13 +730,16 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
     P->order = ids;
     P->bucket_start.clear();
     P->bucket_lpn.clear();
+    P->bucket_bs.clear();
     int prev = -1;
     for (int i = 0; i < n_inst; ++i) {
-      const int l = batch_lpn_for(c, off[ids[i] + 1] - off[ids[i]]);
-      if (l != prev) {
+      int l, bs;
+      batch_cfg_for(c, off[ids[i] + 1] - off[ids[i]], n_inst, ncu, l, bs);
+      if (l * 4096 + bs != prev) {
         P->bucket_start.push_back(i);
         P->bucket_lpn.push_back(l);
-        prev = l;
+        P->bucket_bs.push_back(bs);
+        prev = l * 4096 + bs;
       }
     }
     P->bucket_start.push_back(n_inst);
@@ -692,6 +777,8 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   a.rn = a.xg + M;
   a.fw = w.sortk.as<uint8_t>();
   a.fc = a.fw + M;
+  a.efoff = c->b_efoff.as<int32_t>();
+  a.ef = c->b_ef.as<ESlot>();
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;
@@ -701,8 +788,6 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
     a.clk = c->b_clk.as<long long>();
   }
 
-  int ncu = 0;
-  F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   auto enqueue = [&]() -> f2m_status {
     const int nb = (int)P->bucket_lpn.size();
     hipLaunchKernelGGL(b_zero_i32, dim3(1), dim3(64), 0, c->stream, queues, 16);
@@ -712,31 +797,7 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
       ak.order = c->b_order.as<int32_t>() + P->bucket_start[k];
       ak.n_order = P->bucket_start[k + 1] - P->bucket_start[k];
       ak.queue = queues + k;
-      int per_cu = 0;
-      int grid;
-      switch (P->bucket_lpn[k]) {
-        case 1:
-          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<1>, BB, 0));
-          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
-          hipLaunchKernelGGL(b_persistent<1>, dim3(grid), dim3(BB), 0, c->stream, ak);
-          break;
-        case 2:
-          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<2>, BB, 0));
-          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
-          hipLaunchKernelGGL(b_persistent<2>, dim3(grid), dim3(BB), 0, c->stream, ak);
-          break;
-        case 8:
-          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<8>, BB, 0));
-          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
-          hipLaunchKernelGGL(b_persistent<8>, dim3(grid), dim3(BB), 0, c->stream, ak);
-          break;
-        default:
-          F2M_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<4>, BB, 0));
-          grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
-          hipLaunchKernelGGL(b_persistent<4>, dim3(grid), dim3(BB), 0, c->stream, ak);
-          break;
-      }
-      F2M_LAUNCH_CHECK(c);
+      F2M_HIP(c, launch_bucket_any(ak, P->bucket_bs[k], P->bucket_lpn[k], ncu, c->stream));
     }
     return F2M_OK;
   };
@@ -800,14 +861,16 @@ f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o,
   if (!c || !b || b->n_inst < 1) return F2M_ERR_ARG;
   static const int64_t bounds[] = {500, 1000, 2000, 4000, 8000, 16000, INT64_MAX};
   constexpr int NC = sizeof(bounds) / sizeof(bounds[0]);
-  static const int cand[] = {1, 2, 4, 8};
+  static const int cand_lpn[] = {1, 2, 4, 8};
+  static const int cand_bs[] = {256, 512, 1024};
+  using TE = f2m_ctx::TuneEntry;
   if (cache_path) {
     FILE* f = std::fopen(cache_path, "r");
     if (f) {
-      std::vector<std::pair<int64_t, int32_t>> t;
+      std::vector<TE> t;
       long long mx;
-      int l;
-      while (std::fscanf(f, "%lld %d", &mx, &l) == 2) t.push_back({(int64_t)mx, (int32_t)l});
+      int l, bs;
+      while (std::fscanf(f, "%lld %d %d", &mx, &l, &bs) == 3) t.push_back(TE{(int64_t)mx, (int32_t)l, (int32_t)bs});
       std::fclose(f);
       if (!t.empty()) {
         c->b_tune = t;
@@ -821,44 +884,47 @@ f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o,
   std::vector<int64_t> off(ns + 1);
   F2M_HIP(c, hipMemcpy(off.data(), b->node_off, (ns + 1) * 8ull, hipMemcpyDefault));
   double best_t[NC];
-  int best_l[NC];
-  for (int k = 0; k < NC; ++k) { best_t[k] = -1; best_l[k] = 0; }
+  TE best[NC];
+  for (int k = 0; k < NC; ++k) { best_t[k] = -1; best[k] = TE{bounds[k], 0, 0}; }
   const auto saved = c->b_tune;
   c->b_clk_on = true;
   std::vector<long long> clk(ns);
   f2m_status st = F2M_OK;
-  for (int l : cand) {
-    c->b_tune = {{INT64_MAX, l}};
-    st = f2m_solve_batched(c, &sub, o, nullptr);
-    if (st) break;
-    if (hipMemcpy(clk.data(), c->b_clk.p, ns * 8ull, hipMemcpyDeviceToHost) != hipSuccess) { st = F2M_ERR_HIP; break; }
-    double tc[NC] = {0};
-    int cnt[NC] = {0};
-    for (int i = 0; i < ns; ++i) {
-      const int64_t n = off[i + 1] - off[i];
-      int k = 0;
-      while (n > bounds[k]) ++k;
-      tc[k] += (double)clk[i];
-      ++cnt[k];
+  for (int bs : cand_bs)
+    for (int l : cand_lpn) {
+      c->b_tune = {TE{INT64_MAX, l, bs}};
+      st = f2m_solve_batched(c, &sub, o, nullptr);
+      if (st) break;
+      if (hipMemcpy(clk.data(), c->b_clk.p, ns * 8ull, hipMemcpyDeviceToHost) != hipSuccess) { st = F2M_ERR_HIP; break; }
+      // a workgroup of bs threads holds bs/256 times the resources of the
+      // smallest: weigh its instance time accordingly
+      double tc[NC] = {0};
+      int cnt[NC] = {0};
+      for (int i = 0; i < ns; ++i) {
+        const int64_t n = off[i + 1] - off[i];
+        int k = 0;
+        while (n > bounds[k]) ++k;
+        tc[k] += (double)clk[i] * (bs / 256);
+        ++cnt[k];
+      }
+      for (int k = 0; k < NC; ++k)
+        if (cnt[k] && (best_t[k] < 0 || tc[k] < best_t[k])) { best_t[k] = tc[k]; best[k] = TE{bounds[k], l, bs}; }
     }
-    for (int k = 0; k < NC; ++k)
-      if (cnt[k] && (best_t[k] < 0 || tc[k] < best_t[k])) { best_t[k] = tc[k]; best_l[k] = l; }
-  }
   c->b_clk_on = false;
   if (st) { c->b_tune = saved; return st; }
   // classes without samples take the nearest sampled class below, else above
-  std::vector<std::pair<int64_t, int32_t>> t;
-  int last = 0;
-  for (int k = 0; k < NC; ++k) if (best_l[k]) { last = best_l[k]; break; }
+  std::vector<TE> t;
+  TE last{0, 0, 0};
+  for (int k = 0; k < NC; ++k) if (best[k].lpn) { last = best[k]; break; }
   for (int k = 0; k < NC; ++k) {
-    if (best_l[k]) last = best_l[k];
-    t.push_back({bounds[k], last ? last : 4});
+    if (best[k].lpn) last = best[k];
+    t.push_back(TE{bounds[k], last.lpn ? last.lpn : 4, last.bs ? last.bs : 256});
   }
   c->b_tune = t;
   if (cache_path) {
     FILE* f = std::fopen(cache_path, "w");
     if (f) {
-      for (auto& e : t) std::fprintf(f, "%lld %d\n", (long long)e.first, e.second);
+      for (auto& e : t) std::fprintf(f, "%lld %d %d\n", (long long)e.max_n, e.lpn, e.bs);
       std::fclose(f);
     }
   }

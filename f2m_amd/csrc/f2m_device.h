@@ -5,6 +5,19 @@
 
 namespace f2m {
 
+// Active-list slot: the reference's EF array (graph_cuda.cu:229-252,
+// get_cuda_EF graph_cuda_grad.hpp:96-111) holds pointers to the active edges;
+// here the (D, nbr) pair itself is copied, 16 bytes, one vector load.
+struct ESlot {
+  double D;
+  int32_t nbr;
+  int32_t pad;
+};
+
+// Threads that run the least squares of one component: fixed, so that its
+// reductions add in the same order whatever the launch's block size.
+constexpr int AXB_T = 256;
+
 __device__ __forceinline__ double wave_max_d(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -134,6 +147,8 @@ __device__ __forceinline__ void uf_unite(int* par, int a, int c) {
 // search; stop when max |r_j| < 1e-8; at most 999 iterations; x stored
 // rounded to float (XB is float, F2M.h:202).  Returns go_max (0 = failed).
 // rp64/base: slot offsets of node v are [rp(v), rp(v)+clen[v]).
+// Only threads [0, AXB_T) take nodes; the others join the barriers and add
+// zeros to the reductions, which leaves the sums and maxima unchanged.
 template <class RP>
 __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const int32_t* __restrict__ clen,
                         const int32_t* __restrict__ A, const int32_t* __restrict__ T, double eps,
@@ -145,7 +160,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     const double d = dval(D[s], Fv, F[w], v, w);
     return d >= -eps && d <= eps;
   };
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+  for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
     const int v = nodes[i];
     const int64_t b = rp(v);
     const double Fv = F[v];
@@ -156,7 +171,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
   int go_max = 0;
   for (int go = 1; go < 1000; ++go) {
     double dmax = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
       const int v = nodes[i];
       const int64_t b = rp(v);
       const double Fv = F[v];
@@ -169,7 +184,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     }
     dmax = block_max_all(dmax, sh);  // barriers publish rn
     if (dmax < 1e-8) { go_max = go; break; }
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
       const int v = nodes[i];
       const int64_t b = rp(v);
       const double Fv = F[v];
@@ -182,7 +197,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     }
     __syncthreads();
     double gg = 0.0, gd = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
       const int v = nodes[i];
       const int64_t b = rp(v);
       const double Fv = F[v];
@@ -196,7 +211,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     gd = block_sum_all(gd, sh);
     if (gg == 0.0) break;  // the reference would take 0/0 = NaN here; treated as no convergence
     const double alfa = -gd / gg;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
       const int v = nodes[i];
       const int64_t b = rp(v);
       const double Fv = F[v];
@@ -205,7 +220,7 @@ __device__ int axb_comb(const int32_t* __restrict__ nodes, int m, RP rp, const i
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+  for (int i = threadIdx.x < AXB_T ? (int)threadIdx.x : m; i < m; i += AXB_T) {
     const int v = nodes[i];
     const int64_t b = rp(v);
     const double Fv = F[v];

@@ -65,10 +65,15 @@ struct f2m_ctx {
   f2m::DevBuf stage[6];
   f2m::SolveWs ws;
   // batched storage
-  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc;
+  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc, b_ef, b_efoff;
   f2m::SolveWs bws;
   void* b_plan = nullptr;                          // captured launch plan (f2m_batch.hip)
-  std::vector<std::pair<int64_t, int32_t>> b_tune;  // (max instance size, lanes per node)
+  struct TuneEntry {
+    int64_t max_n;  // instances with n <= max_n ...
+    int32_t lpn;    // ... use this many lanes per node
+    int32_t bs;     // ... and workgroups of this many threads
+  };
+  std::vector<TuneEntry> b_tune;
   f2m::DevBuf b_clk;                               // per-instance ticks while autotuning
   bool b_clk_on = false;
   int64_t b_n_total = 0, b_nslots = 0;

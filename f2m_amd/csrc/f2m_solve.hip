@@ -115,15 +115,6 @@ __global__ void s_act_min(int n, const int32_t* __restrict__ deg, int32_t* __res
   if (v < n) act[v] = deg[v] > k ? k : deg[v];
 }
 
-// Active-list slot: the reference's EF array (graph_cuda.cu:229-252,
-// get_cuda_EF graph_cuda_grad.hpp:96-111) holds pointers to the active edges;
-// here the (D, nbr) pair itself is copied, 16 bytes, one vector load.
-struct ESlot {
-  double D;
-  int32_t nbr;
-  int32_t pad;
-};
-
 // Compact the first act[v] working slots of every node into ef[efoff[v]..].
 __global__ void s_ef_fill(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                           const int32_t* __restrict__ efoff, const int32_t* __restrict__ nbr,

@@ -206,12 +206,14 @@ f2m_status f2m_knn_build_batched(f2m_ctx* ctx, int32_t n_inst, const int64_t* no
 /* Solves every instance with the plain schedule, all in one persistent launch
  * per phase.  st [host] n_inst stats (may be NULL). */
 f2m_status f2m_solve_batched(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, f2m_stats* st);
-/* Autotuner (new): times lanes-per-node schedules {1,2,4,8} on the first
- * `sample` instances (per-instance device clocks), keeps the fastest per size
- * class (<=500, 1000, 2000, 4000, 8000, 16000, larger) for later
- * f2m_solve_batched calls on this context.  With cache_path, an existing file
- * is loaded instead of tuning, else the chosen table is written there.  The
- * schedules differ in speed only: results are bit-identical. */
+/* Autotuner (new): times the schedules (lanes per node 1/2/4/8 x workgroup
+ * 256/512/1024 threads) on the first `sample` instances (per-instance device
+ * clocks, weighted by workgroup size), keeps the fastest per size class
+ * (<=500, 1000, 2000, 4000, 8000, 16000, larger) for later f2m_solve_batched
+ * calls on this context.  With cache_path, an existing file ("max_n lanes
+ * threads" per line) is loaded instead of tuning, else the chosen table is
+ * written there.  The schedules differ in speed only: results are
+ * bit-identical. */
 f2m_status f2m_batch_autotune(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, int32_t sample,
                               const char* cache_path);
 /* Copy a batch's arrays to caller arrays [dev|host]: node_off (n_inst+1),

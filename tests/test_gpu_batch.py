@@ -75,21 +75,35 @@ def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
     plain.close()
 
 
+def same_stats(a, b):
+    """Schedules and components identical; the FP64 sums (objective, dual) may
+    differ in the last bits across workgroup sizes (reduction width)."""
+    for sa, sb in zip(a, b):
+        for k in sa:
+            if k in ("objective", "dual"):
+                assert abs(sa[k] - sb[k]) <= 1e-12 * abs(sb[k]), (k, sa[k], sb[k])
+            else:
+                assert sa[k] == sb[k], (k, sa[k], sb[k])
+    assert len(a) == len(b)
+
+
 def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch):
     xs, ys = batch_case
     b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
     ref = None
-    for lpn in ("1", "2", "4", "8"):
+    for lpn, bs in (("1", "256"), ("2", "256"), ("4", "512"), ("8", "1024"), ("2", "1024")):
         monkeypatch.setenv("F2M_BATCH_LPN", lpn)
-        say("batch lpn", lpn)
+        monkeypatch.setenv("F2M_BATCH_BS", bs)
+        say("batch lpn", lpn, "threads", bs)
         sts = ctx.solve_batched(b)
         res = ctx.batch_result(b)
         if ref is None:
             ref = (sts, res)
         else:
-            assert sts == ref[0]
+            same_stats(sts, ref[0])
             assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
     monkeypatch.delenv("F2M_BATCH_LPN")
+    monkeypatch.delenv("F2M_BATCH_BS")
     # captured graph: first call captures, later calls replay; eager work between
     outs = []
     for rep in range(3):
@@ -99,10 +113,12 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
         outs.append((sts, res))
         _ = ctx.knn_directed(xs[0], ys[0], 5, W.COST_NINT)  # unrelated eager work on the stream
     for sts, res in outs:
-        assert sts == ref[0] and np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+        assert sts == outs[0][0]  # replays of one captured graph: identical bits
+        same_stats(sts, ref[0])
+        assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
     monkeypatch.setenv("F2M_NO_GRAPH", "1")
     sts = ctx.solve_batched(b)
-    assert sts == ref[0]
+    assert sts == outs[0][0]
 
 
 def test_batch_autotune_keeps_results(ctx, batch_case, tmp_path):
@@ -113,8 +129,8 @@ def test_batch_autotune_keeps_results(ctx, batch_case, tmp_path):
     say("autotune")
     ctx.batch_autotune(b, sample=len(xs), cache_path=cache)
     lines = open(cache).read().split()
-    assert len(lines) == 14 and set(lines[1::2]) <= {"1", "2", "4", "8"}
-    assert ctx.solve_batched(b) == ref
+    assert len(lines) == 21 and set(lines[1::3]) <= {"1", "2", "4", "8"} and set(lines[2::3]) <= {"256", "512", "1024"}
+    same_stats(ctx.solve_batched(b), ref)
     ctx.batch_autotune(b, sample=4, cache_path=cache)  # loads the cache
 
 
