@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-every", type=int, default=16)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the solves, copy a buffer of this many MiB (counter calibration)")
     ap.add_argument("--no-fwd-bwd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -68,6 +70,15 @@ def main():
 
     from f2m_amd import workloads as W
     from f2m_amd.solver import Context
+
+    if args.calibrate_copy_mib:
+        import torch
+        a = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy", args.calibrate_copy_mib, "MiB")
+        del a, b
 
     x, y, mode, K = W.c2()
     n = len(x)
