@@ -91,10 +91,7 @@ def lp_opt(rowptr, nbr, cost):
                    shape=(n, E)).tocsr()
     r = linprog(c, A_eq=A, b_eq=2 * np.ones(n), bounds=(0, 1), method="highs")
     assert r.status == 0, r.message
-    xs = r.x
-    # uniqueness probe: re-solve with a tiny perturbation of the costs in two
-    # directions; if both give the same support the optimum is taken as unique
-    return float(r.fun), u, w, xs
+    return float(r.fun), u, w, r.x
 
 
 def main():
@@ -148,5 +145,37 @@ def main():
                 fo.write(fi.read())
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and "--heavy" not in sys.argv:
     main()
+
+
+def heavy_known_answers():
+    """`python tests/golden/make_golden.py --heavy`: HiGHS optimum of the LP on
+    the sra104815 K=50 edge set (~1.43M edges, a few minutes), added to
+    golden.json as known[...]['lp_opt_same_edges']."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from oracle import tsplib
+    path = os.path.join(HERE, "golden.json")
+    g = json.load(open(path))
+    lab, x, y, _ = tsplib.read(os.path.join("/root/reference/examples/vsli", "sra104815.tsp"))
+    rp, nb, c = O.graph_build(x, y, 50, 0)
+    n = len(rp) - 1
+    src = np.repeat(np.arange(n), np.diff(rp))
+    m = src < nb
+    u, w, cc = src[m], nb[m], c[m]
+    E = len(u)
+    A = coo_matrix((np.ones(2 * E), (np.concatenate([u, w]), np.concatenate([np.arange(E)] * 2))),
+                   shape=(n, E)).tocsr()
+    r = linprog(cc, A_eq=A, b_eq=2 * np.ones(n), bounds=(0, 1), method="highs-ds")
+    assert r.status == 0, r.message
+    for k in g["known"]:
+        if k["instance"] == "sra104815" and k["K"] == 50:
+            k["lp_opt_same_edges"] = float(r.fun)
+            k["lp_method"] = "scipy.optimize.linprog(method='highs-ds')"
+    json.dump(g, open(path, "w"), indent=1)
+    print("sra104815 K=50 LP optimum", r.fun)
+
+
+if __name__ == "__main__" and "--heavy" in sys.argv:
+    heavy_known_answers()
