@@ -239,7 +239,7 @@ f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out) {
 static void release_ws(f2m::SolveWs& w) {
   f2m::DevBuf* all[] = {&w.nbr, &w.D, &w.orig, &w.F0, &w.F1, &w.g, &w.act, &w.clen, &w.deg,
                         &w.A, &w.T, &w.root, &w.irr, &w.frozen, &w.xw, &w.xc, &w.red, &w.axb,
-                        &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp, &w.ef, &w.efoff};
+                        &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp, &w.ef, &w.efoff, &w.psl, &w.povo, &w.ptmp};
   for (auto* b : all) b->release();
 }
 

@@ -45,6 +45,7 @@ struct SolveWs {
   DevBuf axb;                   // least-squares scratch
   DevBuf sortk, sortv, sortk2, sortv2, sorttmp;  // comb node sort
   DevBuf ef, efoff;             // compacted active lists (16-byte slots) + offsets
+  DevBuf psl, povo, ptmp;       // padded active lists (v*8 + k, overflow after n*8) + offsets
   int cur = 0;                  // which F buffer holds the current potentials
   int64_t n = 0, m = 0;
 };

@@ -169,6 +169,78 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
   if (maxslots) block_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
 }
 
+// ---- padded layout ----------------------------------------------------------
+// The compacted sweep above reads three dependent levels: list offset ->
+// slot -> neighbour potential.  The padded layout keeps each node's first PP
+// active slots at a fixed stride (v*PP + k; longer lists continue in an
+// overflow region after n*PP), so the slot loads need no offset and are
+// issued unconditionally together with act[v] and F[v]: two levels (slots,
+// then neighbour potentials) for every node with at most PP active slots.
+// (Pushing potentials into per-slot copies instead, to save the second
+// level, was measured slower: the scattered copy stores cost more than the
+// gather they replace.)
+constexpr int PP = 8;    // padded active slots per node
+constexpr int PLPN = 4;  // lanes per node of the padded sweep (PP = 2 * PLPN)
+
+__global__ void s_excess(int n, const int32_t* __restrict__ a, int p, int32_t* __restrict__ out) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) out[v] = a[v] > p ? a[v] - p : 0;
+}
+__global__ void s_pad_fill(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
+                           const int32_t* __restrict__ ovo, const int32_t* __restrict__ nbr,
+                           const double* __restrict__ D, ESlot* __restrict__ sl) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v], L = act[v], oo = ovo[v];
+  for (int k = 0; k < L; ++k) {
+    const int64_t idx = k < PP ? (int64_t)v * PP + k : (int64_t)n * PP + oo + (k - PP);
+    sl[idx] = ESlot{D[b + k], nbr[b + k], 0};
+  }
+}
+
+template <bool FROZEN, bool WRITE_F>
+__global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restrict__ act,
+                                                  const int32_t* __restrict__ ovo, const ESlot* __restrict__ sl,
+                                                  const double* __restrict__ Fin, double* __restrict__ Fout,
+                                                  const uint8_t* __restrict__ frozen,
+                                                  unsigned long long* __restrict__ maxslots) {
+  constexpr int LPN = PLPN;
+  const int gt = blockIdx.x * BS + threadIdx.x;
+  const int v = gt / LPN, lane = gt % LPN;
+  Top3 t;
+  double Fv = 0.0;
+  const bool ok = v < n;
+  if (ok) {
+    const int L = act[v];
+    const int oo = ovo[v];
+    Fv = Fin[v];
+    const int64_t p = (int64_t)v * PP + lane;  // padded entries: loaded unconditionally
+    const ESlot s0 = sl[p], s1 = sl[p + LPN];
+    if (lane < L) t.push(dval(s0.D, Fv, Fin[s0.nbr], v, s0.nbr));
+    if (lane + LPN < L) t.push(dval(s1.D, Fv, Fin[s1.nbr], v, s1.nbr));
+    for (int k = lane + PP; k < L; k += LPN) {
+      const ESlot sk = sl[(int64_t)n * PP + oo + (k - PP)];
+      t.push(dval(sk.D, Fv, Fin[sk.nbr], v, sk.nbr));
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < LPN; off <<= 1) {
+    const double a2 = __shfl_xor(t.a, off, 64);
+    const double b2 = __shfl_xor(t.b, off, 64);
+    const double c2 = __shfl_xor(t.c, off, 64);
+    t.push(a2);
+    t.push(b2);
+    t.push(c2);
+  }
+  double gabs = 0.0;
+  if (ok && lane == 0) {
+    const double g = t.grad();
+    if (WRITE_F) Fout[v] = (FROZEN && frozen[v]) ? Fv : Fv + g;
+    gabs = fabs(g);
+  }
+  if (maxslots) block_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
+}
+
 // cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
 __global__ void s_part_v4(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
                           int32_t* __restrict__ act, int kk, int32_t* __restrict__ nbr, double* __restrict__ D,
@@ -443,6 +515,7 @@ struct Runner {
   double dmin = 0, eps = 0;
   int64_t sweeps = 0;
   int lpn = 4;
+  bool padded = true;  // padded layout (default) or compacted lists (F2M_SWEEP=compact)
   // instrumentation: active slots of the current lists, totals
   unsigned long long cur_actsum = 0;
   int64_t active_slots = 0;
@@ -482,6 +555,9 @@ struct Runner {
     F2M_HIP(c, w.axb.reserve(m * 8 + (int64_t)n * 8 + (int64_t)(n + 1) * 4));
     F2M_HIP(c, w.ef.reserve(m * 16));
     F2M_HIP(c, w.efoff.reserve((n + 1) * 4));
+    F2M_HIP(c, w.psl.reserve(((int64_t)n * PP + m) * 16));
+    F2M_HIP(c, w.povo.reserve((n + 1) * 4));
+    F2M_HIP(c, w.ptmp.reserve(n * 4));
     F2M_HIP(c, w.red.reserve(MAXS * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
     maxs = w.red.as<unsigned long long>();
     part = reinterpret_cast<double*>(maxs + MAXS);
@@ -509,6 +585,8 @@ struct Runner {
     F2M_LAUNCH_CHECK(c);
     eps = o.err_grad * o.err_comb_mult;
     lpn = env_int("F2M_LPN", 4);
+    const char* v = std::getenv("F2M_SWEEP");
+    padded = !(v && std::strcmp(v, "compact") == 0);
     return F2M_OK;
   }
 
@@ -530,6 +608,17 @@ struct Runner {
       hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, eo, ef, fin, fout, nullptr, fz,
                          mslots);
   }
+  void launch_padded(bool write, bool frozen, unsigned long long* mslots) {
+    dim3 grid(grid_for((int64_t)n * PLPN, BS)), block(BS);
+    const uint8_t* fz = w.frozen.as<uint8_t>();
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, block, 0, c->stream, n, I(w.act), I(w.povo), w.psl.as<ESlot>(), F(), Fo(), fz,
+                         mslots);
+    };
+    if (!write) go(s_sweep_pad<false, false>);
+    else if (frozen) go(s_sweep_pad<true, true>);
+    else go(s_sweep_pad<false, true>);
+  }
   f2m_status launch_sweep(bool write, bool frozen, unsigned long long* mslots) {
     const bool timed = write && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
     if (timed) {
@@ -540,7 +629,8 @@ struct Runner {
       }
       F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
     }
-    switch (lpn) {
+    if (padded) launch_padded(write, frozen, mslots);
+    else switch (lpn) {
       case 1: launch_sweep_t<1>(write, frozen, mslots); break;
       case 2: launch_sweep_t<2>(write, frozen, mslots); break;
       case 8: launch_sweep_t<8>(write, frozen, mslots); break;
@@ -579,8 +669,16 @@ struct Runner {
   // prefixes.  With profiling on, the total is read back for the byte count.
   f2m_status ef_rebuild() {
     F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
-    hipLaunchKernelGGL(s_ef_fill, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.act), I(w.efoff),
-                       I(w.nbr), w.D.as<double>(), w.ef.as<ESlot>());
+    const dim3 gn(grid_for(n, BS)), bs(BS);
+    if (padded) {
+      hipLaunchKernelGGL(s_excess, gn, bs, 0, c->stream, n, I(w.act), PP, I(w.ptmp));
+      F2M_HIP(c, scan_i32_to_i32(c, I(w.ptmp), I(w.povo), n, nullptr));
+      hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
+                         w.psl.as<ESlot>());
+    } else {
+      hipLaunchKernelGGL(s_ef_fill, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.efoff), I(w.nbr), w.D.as<double>(),
+                         w.ef.as<ESlot>());
+    }
     F2M_LAUNCH_CHECK(c);
     if (c->prof_every > 0) {
       int32_t* hp = static_cast<int32_t*>(c->pinned);
