@@ -145,7 +145,7 @@ def main():
                 fo.write(fi.read())
 
 
-if __name__ == "__main__" and "--heavy" not in sys.argv:
+if __name__ == "__main__" and "--heavy" not in sys.argv and "--lattice" not in sys.argv:
     main()
 
 
@@ -179,3 +179,36 @@ def heavy_known_answers():
 
 if __name__ == "__main__" and "--heavy" in sys.argv:
     heavy_known_answers()
+
+
+def lattice_optima(full: bool):
+    """LP optima (HiGHS) on lattice instances with many equal-cost ties (the C3
+    generator): n=20000 always, n=1e6 (C3 itself) with --c3."""
+    path = os.path.join(HERE, "golden.json")
+    g = json.load(open(path))
+    g.setdefault("lattice", [])
+    sizes = [2000, 20000] + ([1_000_000] if full else [])
+    for n in sizes:
+        if any(e["n"] == n for e in g["lattice"]):
+            continue
+        x, y, mode, K = W.c3(n=n)
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        rp, nb, c = O.graph_build(x, y, K, mode)
+        nn = len(rp) - 1
+        src = np.repeat(np.arange(nn), np.diff(rp))
+        m = src < nb
+        u, w, cc = src[m], nb[m], c[m]
+        E = len(u)
+        A = coo_matrix((np.ones(2 * E), (np.concatenate([u, w]), np.concatenate([np.arange(E)] * 2))),
+                       shape=(nn, E)).tocsr()
+        r = linprog(cc, A_eq=A, b_eq=2 * np.ones(nn), bounds=(0, 1), method="highs-ipm" if n > 100000 else "highs-ds")
+        assert r.status == 0, r.message
+        g["lattice"].append({"n": n, "K": K, "mode": mode, "nslots": int(len(nb)),
+                             "graph_sha256": graph_hash(rp, nb, c), "lp_opt": float(r.fun)})
+        json.dump(g, open(path, "w"), indent=1)
+        print("lattice", n, "LP optimum", r.fun, flush=True)
+
+
+if __name__ == "__main__" and "--lattice" in sys.argv:
+    lattice_optima("--c3" in sys.argv)
