@@ -34,6 +34,8 @@ class Opts(ctypes.Structure):
         ("factor_step", c_i32), ("factor_max", c_i32), ("check_every", c_i32), ("early_exit", c_i32),
         ("repair", c_i32), ("shift0", c_f64), ("shift_max", c_f64), ("fix_rounds", c_i32),
         ("fix_factor", c_i32), ("frozen_sweeps", c_i32), ("stop_phase", c_i32),
+        ("exact", c_i32), ("exact_alpha", c_i32), ("exact_eps0", c_f64), ("exact_tol", c_f64),
+        ("exact_max_rounds", c_i64), ("exact_max_phases", c_i32), ("exact_reserved", c_i32),
     ]
 
 
@@ -42,6 +44,8 @@ class Stats(ctypes.Structure):
         ("objective", c_f64), ("dual", c_f64), ("max_grad", c_f64), ("d_min", c_f64), ("sweeps", c_i64),
         ("glob_rounds", c_i32), ("outer_rounds", c_i32), ("n_combs", c_i32), ("n_irregular_a", c_i32),
         ("n_irregular_b", c_i32), ("n_repairs", c_i32), ("status", c_i32), ("max_rank", c_i32),
+        ("exact_ran", c_i32), ("exact_phases", c_i32), ("exact_rounds", c_i64), ("exact_gap", c_f64),
+        ("exact_certified", c_i32), ("exact_reserved", c_i32),
     ]
 
     def as_dict(self):

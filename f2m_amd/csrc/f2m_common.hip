@@ -189,6 +189,13 @@ void f2m_default_opts(f2m_opts* o) {
   o->fix_factor = 500;
   o->frozen_sweeps = 20;
   o->stop_phase = 0;
+  o->exact = 1;
+  o->exact_alpha = 4;
+  o->exact_eps0 = 0.02;
+  o->exact_tol = 1e-10;
+  o->exact_max_rounds = 1000000;
+  o->exact_max_phases = 40;
+  o->exact_reserved = 0;
 }
 
 const char* f2m_strerror(f2m_status s) {
@@ -239,7 +246,8 @@ f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out) {
 static void release_ws(f2m::SolveWs& w) {
   f2m::DevBuf* all[] = {&w.nbr, &w.D, &w.orig, &w.F0, &w.F1, &w.g, &w.act, &w.clen, &w.deg,
                         &w.A, &w.T, &w.root, &w.irr, &w.frozen, &w.xw, &w.xc, &w.red, &w.axb,
-                        &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp, &w.ef, &w.efoff, &w.psl, &w.povo, &w.ptmp};
+                        &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp, &w.ef, &w.efoff, &w.psl, &w.povo, &w.ptmp,
+                        &w.xmir, &w.xf, &w.xp, &w.xe, &w.xact, &w.xcnt};
   for (auto* b : all) b->release();
 }
 

@@ -1,0 +1,462 @@
+// f2m_exact.hip -- exact primal recovery (no reference counterpart).
+//
+// Replaces the reference's least-squares recovery and dual-perturbation loop
+// (comb_thread.cpp:272-304, Ax_b.cpp:139-158, graph_cuda.cu:69-126) where they
+// run (irregular tight components: float least squares, often x outside
+// [0,1] on tie-heavy instances) or fail (status 1), or always on request.
+//
+// The fractional 2-matching LP equals half the min-cost transportation
+// problem on the bipartite double cover: L_v (supply 2) -> R_w (demand 2),
+// arc capacity 1, cost c_vw, both orientations of every edge;
+// x_e = (f_vw + f_wv) / 2.  It is solved by cost-scaling push-relabel with
+// eps-optimality  rc(v,w) = (c + pL[v]) - pR[w] >= -eps  on residual arcs.
+//
+// Schedule (synchronous, hence deterministic and equal bit for bit to the
+// oracle's restatement): one round = an L step over every left node with
+// positive excess, then an R step over every right node with positive
+// excess.  No arc joins two nodes of one side, so a step is a valid sequence
+// of push/relabel operations in any order.  A node discharges completely:
+//   L_v: repeatedly the residual arc of largest pR[w] - c (ties: lowest slot);
+//        relabel pL = (pR[w] - c) - eps when it is not admissible; push 1.
+//   R_w: repeatedly the in-flow arc of largest c + pL[v] (ties: lowest slot);
+//        relabel pR = (c + pL[v]) - eps when not admissible; return 1.
+// then relabels once more when no admissible arc is left (a push becomes an
+// auction bid with the full margin to the next-best arc).
+// Phases: prices from the dual descent (pL = d_min F/2, pR = -pL, rc = d_min
+// d_e), f = [rc < 0]; eps = eps0 d_min, divided by alpha per phase, with the
+// violating arcs saturated or emptied at each new eps.  After each phase the
+// duality gap P - D of x and y = (pR - pL)/2 is summed in a fixed order; the
+// stage stops when it is within tolerance (< 1/2 for integer costs, where
+// the LP optimum is a multiple of 1/2 and P is then exactly optimal).
+//
+// Layout.  Per slot (canonical CSR): flow byte f, mirror slot index.  Per
+// node: pL, pR (double), excess eL, eR (int32).  Active lists: actL (two
+// buffers, by round parity), actR, with counters in a ring of 4 rounds;
+// a node joins the other side's list when its excess crosses 0 -> 1.
+// Discharge steps: 16 lanes per node, grid-stride over the active list.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "f2m_device.h"
+
+namespace f2m {
+namespace {
+
+constexpr int XBS = 256;    // threads per block
+constexpr int XG = 16;      // lanes per node in the discharge steps
+constexpr int XGRID = 512;  // blocks of a discharge step
+constexpr int RING = 4;     // counter ring, rounds mod 4
+constexpr int DET_T = 1024; // threads of the fixed-order sum
+
+struct XDev {
+  int n;
+  const int32_t* rp;
+  const int32_t* nbr;
+  const double* c;
+  const int32_t* mir;
+  uint8_t* f;
+  double* pL;
+  double* pR;
+  int32_t* eL;
+  int32_t* eR;
+  int32_t* actL0;
+  int32_t* actL1;
+  int32_t* actR;
+  int32_t* cnt;                  // [2*RING]: nL, nR of round r at 2*(r%RING)
+  unsigned long long* rounds;    // rounds with an active node
+};
+
+__global__ void x_mirror(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
+                         const double* __restrict__ c, int32_t* __restrict__ mir, int32_t* __restrict__ flag) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = rp[v], e = rp[v + 1];
+  if (e - b < 2) atomicOr(&flag[1], 1);
+  for (int s = b; s < e; ++s) {
+    const int w = nbr[s];
+    int q = rp[w];
+    const int qe = rp[w + 1];
+    while (q < qe && nbr[q] != v) ++q;
+    if (q == qe || c[q] != c[s]) {
+      atomicOr(&flag[0], 1);
+      mir[s] = s;
+    } else {
+      mir[s] = q;
+    }
+    if (c[s] != floor(c[s])) atomicOr(&flag[2], 1);
+  }
+}
+
+__global__ void x_prices(int n, const double* __restrict__ F, double dmin, double* __restrict__ pL,
+                         double* __restrict__ pR) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const double p = 0.5 * (dmin * F[v]);
+  pL[v] = p;
+  pR[v] = -p;
+}
+
+// init (saturate = false): f = [rc < 0]; later phases: saturate / empty the
+// arcs violating eps-optimality at the new eps
+template <bool SAT>
+__global__ void x_flow(XDev X, double eps) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  const double pv = X.pL[v];
+  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+    const double r = (X.c[s] + pv) - X.pR[X.nbr[s]];
+    if (!SAT) {
+      X.f[s] = r < 0.0 ? 1 : 0;
+    } else {
+      const uint8_t fs = X.f[s];
+      if (!fs && r < -eps) X.f[s] = 1;
+      else if (fs && r > eps) X.f[s] = 0;
+    }
+  }
+}
+
+__global__ void x_zero_ring(int32_t* cnt) {
+  if (threadIdx.x < 2 * RING) cnt[threadIdx.x] = 0;
+}
+
+// excesses of every node and the active lists of round r
+__global__ void x_excess(XDev X, int64_t r) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  int out = 0, in = 0;
+  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+    out += X.f[s];
+    in += X.f[X.mir[s]];
+  }
+  const int el = 2 - out, er = in - 2;
+  X.eL[v] = el;
+  X.eR[v] = er;
+  const int k = (int)(r % RING);
+  if (el > 0) ((r & 1) ? X.actL1 : X.actL0)[atomicAdd(&X.cnt[2 * k], 1)] = v;
+  if (er > 0) X.actR[atomicAdd(&X.cnt[2 * k + 1], 1)] = v;
+}
+
+// (key, slot) arg-max over the XG lanes of a node group: larger key wins,
+// equal keys go to the lower slot.  A total order, so any reduction tree
+// returns the same pair.
+__device__ __forceinline__ void group_best(double& k, int& s) {
+#pragma unroll
+  for (int off = XG / 2; off > 0; off >>= 1) {
+    const double k2 = __shfl_xor(k, off, XG);
+    const int s2 = __shfl_xor(s, off, XG);
+    if (k2 > k || (k2 == k && s2 < s)) { k = k2; s = s2; }
+  }
+}
+
+// best residual arc of L_v: max pR[w] - c over slots with f == 0
+__device__ __forceinline__ void l_best(const XDev& X, int b, int e, int lane, double& bv, int& bs) {
+  bv = -INFINITY;
+  bs = INT_MAX;
+  for (int s = b + lane; s < e; s += XG) {
+    if (X.f[s]) continue;
+    const double val = X.pR[X.nbr[s]] - X.c[s];
+    if (val > bv || (val == bv && s < bs)) { bv = val; bs = s; }
+  }
+  group_best(bv, bs);
+}
+// costliest in-flow arc of R_w: max c + pL[v] over slots whose mirror has f == 1
+__device__ __forceinline__ void r_best(const XDev& X, int b, int e, int lane, double& bt, int& bs) {
+  bt = -INFINITY;
+  bs = INT_MAX;
+  for (int s = b + lane; s < e; s += XG) {
+    if (!X.f[X.mir[s]]) continue;
+    const double t = X.c[s] + X.pL[X.nbr[s]];
+    if (t > bt || (t == bt && s < bs)) { bt = t; bs = s; }
+  }
+  group_best(bt, bs);
+}
+
+__global__ void __launch_bounds__(XBS) x_lstep(XDev X, int64_t r, double eps) {
+  const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
+  if (blockIdx.x == 0 && threadIdx.x == 0) X.cnt[2 * k1] = 0;  // nL of round r+1, filled by R(r)
+  const int na = X.cnt[2 * k];
+  const int32_t* list = (r & 1) ? X.actL1 : X.actL0;
+  const int lane = threadIdx.x % XG;
+  const int groups = gridDim.x * (XBS / XG);
+  for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
+    const int v = list[i];
+    int e = X.eL[v];
+    if (e <= 0) continue;
+    const int b = X.rp[v], en = X.rp[v + 1];
+    double p = X.pL[v];
+    while (e > 0) {
+      double bv;
+      int bs;
+      l_best(X, b, en, lane, bv, bs);
+      if (bs == INT_MAX) break;
+      const int w = X.nbr[bs];
+      if (!((X.c[bs] + p) - X.pR[w] < 0.0)) p = bv - eps;
+      if ((bs - b) % XG == lane) {
+        X.f[bs] = 1;
+        if (atomicAdd(&X.eR[w], 1) == 0) X.actR[atomicAdd(&X.cnt[2 * k + 1], 1)] = w;
+      }
+      --e;
+    }
+    {
+      double bv;
+      int bs;
+      l_best(X, b, en, lane, bv, bs);
+      if (bs != INT_MAX && !((X.c[bs] + p) - X.pR[X.nbr[bs]] < 0.0)) p = bv - eps;
+    }
+    if (lane == 0) {
+      X.pL[v] = p;
+      X.eL[v] = e;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
+  const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
+  const int na = X.cnt[2 * k + 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    X.cnt[2 * k1 + 1] = 0;  // nR of round r+1, filled by L(r+1)
+    if (X.cnt[2 * k] + na > 0) atomicAdd(X.rounds, 1ull);
+  }
+  int32_t* next = (r & 1) ? X.actL0 : X.actL1;
+  const int lane = threadIdx.x % XG;
+  const int groups = gridDim.x * (XBS / XG);
+  for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
+    const int w = X.actR[i];
+    int e = X.eR[w];
+    if (e <= 0) continue;
+    const int b = X.rp[w], en = X.rp[w + 1];
+    double p = X.pR[w];
+    while (e > 0) {
+      double bt;
+      int bs;
+      r_best(X, b, en, lane, bt, bs);
+      if (bs == INT_MAX) break;
+      if (!(bt - p > 0.0)) p = bt - eps;
+      if ((bs - b) % XG == lane) {
+        const int v = X.nbr[bs];
+        X.f[X.mir[bs]] = 0;
+        if (atomicAdd(&X.eL[v], 1) == 0) next[atomicAdd(&X.cnt[2 * k1], 1)] = v;
+      }
+      --e;
+    }
+    {
+      double bt;
+      int bs;
+      r_best(X, b, en, lane, bt, bs);
+      if (bs != INT_MAX && !(bt - p > 0.0)) p = bt - eps;
+    }
+    if (lane == 0) {
+      X.pR[w] = p;
+      X.eR[w] = e;
+    }
+  }
+}
+
+// per-node terms of P = sum c x and D = 2 sum y - sum max(0, y_u + y_w - c)
+__global__ void x_cert_terms(XDev X, double* __restrict__ pv, double* __restrict__ dv) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  const double yv = 0.5 * (X.pR[v] - X.pL[v]);
+  double a = 0.0, bsum = 0.0;
+  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+    const int w = X.nbr[s];
+    if (w <= v) continue;
+    const double x = 0.5 * (double)(X.f[s] + X.f[X.mir[s]]);
+    a += X.c[s] * x;
+    const double yw = 0.5 * (X.pR[w] - X.pL[w]);
+    bsum += fmax(0.0, (yv + yw) - X.c[s]);
+  }
+  pv[v] = a;
+  dv[v] = 2.0 * yv - bsum;
+}
+// fixed-order sums: DET_T contiguous chunks, each summed in order, then the
+// partials in order (one block)
+__global__ void __launch_bounds__(DET_T) x_det_sum(int n, const double* __restrict__ pv,
+                                                   const double* __restrict__ dv, double* __restrict__ out) {
+  __shared__ double sp[DET_T], sd[DET_T];
+  const int t = threadIdx.x;
+  const int64_t chunk = ((int64_t)n + DET_T - 1) / DET_T;
+  const int64_t b = t * chunk;
+  const int64_t e = b + chunk < n ? b + chunk : n;
+  double a = 0.0, d = 0.0;
+  for (int64_t i = b; i < e; ++i) {
+    a += pv[i];
+    d += dv[i];
+  }
+  sp[t] = a;
+  sd[t] = d;
+  __syncthreads();
+  if (t == 0) {
+    double P = 0.0, D = 0.0;
+    for (int k = 0; k < DET_T; ++k) {
+      P += sp[k];
+      D += sd[k];
+    }
+    out[0] = P;
+    out[1] = D;
+  }
+}
+
+// outputs: x and its X-flag per canonical slot, F = (pL - pR) / d_min
+__global__ void x_out(XDev X, double dmin, double* __restrict__ xc, uint8_t* __restrict__ fc,
+                      double* __restrict__ F) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+    const double x = 0.5 * (double)(X.f[s] + X.f[X.mir[s]]);
+    xc[s] = x;
+    fc[s] = x > 0.0 ? 1 : 0;
+  }
+  F[v] = (X.pL[v] - X.pR[v]) / dmin;
+}
+__global__ void x_uf_init(int n, int32_t* par) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) par[v] = v;
+}
+__global__ void x_half_union(XDev X, int32_t* par) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+    const int w = X.nbr[s];
+    if (w > v && X.f[s] + X.f[X.mir[s]] == 1) uf_unite(par, v, w);
+  }
+}
+__global__ void x_half_root(XDev X, int32_t* par, int32_t* __restrict__ root, int32_t* __restrict__ nroots) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= X.n) return;
+  bool has = false;
+  for (int s = X.rp[v]; s < X.rp[v + 1] && !has; ++s) has = X.f[s] + X.f[X.mir[s]] == 1;
+  const int r = has ? uf_find(par, v) : -1;
+  root[v] = r;
+  if (r == v) atomicAdd(nroots, 1);
+}
+
+__global__ void x_zero_i32(int32_t* p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* rp, const int32_t* nbr,
+                       const double* cost, double dmin, const f2m_opts& o, double* F, double* xc, uint8_t* fc,
+                       int32_t* root, f2m_stats& z, int& rc) {
+  F2M_HIP(c, w.xmir.reserve(m * 4));
+  F2M_HIP(c, w.xf.reserve(m));
+  F2M_HIP(c, w.xp.reserve((int64_t)n * 8 * 4));  // pL, pR, certificate terms (2n)
+  F2M_HIP(c, w.xe.reserve((int64_t)n * 4 * 2));
+  F2M_HIP(c, w.xact.reserve((int64_t)n * 4 * 3));
+  F2M_HIP(c, w.xcnt.reserve(64 * 8));
+  XDev X;
+  X.n = n;
+  X.rp = rp;
+  X.nbr = nbr;
+  X.c = cost;
+  X.mir = w.xmir.as<int32_t>();
+  X.f = w.xf.as<uint8_t>();
+  X.pL = w.xp.as<double>();
+  X.pR = X.pL + n;
+  double* pv = X.pR + n;
+  double* dv = pv + n;
+  X.eL = w.xe.as<int32_t>();
+  X.eR = X.eL + n;
+  X.actL0 = w.xact.as<int32_t>();
+  X.actL1 = X.actL0 + n;
+  X.actR = X.actL1 + n;
+  X.cnt = w.xcnt.as<int32_t>();                  // [0, 8) ring
+  int32_t* flag = X.cnt + 8;                     // [8, 12) flags, [12] root count
+  X.rounds = reinterpret_cast<unsigned long long*>(X.cnt + 16);
+  double* sums = reinterpret_cast<double*>(X.cnt + 20);
+  const dim3 gn(grid_for(n, XBS)), bs(XBS);
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(x_zero_i32, dim3(1), dim3(64), 0, st, X.cnt, 20);
+  hipLaunchKernelGGL(x_mirror, gn, bs, 0, st, n, rp, nbr, cost, w.xmir.as<int32_t>(), flag);
+  F2M_LAUNCH_CHECK(c);
+  int32_t* hp = static_cast<int32_t*>(c->pinned);
+  F2M_HIP(c, hipMemcpyAsync(hp, flag, 12, hipMemcpyDeviceToHost, st));
+  F2M_HIP(c, hipStreamSynchronize(st));
+  z.exact_ran = 1;
+  if (hp[0]) { rc = -1; return F2M_OK; }
+  if (hp[1]) { rc = -2; return F2M_OK; }
+  const bool integral = hp[2] == 0;
+  hipLaunchKernelGGL(x_prices, gn, bs, 0, st, n, F, dmin, X.pL, X.pR);
+  hipLaunchKernelGGL((x_flow<false>), gn, bs, 0, st, X, 0.0);
+  F2M_LAUNCH_CHECK(c);
+  double eps = o.exact_eps0 * dmin;
+  int64_t r = 0;          // rounds launched (ring / parity index)
+  int64_t done = 0;       // rounds with an active node
+  int phases = 0;
+  bool stuck = false, certified = false;
+  double P = 0, D = 0;
+  struct Ring {
+    int32_t cnt[8];
+    int32_t flag[8];
+    unsigned long long rounds;
+  };
+  Ring* hr = static_cast<Ring*>(c->pinned);
+  auto cert = [&]() -> f2m_status {
+    hipLaunchKernelGGL(x_cert_terms, gn, bs, 0, st, X, pv, dv);
+    hipLaunchKernelGGL(x_det_sum, dim3(1), dim3(DET_T), 0, st, n, pv, dv, sums);
+    F2M_LAUNCH_CHECK(c);
+    double h[2];
+    F2M_HIP(c, hipMemcpyAsync(h, sums, 16, hipMemcpyDeviceToHost, st));
+    F2M_HIP(c, hipStreamSynchronize(st));
+    P = h[0];
+    D = h[1];
+    return F2M_OK;
+  };
+  for (;;) {
+    if (phases > 0) hipLaunchKernelGGL((x_flow<true>), gn, bs, 0, st, X, eps);
+    hipLaunchKernelGGL(x_zero_ring, dim3(1), dim3(64), 0, st, X.cnt);
+    hipLaunchKernelGGL(x_excess, gn, bs, 0, st, X, r);
+    F2M_LAUNCH_CHECK(c);
+    int chunk = 2;
+    for (;;) {
+      const int64_t left = o.exact_max_rounds - done;
+      if (left <= 0) { stuck = true; break; }
+      const int64_t kk = std::min<int64_t>(chunk, left);
+      for (int64_t i = 0; i < kk; ++i, ++r) {
+        hipLaunchKernelGGL(x_lstep, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+        hipLaunchKernelGGL(x_rstep, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+      }
+      F2M_LAUNCH_CHECK(c);
+      F2M_HIP(c, hipMemcpyAsync(hr, X.cnt, sizeof(Ring), hipMemcpyDeviceToHost, st));
+      F2M_HIP(c, hipStreamSynchronize(st));
+      done = (int64_t)hr->rounds;
+      const int kl = (int)((r - 1) % RING);
+      if (hr->cnt[2 * kl] + hr->cnt[2 * kl + 1] == 0) break;
+      chunk = std::min(chunk * 2, 64);
+    }
+    ++phases;
+    if (stuck) break;
+    f2m_status s = cert();
+    if (s) return s;
+    const double tol = std::max(integral ? 0.4 : 0.0, o.exact_tol * std::fabs(P));
+    if (P - D <= tol) { certified = true; break; }
+    eps = eps / (double)o.exact_alpha;
+    if (phases >= o.exact_max_phases) break;
+  }
+  if (stuck) {
+    f2m_status s = cert();
+    if (s) return s;
+  }
+  hipLaunchKernelGGL(x_out, gn, bs, 0, st, X, dmin, xc, fc, F);
+  int32_t* par = X.actL0;  // free now
+  hipLaunchKernelGGL(x_uf_init, gn, bs, 0, st, n, par);
+  hipLaunchKernelGGL(x_half_union, gn, bs, 0, st, X, par);
+  hipLaunchKernelGGL(x_zero_i32, dim3(1), dim3(64), 0, st, flag + 4, 1);
+  hipLaunchKernelGGL(x_half_root, gn, bs, 0, st, X, par, root, flag + 4);
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, hipMemcpyAsync(hp, flag + 4, 4, hipMemcpyDeviceToHost, st));
+  F2M_HIP(c, hipStreamSynchronize(st));
+  z.n_combs = hp[0];
+  z.exact_phases = phases;
+  z.exact_rounds = done;
+  z.exact_gap = P - D;
+  z.exact_certified = certified ? 1 : 0;
+  rc = certified ? 0 : 1;
+  return F2M_OK;
+}
+
+}  // namespace f2m

@@ -46,6 +46,7 @@ struct SolveWs {
   DevBuf sortk, sortv, sortk2, sortv2, sorttmp;  // comb node sort
   DevBuf ef, efoff;             // compacted active lists (16-byte slots) + offsets
   DevBuf psl, povo, ptmp;       // padded active lists (v*8 + k, overflow after n*8) + offsets
+  DevBuf xmir, xf, xp, xe, xact, xcnt;  // exact recovery: mirror slots, flows, prices, excesses, lists
   int cur = 0;                  // which F buffer holds the current potentials
   int64_t n = 0, m = 0;
 };
@@ -108,6 +109,15 @@ hipError_t scan_i32_to_i64(f2m_ctx* c, const int32_t* in, int64_t* out, int64_t 
 
 // Destroys the captured batched-launch plan of a context (f2m_batch.hip).
 void release_batch_plan(f2m_ctx* c);
+
+// Exact primal recovery (f2m_exact.hip).  From potentials F [dev,
+// normalised] on the canonical CSR, writes x, its
+// X-flags and the certifying F; root = components of the x = 1/2 edges;
+// fills the exact_* stats and n_combs.  rc: 0 certified, 1 not certified,
+// -1 asymmetric graph, -2 a node of degree below 2 (outputs untouched).
+f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* rp, const int32_t* nbr,
+                       const double* cost, double dmin, const f2m_opts& o, double* F, double* xc, uint8_t* fc,
+                       int32_t* root, f2m_stats& z, int& rc);
 
 inline int grid_for(int64_t threads, int block) { return (int)((threads + block - 1) / block); }
 

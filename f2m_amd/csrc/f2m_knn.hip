@@ -393,6 +393,26 @@ f2m_status f2m_knn_build(f2m_ctx* c, int32_t n, const double* x, const double* y
   return F2M_OK;
 }
 
+namespace {
+// Caller CSR checks: rowptr starts at 0, ends at nslots and never decreases;
+// neighbours are node ids other than the row's own.  Rows with bad bounds are
+// not scanned.
+__global__ void k_csr_check(int n, int64_t nslots, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
+                            int32_t* __restrict__ bad) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t b = rp[v], e = rp[v + 1];
+  bool ok = (v > 0 || b == 0) && (v < n - 1 || e == nslots) && b >= 0 && e >= b && e <= nslots;
+  if (ok)
+    for (int64_t s = b; s < e; ++s) {
+      const int w = nbr[s];
+      if (w < 0 || w >= n || w == v) { ok = false; break; }
+    }
+  if (!ok) atomicOr(bad, 1);
+}
+__global__ void k_zero1(int32_t* p) { *p = 0; }
+}  // namespace
+
 f2m_status f2m_graph_from_csr(f2m_ctx* c, int32_t n, const int32_t* rowptr, const int32_t* nbr,
                               const double* cost, int64_t nslots, f2m_graph* out) {
   if (!c || !rowptr || !nbr || !cost || !out || n < 1 || nslots < 0) return F2M_ERR_ARG;
@@ -404,7 +424,17 @@ f2m_status f2m_graph_from_csr(f2m_ctx* c, int32_t n, const int32_t* rowptr, cons
   F2M_HIP(c, hipMemcpyAsync(c->g_rowptr.p, rowptr, (n + 1) * 4ull, hipMemcpyDefault, c->stream));
   F2M_HIP(c, hipMemcpyAsync(c->g_nbr.p, nbr, nslots * 4ull, hipMemcpyDefault, c->stream));
   F2M_HIP(c, hipMemcpyAsync(c->g_cost.p, cost, nslots * 8ull, hipMemcpyDefault, c->stream));
+  F2M_HIP(c, c->k_red.reserve(256));
+  int32_t* bad = c->k_red.as<int32_t>();
+  hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, c->stream, bad);
+  hipLaunchKernelGGL(k_csr_check, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, nslots,
+                     c->g_rowptr.as<int32_t>(), c->g_nbr.as<int32_t>(), bad);
+  F2M_LAUNCH_CHECK(c);
+  int32_t* hp = static_cast<int32_t*>(c->pinned);
+  F2M_HIP(c, hipMemcpyAsync(hp, bad, 4, hipMemcpyDeviceToHost, c->stream));
   F2M_HIP(c, hipStreamSynchronize(c->stream));
+  if (hp[0]) return fail(c, F2M_ERR_ARG, "malformed CSR: rowptr must run 0..nslots without decreasing, "
+                                         "neighbours must be other node ids in [0, n)");
   out->n = n;
   out->k = 0;
   out->cost_mode = -1;

@@ -908,9 +908,33 @@ struct Runner {
     return F2M_OK;
   }
 
-  f2m_status finish(f2m_stats& z) {
+  // x and X-flags per canonical slot
+  f2m_status to_canon() {
     hipLaunchKernelGGL(s_to_canon, dim3(grid_for(m, BS)), dim3(BS), 0, c->stream, m, I(w.orig), w.xw.as<double>(),
                        fw(), w.xc.as<double>(), fc());
+    F2M_LAUNCH_CHECK(c);
+    return F2M_OK;
+  }
+  // exact recovery where the reference recovery is not exact (opts.exact 1:
+  // status 1 or an irregular component, whose x came from the float least
+  // squares), or always (2)
+  f2m_status exact(f2m_stats& z) {
+    const bool run = o.exact >= 2 || (o.exact == 1 && (z.status != 0 || z.n_irregular_a > 0));
+    if (!run) return F2M_OK;
+    f2m_stats t = z;
+    int rc = 0;
+    f2m_status s = exact_stage(c, w, n, m, rp, gnbr, gcost, dmin, o, F(), w.xc.as<double>(), fc(), I(w.root), t, rc);
+    if (s) return s;
+    z.exact_ran = t.exact_ran;
+    if (rc >= 0) {
+      z = t;
+      z.status = rc == 0 ? 0 : 1;
+    } else {
+      z.status = 1;
+    }
+    return F2M_OK;
+  }
+  f2m_status finish(f2m_stats& z) {
     hipLaunchKernelGGL(s_sums, dim3(RED_BLOCKS), dim3(BS), 0, c->stream, n, rp, gnbr, gcost, w.xc.as<double>(),
                        I(w.nbr), w.D.as<double>(), F(), part);
     hipLaunchKernelGGL(s_sums_final, dim3(1), dim3(BS), 0, c->stream, part, sums);
@@ -979,6 +1003,8 @@ f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opt
     z.max_grad = M;
   }
   z.status = irr_b == 0 ? 0 : 1;
+  if ((s = R.to_canon())) return s;
+  if ((s = R.exact(z))) return s;
   if ((s = R.finish(z))) return s;
   if ((s = R.harvest())) return s;
   z.sweeps = R.sweeps;

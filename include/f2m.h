@@ -67,6 +67,21 @@ typedef struct f2m_opts {
   int32_t fix_factor;     /* 500                           graph_cuda.cu:216        */
   int32_t frozen_sweeps;  /* 20                            graph_cuda_comb.hpp:437  */
   int32_t stop_phase;     /* 0 full; 1 after warm start; 2 after dual descent      */
+  /* Exact primal recovery (new; replaces the least squares + perturbation of
+   * comb_thread.cpp:272-304 / graph_cuda.cu:69-126 where they fail): warm-
+   * started cost-scaling push-relabel on the bipartite double cover, ending
+   * with a certified duality gap (below 1/2 for integer costs: exact).
+   * x is then in {0, 1/2, 1} with degree exactly 2. */
+  int32_t exact;          /* 1     0 never (the reference's output); 1 when the
+                                   reference recovery is not exact (status 1, or
+                                   an irregular tight component, whose x comes
+                                   from float least squares); 2 always           */
+  int32_t exact_alpha;    /* 4     eps divisor between phases                     */
+  double exact_eps0;      /* 0.02  first-phase eps, units of d_min                */
+  double exact_tol;       /* 1e-10 certified relative duality gap                 */
+  int64_t exact_max_rounds; /* 1e6 push/relabel rounds over all phases            */
+  int32_t exact_max_phases; /* 40                                                 */
+  int32_t exact_reserved;
 } f2m_opts;
 
 typedef struct f2m_stats {
@@ -84,6 +99,13 @@ typedef struct f2m_stats {
   int32_t status;         /* 0 "Optimal Found: Yes", 1 "ERROR: Optimal Solution
                              not Found" (graph_tour.hpp:218-241)                   */
   int32_t max_rank;       /* MAX edge rank (graph_tour.hpp:169-180)                */
+  int32_t exact_ran;      /* 1 if the exact recovery ran (x, F, comb_root are then
+                             its output: comb_root = components of x = 1/2 edges) */
+  int32_t exact_phases;   /* eps phases                                            */
+  int64_t exact_rounds;   /* push/relabel rounds with an active node               */
+  double exact_gap;       /* certified primal - dual gap, raw cost units           */
+  int32_t exact_certified;/* 1 if the gap is within tolerance                      */
+  int32_t exact_reserved;
 } f2m_stats;
 
 /* Device CSR graph (replaces graph + graph_c::init/init_cuda,
@@ -137,8 +159,8 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
  * With sample_every > 0, every sample_every-th sweep launch of f2m_solve is
  * bracketed by HIP events on the context stream; f2m_get_profile reports the
  * sampled average launch duration and the algorithmic bytes of all sweeps:
- * 12 B per active slot (nbr int32 + D double) + 24 B per node (rowptr,
- * length, F in, F out). */
+ * 12 B per active slot (D double + nbr int32) + 20 B per node (list
+ * offset, F in, F out). */
 typedef struct f2m_profile {
   int64_t sweep_launches;   /* sweeps executed by the last solve               */
   int64_t sweep_samples;    /* launches timed with events                      */

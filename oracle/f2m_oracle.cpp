@@ -544,6 +544,13 @@ void orc_default_opts(orc_opts* o) {
   o->fix_factor = 500;
   o->frozen_sweeps = 20;
   o->stop_phase = 0;
+  o->exact = 1;
+  o->exact_alpha = 4;
+  o->exact_eps0 = 0.02;
+  o->exact_tol = 1e-10;
+  o->exact_max_rounds = 1000000;
+  o->exact_max_phases = 40;
+  o->exact_reserved = 0;
 }
 
 double orc_cost(double xu, double yu, double xw, double yw, int mode) {
@@ -790,6 +797,26 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
       int64_t s = S.rp[v] + k;
       if (S.d_at(v, s) <= eps) inX[S.orig[s]] = 1;
     }
+  // exact recovery (f2m_exact_oracle.cpp) where the reference's recovery is
+  // not exact: status 1, or some tight component irregular (its x came from
+  // the float least squares, possibly outside [0,1])
+  const bool run_exact = o.exact >= 2 || (o.exact == 1 && (z.status != 0 || z.n_irregular_a > 0));
+  std::vector<int32_t> hroot;
+  if (run_exact) {
+    std::vector<double> Fx(n);
+    hroot.assign(n, -1);
+    const int rc = orc_exact(n, rowptr, nbr, cost, S.dmin, S.F.data(), &o, Fx.data(), xc.data(), hroot.data(), &z);
+    if (rc >= 0) {
+      z.status = rc == 0 ? 0 : 1;
+      S.F = Fx;
+      for (int64_t k = 0; k < m; ++k) inX[k] = xc[k] > 0.0;
+      z.n_combs = 0;
+      for (int v = 0; v < n; ++v) z.n_combs += hroot[v] == v;
+    } else {
+      hroot.clear();
+      z.status = 1;
+    }
+  }
   if (x_out) std::memcpy(x_out, xc.data(), sizeof(double) * m);
   // objective sum over undirected edges in canonical order (graph_tour.hpp:165-184)
   double obj = 0;
@@ -819,7 +846,7 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
   z.dual = S.dmin * (-sF + sneg);
   z.sweeps = S.sweeps;
   finish_F();
-  if (comb_root) std::memcpy(comb_root, S.root.data(), sizeof(int32_t) * n);
+  if (comb_root) std::memcpy(comb_root, hroot.empty() ? S.root.data() : hroot.data(), sizeof(int32_t) * n);
   if (st) *st = z;
   return 0;
 }

@@ -48,6 +48,16 @@ typedef struct orc_opts {
   int32_t fix_factor;     /* 500    graph_cuda.cu:216                        */
   int32_t frozen_sweeps;  /* 20     graph_cuda_comb.hpp:437                  */
   int32_t stop_phase;     /* 0 full; 1 after block_GLOB; 2 after dual descent */
+  /* exact primal recovery (no reference counterpart, f2m_exact_oracle.cpp) */
+  int32_t exact;          /* 0 never; 1 when the reference recovery is not
+                             exact (status 1 or an irregular tight
+                             component); 2 always                            */
+  int32_t exact_alpha;    /* 4      eps divisor between phases               */
+  double exact_eps0;      /* 0.02   first-phase eps, units of d_min          */
+  double exact_tol;       /* 1e-10  certified relative duality gap           */
+  int64_t exact_max_rounds; /* 1e6  push/relabel rounds over all phases      */
+  int32_t exact_max_phases; /* 40                                           */
+  int32_t exact_reserved;
 } orc_opts;
 
 typedef struct orc_stats {
@@ -64,6 +74,12 @@ typedef struct orc_stats {
   int32_t n_repairs;      /* perturbation attempts                               */
   int32_t status;         /* 0 optimum found, 1 ERROR (graph_tour.hpp:236)       */
   int32_t max_rank;       /* MAX edge rank (graph_tour.hpp:169-180)              */
+  int32_t exact_ran;      /* 1 if the exact recovery ran                         */
+  int32_t exact_phases;   /* eps phases                                          */
+  int64_t exact_rounds;   /* push/relabel rounds with an active node             */
+  double exact_gap;       /* certified P - D, raw units                          */
+  int32_t exact_certified;/* gap within tolerance                                */
+  int32_t exact_reserved;
 } orc_stats;
 
 void orc_default_opts(orc_opts* o);
@@ -102,6 +118,17 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
  * x_out[nv]; returns iterations used (go_max) or 0 if not converged. */
 int orc_axb(int m, const int32_t* row_ptr, const int32_t* col, const int32_t* b,
             int nv, double* x_out, double* max_err);
+
+/* --- exact primal recovery (f2m_exact_oracle.cpp; no reference counterpart) --
+ * Warm-started cost-scaling push-relabel on the bipartite double cover from
+ * potentials F_in (normalised, as orc_solve's F).  Writes x per canonical
+ * slot, the certifying potentials F_out, half_root[v] = smallest id of v's
+ * component of x = 1/2 edges (-1 if none) and the exact_* stats.  Returns 0
+ * if certified, 1 if not, -1 on an asymmetric graph, -2 on a node of degree
+ * below 2. */
+int orc_exact(int n, const int32_t* rowptr, const int32_t* nbr, const double* cost, double dmin,
+              const double* F_in, const orc_opts* o, double* F_out, double* x_out, int32_t* half_root,
+              orc_stats* st);
 
 /* --- backward: dV/dp for V = sum x_e |p_u - p_w| (envelope theorem) -------- */
 void orc_bound_backward(int n, const int32_t* rowptr, const int32_t* nbr,

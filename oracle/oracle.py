@@ -52,6 +52,13 @@ class Opts(ctypes.Structure):
         ("fix_factor", ctypes.c_int32),
         ("frozen_sweeps", ctypes.c_int32),
         ("stop_phase", ctypes.c_int32),
+        ("exact", ctypes.c_int32),
+        ("exact_alpha", ctypes.c_int32),
+        ("exact_eps0", ctypes.c_double),
+        ("exact_tol", ctypes.c_double),
+        ("exact_max_rounds", ctypes.c_int64),
+        ("exact_max_phases", ctypes.c_int32),
+        ("exact_reserved", ctypes.c_int32),
     ]
 
 
@@ -70,6 +77,12 @@ class Stats(ctypes.Structure):
         ("n_repairs", ctypes.c_int32),
         ("status", ctypes.c_int32),
         ("max_rank", ctypes.c_int32),
+        ("exact_ran", ctypes.c_int32),
+        ("exact_phases", ctypes.c_int32),
+        ("exact_rounds", ctypes.c_int64),
+        ("exact_gap", ctypes.c_double),
+        ("exact_certified", ctypes.c_int32),
+        ("exact_reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -105,6 +118,9 @@ def _setup(L):
     L.orc_axb.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double)]
     L.orc_axb.restype = ctypes.c_int
     L.orc_bound_backward.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, _P, _P]
+    L.orc_exact.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Opts), _P, _P, _P,
+                            ctypes.POINTER(Stats)]
+    L.orc_exact.restype = ctypes.c_int
 
 
 def default_opts(**kw) -> Opts:
@@ -198,3 +214,22 @@ def bound_backward(rowptr, nbr, x_slot, px, py):
                              _ptr(np.ascontiguousarray(x_slot, np.float64)), _ptr(np.ascontiguousarray(px, np.float64)),
                              _ptr(np.ascontiguousarray(py, np.float64)), _ptr(gx), _ptr(gy))
     return gx, gy
+
+
+def exact(rowptr, nbr, cst, F, d_min=None, opts: Opts | None = None):
+    """Exact recovery from potentials F (normalised units): returns rc and a
+    dict with x per canonical slot, the certifying F, half_root and stats."""
+    n = len(rowptr) - 1
+    m = len(nbr)
+    cst = np.ascontiguousarray(cst, np.float64)
+    if d_min is None:
+        d_min = float(cst[cst > 0].min())
+    o = opts or default_opts()
+    Fo = np.zeros(n)
+    x = np.zeros(m)
+    hr = np.zeros(n, np.int32)
+    st = Stats()
+    rc = lib().orc_exact(n, _ptr(np.ascontiguousarray(rowptr, np.int32)), _ptr(np.ascontiguousarray(nbr, np.int32)),
+                         _ptr(cst), d_min, _ptr(np.ascontiguousarray(F, np.float64)), ctypes.byref(o), _ptr(Fo),
+                         _ptr(x), _ptr(hr), ctypes.byref(st))
+    return rc, {"F": Fo, "x": x, "half_root": hr, "stats": st.as_dict()}

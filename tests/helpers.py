@@ -40,3 +40,17 @@ def sym_check(rowptr, nbr, x):
     ia = np.argsort(key_a)
     ib = np.argsort(key_b)
     return bool(np.array_equal(x[ia], x[ib]))
+
+
+def certificate(rowptr, nbr, cost, x, F, d_min):
+    """Primal value P = sum_e c_e x_e and the dual bound
+    D = 2 sum_v y_v - sum_e max(0, y_u + y_w - c_e) with y = -d_min F / 2,
+    recomputed in numpy.  Any y gives D <= LP optimum (weak duality)."""
+    n = len(rowptr) - 1
+    src = np.repeat(np.arange(n), np.diff(rowptr))
+    m = src < nbr
+    u, w, c, xe = src[m], nbr[m].astype(np.int64), cost[m], x[m]
+    y = -0.5 * d_min * np.asarray(F, np.float64)
+    P = float(np.dot(c, xe))
+    D = float(2 * y.sum() - np.maximum(0.0, y[u] + y[w] - c).sum())
+    return P, D

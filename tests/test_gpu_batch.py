@@ -43,7 +43,7 @@ def _slices(node_off, rowptr, i):
 
 @pytest.mark.parametrize("mode", [W.COST_NINT, W.COST_RAW_FP64])
 def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
-    from f2m_amd.solver import Context
+    from f2m_amd.solver import Context, default_opts
     xs, ys = batch_case
     say("batch build", len(xs), mode)
     b = ctx.knn_build_batched(xs, ys, 16, mode)
@@ -59,7 +59,7 @@ def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
         assert np.array_equal(rowptr[a:e + 1] - sa, rp) and np.array_equal(nbr[sa:se], nb)
         assert np.array_equal(cost[sa:se], c)
         say("plain solve", i, len(xs[i]))
-        st = plain.solve(g)
+        st = plain.solve(g, default_opts(exact=0))
         r = plain.result()
         sb = sts[i]
         for key in ("sweeps", "glob_rounds", "outer_rounds", "n_combs", "n_irregular_a", "n_irregular_b",
@@ -143,7 +143,7 @@ def _nondegenerate_fd(orc, x, y, K, u, h):
         xx = x.copy()
         xx[u] += sgn * h
         rp, nb, c = orc.graph_build(xx, y, K, W.COST_RAW_FP64)
-        r = orc.solve(rp, nb, c)
+        r = orc.solve(rp, nb, c, orc.default_opts(exact=0))
         key = (rp.tobytes(), nb.tobytes(), r["x"].tobytes())
         if ref is None:
             ref = key
@@ -186,7 +186,7 @@ def test_bound_backward_fp64_and_fd(orc):
         gget = np.concatenate([gx[a:e], gy[a:e]])
         assert np.linalg.norm(gget - gref) <= 1e-4 * np.linalg.norm(gref), i
         # objective agrees with an FP64 oracle solve of the same RAW_FP32 graph
-        r = orc.solve(rp, nbr[sa:se], cost[sa:se])
+        r = orc.solve(rp, nbr[sa:se], cost[sa:se], orc.default_opts(exact=0))
         assert abs(r["stats"]["objective"] - V[i].item()) <= 1e-9 * abs(r["stats"]["objective"])
     # central finite differences on the first instance (FP64 costs), eps = 1e-4
     x0, y0 = xs[0].astype(np.float64), ys[0].astype(np.float64)

@@ -64,18 +64,18 @@ def test_cli_files_and_optimum(orc, inst, K):
 
 
 def test_known_answer_sra104815_k50(ctx, orc, golden):
-    from f2m_amd.solver import read_tsplib
+    from f2m_amd.solver import default_opts, read_tsplib
     known = [k for k in golden["known"] if k["instance"] == "sra104815"][0]
     tmp = tempfile.mkdtemp()
     try:
         p = _unpack(tmp, "sra104815")
         pts = read_tsplib(p)
-        say("sra104815 K=50 solve")
+        say("sra104815 K=50 solve, reference recovery")
         g = ctx.knn_build(pts.x, pts.y, 50, 0)
-        st = ctx.solve(g)
-        res = ctx.result()
         rp, nb, c = ctx.graph_csr(g)
-        r = orc.solve(rp, nb, c)
+        st = ctx.solve(g, default_opts(exact=0))
+        res = ctx.result()
+        r = orc.solve(rp, nb, c, orc.default_opts(exact=0))
         # the GPU follows the oracle exactly, repairs included
         assert np.array_equal(res["F"], r["F"]) and np.array_equal(res["comb_root"], r["comb_root"])
         assert st["n_repairs"] == r["stats"]["n_repairs"] and st["status"] == r["stats"]["status"] == 0
@@ -84,5 +84,17 @@ def test_known_answer_sra104815_k50(ctx, orc, golden):
         # the reference's published value is a least-squares artefact as well
         # (x outside [0,1]); ours lies within 2e-6 relative of it
         assert abs(st["objective"] - known["reference_opt"]) <= 2e-6 * known["reference_opt"]
+        assert res["x"].min() < 0 or res["x"].max() > 1
+        # default: exact recovery, the LP optimum of HiGHS on the same edges
+        say("sra104815 K=50 solve, exact recovery")
+        st = ctx.solve(g)
+        res = ctx.result()
+        r = orc.solve(rp, nb, c)
+        for k in ("exact_ran", "exact_phases", "exact_rounds", "exact_certified", "exact_gap", "status"):
+            assert st[k] == r["stats"][k], k
+        assert np.array_equal(res["x"], r["x"]) and np.array_equal(res["F"], r["F"])
+        opt = known["lp_opt_same_edges"]
+        assert st["exact_certified"] == 1 and abs(st["objective"] - opt) <= 1e-9 * opt
+        assert res["x"].min() == 0.0 and res["x"].max() == 1.0
     finally:
         shutil.rmtree(tmp)

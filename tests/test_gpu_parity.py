@@ -120,9 +120,11 @@ def _full_compare(ctx, orc, x, y, K, mode, name, lp_opt=None, exact_x=True):
     assert abs(st["objective"] - so["objective"]) <= 1e-9 * abs(so["objective"])
     assert abs(st["dual"] - so["dual"]) <= 1e-9 * abs(so["dual"])
     assert st["max_rank"] == so["max_rank"]
+    for k in ("exact_ran", "exact_phases", "exact_rounds", "exact_certified", "exact_gap"):
+        assert st[k] == so[k], (name, k, st[k], so[k])
     if lp_opt is not None:
         assert abs(st["objective"] - lp_opt) <= 1e-9 * abs(lp_opt)
-    if exact_x and so["n_irregular_a"] == 0:
+    if so["exact_ran"] or (exact_x and so["n_irregular_a"] == 0):
         assert np.array_equal(res["x"], r["x"])
     else:
         # irregular components: least squares on the GPU sums in another order
