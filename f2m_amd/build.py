@@ -9,6 +9,7 @@ from __future__ import annotations
 import concurrent.futures as cf
 import glob
 import os
+import re
 import subprocess
 import sys
 
@@ -35,6 +36,59 @@ def _obj(src):
     return os.path.join(BUILD, os.path.basename(src) + ".o")
 
 
+def _asm(src):
+    return os.path.join(BUILD, os.path.basename(src) + ".gfx950.s")
+
+
+_LABEL = re.compile(r"^(\.LBB\S+:|; %bb\.\d+:|_Z\S+:)")
+_SPILL = re.compile(r"(Folded Spill|Folded Reload|Spill$|Reload$)")
+
+
+def lost_spills(asm_text: str):
+    """Spill stores/reloads that execute before their block's exec restore.
+
+    A join block of divergent code starts with the narrowed exec mask of the
+    branch it ends and restores the full mask with `s_or_b64 exec, exec, ...`.
+    A spill placed ahead of that restore runs for the narrowed (often empty)
+    set of lanes, so the other lanes' values are lost; the compiler has been
+    seen to place spills there.  Returns (kernel, line number, instruction)
+    for every such spill."""
+    out = []
+    kernel = None
+    block = []
+    def flush():
+        restore = next((i for i, (_, t) in enumerate(block) if t.startswith("s_or_b64 exec, exec")), None)
+        if restore is not None:
+            for ln, t in block[:restore]:
+                if _SPILL.search(t) and t.startswith(("scratch_", "buffer_", "v_accvgpr")):
+                    out.append((kernel, ln, t))
+    for ln, raw in enumerate(asm_text.split("\n"), 1):
+        line = raw.strip()
+        m = _LABEL.match(line)
+        if m:
+            flush()
+            block = []
+            if line.startswith("_Z"):
+                kernel = line.split(":")[0]
+            continue
+        if line and not line.startswith((";", ".")):
+            block.append((ln, line))
+    flush()
+    return out
+
+
+def check_isa(srcs):
+    """Fail the build when a device spill can run with lanes masked off."""
+    bad = []
+    for s in srcs:
+        with open(_asm(s)) as f:
+            bad += [(os.path.basename(s),) + b for b in lost_spills(f.read())]
+    if bad:
+        msg = "\n".join(f"  {f}: {k} line {ln}: {t}" for f, k, ln, t in bad[:20])
+        raise RuntimeError("device code places spills before an exec restore (values of masked lanes "
+                           "would be lost):\n" + msg)
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     hip_srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -42,12 +96,14 @@ def build(verbose: bool = False) -> str:
     jobs = []
     for s in hip_srcs:
         jobs.append([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", s, "-o", _obj(s)])
+        jobs.append([HIPCC, *COMMON, *DEVICE, "-x", "hip", "--cuda-device-only", "-S", s, "-o", _asm(s)])
     for s in cpp_srcs:
         jobs.append(["g++", *COMMON, "-c", s, "-o", _obj(s)])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout + r.stderr)
+    check_isa(hip_srcs)
     objs = [_obj(s) for s in hip_srcs + cpp_srcs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])

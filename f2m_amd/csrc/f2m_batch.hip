@@ -24,7 +24,7 @@
 #include <numeric>
 #include <vector>
 
-#include "f2m_device.h"
+#include "f2m_exact_device.h"
 
 namespace f2m {
 namespace {
@@ -54,6 +54,14 @@ struct BArgs {
   f2m_stats* stats;
   int32_t* cur_out;
   long long* clk;  // per-instance elapsed wall_clock64 ticks (autotuning), or null
+  // exact recovery: per slot mirror + flow; per node prices, excesses,
+  // lists, certificate terms (arrays of n_total at these bases)
+  int64_t n_total;
+  int32_t* xmir;
+  uint8_t* xf;
+  double* xp;      // pL | pR | P terms | D terms
+  int32_t* xe;     // eL | eR
+  int32_t* xact;   // L list | R list
 };
 
 // Instance view: local node ids, pointers offset to the instance.
@@ -73,6 +81,9 @@ struct Inst {
   uint8_t *fw, *fc;
   int32_t* efoff;
   ESlot* ef;
+  XState X;            // exact recovery state (instance-local ids and slots)
+  double *xpv, *xdv;   // certificate terms
+  int32_t *xactL, *xactR;
   __device__ __forceinline__ int64_t b(int v) const { return rp[v] - sb; }
 };
 
@@ -346,8 +357,180 @@ struct Sched {
   }
 };
 
+// Fixed-order sums of the certificate terms, equal to the single-graph
+// stage's: DET_T contiguous chunks, each summed in order from 0, then the
+// chunk sums in order from 0.  xdet: 2*BS doubles of shared memory.
+template <int BS>
+__device__ void wg_det_sum(const double* pv, const double* dv, int n, double* xdet, double& P, double& D) {
+  const int64_t chunk = ((int64_t)n + DET_T - 1) / DET_T;
+  double accP = 0.0, accD = 0.0;
+  for (int k0 = 0; k0 < DET_T; k0 += BS) {
+    const int t = k0 + threadIdx.x;
+    double a = 0.0, d = 0.0;
+    if (t < DET_T) {
+      const int64_t b = t * chunk;
+      const int64_t e = b + chunk < n ? b + chunk : n;
+      for (int64_t i = b; i < e; ++i) {
+        a += pv[i];
+        d += dv[i];
+      }
+    }
+    xdet[threadIdx.x] = a;
+    xdet[BS + threadIdx.x] = d;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < BS && k0 + j < DET_T; ++j) {
+        accP += xdet[j];
+        accD += xdet[BS + j];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    xdet[0] = accP;
+    xdet[1] = accD;
+  }
+  __syncthreads();
+  P = xdet[0];
+  D = xdet[1];
+  __syncthreads();
+}
+
+// Exact primal recovery of one instance by its workgroup: the schedule of the
+// single-graph stage (f2m_exact.hip) with workgroup barriers for kernel
+// boundaries and shared-memory list counters, so rounds cost two barriers.
+// F: the instance's current potentials (normalised), overwritten with the
+// certifying ones; x / X-flags into I.xc / I.fc; root: x = 1/2 components.
+// Returns 0 certified, 1 not certified, -1 asymmetric, -2 degree below 2.
+template <int BS>
+__device__ int wg_exact(const Inst& I, const f2m_opts& o, double dmin, double* F, f2m_stats& z, double* sh,
+                        double* xdet, int* xcnt) {
+  const XState& X = I.X;
+  const LocalRP rpf{I.rp, I.sb};
+  const int lane = threadIdx.x % XG, gid = threadIdx.x / XG;
+  constexpr int NG = BS / XG;
+  if (threadIdx.x == 0) xcnt[4] = xcnt[5] = xcnt[6] = 0;
+  __syncthreads();
+  for (int v = threadIdx.x; v < I.n; v += BS) {
+    if (I.b(v + 1) - I.b(v) < 2) atomicOr(&xcnt[5], 1);
+    bool nonint = false;
+    if (!x_mirror_node(v, rpf, X.nbr, X.c, const_cast<int32_t*>(X.mir), nonint)) atomicOr(&xcnt[4], 1);
+    if (nonint) atomicOr(&xcnt[6], 1);
+  }
+  __syncthreads();
+  z.exact_ran = 1;
+  if (xcnt[4]) return -1;
+  if (xcnt[5]) return -2;
+  const bool integral = xcnt[6] == 0;
+  for (int v = threadIdx.x; v < I.n; v += BS) {
+    const double p = 0.5 * (dmin * F[v]);
+    X.pL[v] = p;
+    X.pR[v] = -p;
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < I.n; v += BS) x_flow_node<false>(X, v, (int)I.b(v), (int)I.b(v + 1), 0.0);
+  __syncthreads();
+  double eps = o.exact_eps0 * dmin;
+  int64_t r = 0, rounds = 0;
+  int phases = 0;
+  bool stuck = false, certified = false;
+  double P = 0.0, D = 0.0;
+  for (;;) {
+    if (phases > 0) {
+      for (int v = threadIdx.x; v < I.n; v += BS) x_flow_node<true, true>(X, v, (int)I.b(v), (int)I.b(v + 1), eps);
+      __syncthreads();
+    }
+    // excesses; lists of round r: counters xcnt[par] (L), xcnt[2 + par] (R)
+    if (threadIdx.x < 4) xcnt[threadIdx.x] = 0;
+    __syncthreads();
+    {
+      const int par = (int)(r & 1);
+      for (int v = threadIdx.x; v < I.n; v += BS) {
+        int el, er;
+        if (phases == 0) {
+          x_excess_node(X, v, (int)I.b(v), (int)I.b(v + 1), el, er);
+        } else {  // maintained through the flips of the saturation pass
+          el = X.eL[v];
+          er = X.eR[v];
+        }
+        if (el > 0) I.xactL[atomicAdd(&xcnt[par], 1)] = v;
+        if (er > 0) I.xactR[atomicAdd(&xcnt[2 + par], 1)] = v;
+      }
+    }
+    __syncthreads();
+    for (;;) {
+      if (rounds >= o.exact_max_rounds) { stuck = true; break; }
+      const int par = (int)(r & 1);
+      const int nl = xcnt[par];
+      if (threadIdx.x == 0) xcnt[par ^ 1] = 0;  // L list of round r+1, filled by the R step
+      for (int i = gid; i < nl; i += NG) {
+        const int v = I.xactL[i];
+        x_discharge_l(X, v, (int)I.b(v), (int)I.b(v + 1), lane, eps,
+                      [&](int w) { I.xactR[atomicAdd(&xcnt[2 + par], 1)] = w; });
+      }
+      __syncthreads();
+      const int nr = xcnt[2 + par];
+      if (threadIdx.x == 0) xcnt[2 + (par ^ 1)] = 0;  // R list of round r+1, filled by its L step
+      for (int i = gid; i < nr; i += NG) {
+        const int w = I.xactR[i];
+        x_discharge_r(X, w, (int)I.b(w), (int)I.b(w + 1), lane, eps,
+                      [&](int v) { I.xactL[atomicAdd(&xcnt[par ^ 1], 1)] = v; });
+      }
+      __syncthreads();
+      ++r;
+      if (nl + nr == 0) break;
+      ++rounds;
+    }
+    ++phases;
+    if (stuck) break;
+    for (int v = threadIdx.x; v < I.n; v += BS) x_cert_node(X, v, (int)I.b(v), (int)I.b(v + 1), I.xpv[v], I.xdv[v]);
+    __syncthreads();
+    wg_det_sum<BS>(I.xpv, I.xdv, I.n, xdet, P, D);
+    const double tol = fmax(integral ? 0.4 : 0.0, o.exact_tol * fabs(P));
+    if (P - D <= tol) { certified = true; break; }
+    eps = eps / (double)o.exact_alpha;
+    if (phases >= o.exact_max_phases) break;
+  }
+  if (stuck) {
+    for (int v = threadIdx.x; v < I.n; v += BS) x_cert_node(X, v, (int)I.b(v), (int)I.b(v + 1), I.xpv[v], I.xdv[v]);
+    __syncthreads();
+    wg_det_sum<BS>(I.xpv, I.xdv, I.n, xdet, P, D);
+  }
+  // outputs, then the x = 1/2 components (parents in I.aux)
+  for (int v = threadIdx.x; v < I.n; v += BS) {
+    for (int64_t s = I.b(v); s < I.b(v + 1); ++s) {
+      const double xv = 0.5 * (double)(X.f[s] + X.f[X.mir[s]]);
+      I.xc[s] = xv;
+      I.fc[s] = xv > 0.0 ? 1 : 0;
+    }
+    F[v] = (X.pL[v] - X.pR[v]) / dmin;
+    I.aux[v] = v;
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < I.n; v += BS)
+    for (int64_t s = I.b(v); s < I.b(v + 1); ++s) {
+      const int w = X.nbr[s];
+      if (w > v && X.f[s] + X.f[X.mir[s]] == 1) uf_unite(I.aux, v, w);
+    }
+  __syncthreads();
+  double nroots = 0.0;
+  for (int v = threadIdx.x; v < I.n; v += BS) {
+    bool has = false;
+    for (int64_t s = I.b(v); s < I.b(v + 1) && !has; ++s) has = X.f[s] + X.f[X.mir[s]] == 1;
+    const int rt = has ? uf_find(I.aux, v) : -1;
+    I.root[v] = rt;
+    if (rt == v) nroots += 1.0;
+  }
+  z.n_combs = (int)block_sum_all(nroots, sh);
+  z.exact_phases = phases;
+  z.exact_rounds = rounds;
+  z.exact_gap = P - D;
+  z.exact_certified = certified ? 1 : 0;
+  return certified ? 0 : 1;
+}
+
 template <int BS, int LPN>
-__device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
+__device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, double* xdet, int* xcnt,
+                               double* sdmin) {
   Inst I;
   I.nb = a.node_off[inst];
   I.n = (int)(a.node_off[inst + 1] - I.nb);
@@ -379,6 +562,18 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   I.fc = a.fc + I.sb;
   I.efoff = a.efoff + I.nb + inst;
   I.ef = a.ef + I.sb;
+  I.X.nbr = I.gnbr;
+  I.X.c = I.gcost;
+  I.X.mir = a.xmir + I.sb;
+  I.X.f = a.xf + I.sb;
+  I.X.pL = a.xp + I.nb;
+  I.X.pR = a.xp + a.n_total + I.nb;
+  I.xpv = a.xp + 2 * a.n_total + I.nb;
+  I.xdv = a.xp + 3 * a.n_total + I.nb;
+  I.X.eL = a.xe + I.nb;
+  I.X.eR = a.xe + a.n_total + I.nb;
+  I.xactL = a.xact + I.nb;
+  I.xactR = a.xact + a.n_total + I.nb;
   const f2m_opts& o = a.o;
   f2m_stats z;
   memset(&z, 0, sizeof z);
@@ -397,6 +592,11 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
     }
     return;
   }
+  // d_min is kept in shared memory for its later uses: a register holding it
+  // across the whole solve gets spilled in the 1024-thread variants, and the
+  // compiler has placed such a spill store where no lane is active (before a
+  // join block's exec restore), losing it (the build scans the ISA for this)
+  if (threadIdx.x == 0) *sdmin = dmin;
   for (int64_t s = threadIdx.x; s < I.m; s += BS) {
     I.nbr[s] = I.gnbr[s];
     I.D[s] = I.gcost[s] / dmin;
@@ -451,13 +651,28 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
     z.max_grad = M;
   }
   z.status = irr_b == 0 ? 0 : 1;
-  // x to canonical slots, objective, dual, MAX edge rank
+  // x to canonical slots
   for (int64_t s = threadIdx.x; s < I.m; s += BS) {
     const int oo = I.orig[s];
     I.xc[oo] = I.xw[s];
     I.fc[oo] = I.fw[s];
   }
   __syncthreads();
+  // exact recovery where the reference recovery is not exact (opts.exact 1)
+  // or always (2), as in the single-graph solve
+  if (o.exact >= 2 || (o.exact == 1 && (z.status != 0 || z.n_irregular_a > 0))) {
+    f2m_stats t = z;
+    const int rc = wg_exact<BS>(I, o, *sdmin, I.F[S.cur], t, sh, xdet, xcnt);
+    if (rc >= 0) {
+      z = t;
+      z.status = rc == 0 ? 0 : 1;
+    } else {
+      z.exact_ran = 1;
+      z.status = 1;
+    }
+    __syncthreads();
+  }
+  // objective, dual, MAX edge rank
   const double* F = I.F[S.cur];
   double obj = 0.0, sf = 0.0, sneg = 0.0, mr = 0.0;
   for (int v = threadIdx.x; v < I.n; v += BS) {
@@ -484,7 +699,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi) {
   sneg = block_sum_all(sneg, sh);
   mr = block_max_all(mr, sh);
   z.objective = obj;
-  z.dual = dmin * (-sf + sneg);
+  z.dual = *sdmin * (-sf + sneg);
   z.max_rank = (int)mr;
   z.sweeps = S.sweeps;
   if (threadIdx.x == 0) {
@@ -498,6 +713,9 @@ template <int BS, int LPN>
 __global__ void __launch_bounds__(BS) b_persistent(BArgs a) {
   __shared__ double sh[BS / 64];
   __shared__ int shi[BS / 64 + 1];
+  __shared__ double xdet[2 * BS];
+  __shared__ int xcnt[8];
+  __shared__ double sdmin;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) shi[BS / 64] = atomicAdd(a.queue, 1);
@@ -505,7 +723,7 @@ __global__ void __launch_bounds__(BS) b_persistent(BArgs a) {
     const int idx = shi[BS / 64];
     if (idx >= a.n_order) break;
     const long long t0 = wall_clock64();
-    solve_instance<BS, LPN>(a, a.order[idx], sh, shi);
+    solve_instance<BS, LPN>(a, a.order[idx], sh, shi, xdet, xcnt, &sdmin);
     if (a.clk && threadIdx.x == 0) a.clk[a.order[idx]] = wall_clock64() - t0;
   }
 }
@@ -693,6 +911,13 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   F2M_HIP(c, c->b_stats.reserve(n_inst * sizeof(f2m_stats) + 64 * 4));
   F2M_HIP(c, c->b_order.reserve((n_inst + 64) * 4ull));
   F2M_HIP(c, c->b_cur.reserve((n_inst + 64) * 4ull));
+  if (oo.exact > 0) {
+    F2M_HIP(c, w.xmir.reserve(M * 4));
+    F2M_HIP(c, w.xf.reserve(M));
+    F2M_HIP(c, w.xp.reserve(N * 8 * 4));
+    F2M_HIP(c, w.xe.reserve(N * 4 * 2));
+    F2M_HIP(c, w.xact.reserve(N * 4 * 2));
+  }
 
   int ncu = 0;
   F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -782,6 +1007,12 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;
+  a.n_total = N;
+  a.xmir = w.xmir.as<int32_t>();
+  a.xf = w.xf.as<uint8_t>();
+  a.xp = w.xp.as<double>();
+  a.xe = w.xe.as<int32_t>();
+  a.xact = w.xact.as<int32_t>();
   a.clk = nullptr;
   if (c->b_clk_on) {
     F2M_HIP(c, c->b_clk.reserve((n_inst + 64) * 8ull));

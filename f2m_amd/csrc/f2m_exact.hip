@@ -39,28 +39,19 @@
 #include <cmath>
 #include <cstring>
 
-#include "f2m_device.h"
+#include "f2m_exact_device.h"
 
 namespace f2m {
 namespace {
 
 constexpr int XBS = 256;    // threads per block
-constexpr int XG = 16;      // lanes per node in the discharge steps
 constexpr int XGRID = 512;  // blocks of a discharge step
 constexpr int RING = 4;     // counter ring, rounds mod 4
-constexpr int DET_T = 1024; // threads of the fixed-order sum
 
 struct XDev {
+  XState S;
   int n;
   const int32_t* rp;
-  const int32_t* nbr;
-  const double* c;
-  const int32_t* mir;
-  uint8_t* f;
-  double* pL;
-  double* pR;
-  int32_t* eL;
-  int32_t* eR;
   int32_t* actL0;
   int32_t* actL1;
   int32_t* actR;
@@ -68,25 +59,19 @@ struct XDev {
   unsigned long long* rounds;    // rounds with an active node
 };
 
+struct RP32 {
+  const int32_t* rp;
+  __device__ __forceinline__ int64_t operator()(int v) const { return rp[v]; }
+};
+
 __global__ void x_mirror(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ nbr,
                          const double* __restrict__ c, int32_t* __restrict__ mir, int32_t* __restrict__ flag) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  const int b = rp[v], e = rp[v + 1];
-  if (e - b < 2) atomicOr(&flag[1], 1);
-  for (int s = b; s < e; ++s) {
-    const int w = nbr[s];
-    int q = rp[w];
-    const int qe = rp[w + 1];
-    while (q < qe && nbr[q] != v) ++q;
-    if (q == qe || c[q] != c[s]) {
-      atomicOr(&flag[0], 1);
-      mir[s] = s;
-    } else {
-      mir[s] = q;
-    }
-    if (c[s] != floor(c[s])) atomicOr(&flag[2], 1);
-  }
+  if (rp[v + 1] - rp[v] < 2) atomicOr(&flag[1], 1);
+  bool nonint = false;
+  if (!x_mirror_node(v, RP32{rp}, nbr, c, mir, nonint)) atomicOr(&flag[0], 1);
+  if (nonint) atomicOr(&flag[2], 1);
 }
 
 __global__ void x_prices(int n, const double* __restrict__ F, double dmin, double* __restrict__ pL,
@@ -98,81 +83,38 @@ __global__ void x_prices(int n, const double* __restrict__ F, double dmin, doubl
   pR[v] = -p;
 }
 
-// init (saturate = false): f = [rc < 0]; later phases: saturate / empty the
-// arcs violating eps-optimality at the new eps
+// init (SAT = false): f = [rc < 0]; later phases: saturate / empty the arcs
+// violating eps-optimality at the new eps, excesses updated with the flips
 template <bool SAT>
 __global__ void x_flow(XDev X, double eps) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= X.n) return;
-  const double pv = X.pL[v];
-  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
-    const double r = (X.c[s] + pv) - X.pR[X.nbr[s]];
-    if (!SAT) {
-      X.f[s] = r < 0.0 ? 1 : 0;
-    } else {
-      const uint8_t fs = X.f[s];
-      if (!fs && r < -eps) X.f[s] = 1;
-      else if (fs && r > eps) X.f[s] = 0;
-    }
-  }
+  if (v < X.n) x_flow_node<SAT, SAT>(X.S, v, X.rp[v], X.rp[v + 1], eps);
 }
 
 __global__ void x_zero_ring(int32_t* cnt) {
   if (threadIdx.x < 2 * RING) cnt[threadIdx.x] = 0;
 }
 
-// excesses of every node and the active lists of round r
+// excesses of every node (COUNT; else as maintained) and the active lists
+// of round r
+template <bool COUNT>
 __global__ void x_excess(XDev X, int64_t r) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= X.n) return;
-  int out = 0, in = 0;
-  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
-    out += X.f[s];
-    in += X.f[X.mir[s]];
+  int el, er;
+  if (COUNT) {
+    x_excess_node(X.S, v, X.rp[v], X.rp[v + 1], el, er);
+  } else {
+    el = X.S.eL[v];
+    er = X.S.eR[v];
   }
-  const int el = 2 - out, er = in - 2;
-  X.eL[v] = el;
-  X.eR[v] = er;
   const int k = (int)(r % RING);
   if (el > 0) ((r & 1) ? X.actL1 : X.actL0)[atomicAdd(&X.cnt[2 * k], 1)] = v;
   if (er > 0) X.actR[atomicAdd(&X.cnt[2 * k + 1], 1)] = v;
 }
 
-// (key, slot) arg-max over the XG lanes of a node group: larger key wins,
-// equal keys go to the lower slot.  A total order, so any reduction tree
-// returns the same pair.
-__device__ __forceinline__ void group_best(double& k, int& s) {
-#pragma unroll
-  for (int off = XG / 2; off > 0; off >>= 1) {
-    const double k2 = __shfl_xor(k, off, XG);
-    const int s2 = __shfl_xor(s, off, XG);
-    if (k2 > k || (k2 == k && s2 < s)) { k = k2; s = s2; }
-  }
-}
-
-// best residual arc of L_v: max pR[w] - c over slots with f == 0
-__device__ __forceinline__ void l_best(const XDev& X, int b, int e, int lane, double& bv, int& bs) {
-  bv = -INFINITY;
-  bs = INT_MAX;
-  for (int s = b + lane; s < e; s += XG) {
-    if (X.f[s]) continue;
-    const double val = X.pR[X.nbr[s]] - X.c[s];
-    if (val > bv || (val == bv && s < bs)) { bv = val; bs = s; }
-  }
-  group_best(bv, bs);
-}
-// costliest in-flow arc of R_w: max c + pL[v] over slots whose mirror has f == 1
-__device__ __forceinline__ void r_best(const XDev& X, int b, int e, int lane, double& bt, int& bs) {
-  bt = -INFINITY;
-  bs = INT_MAX;
-  for (int s = b + lane; s < e; s += XG) {
-    if (!X.f[X.mir[s]]) continue;
-    const double t = X.c[s] + X.pL[X.nbr[s]];
-    if (t > bt || (t == bt && s < bs)) { bt = t; bs = s; }
-  }
-  group_best(bt, bs);
-}
-
+// L step of round r: every listed left node discharges; right nodes whose
+// excess crosses 0 -> 1 join the R list of round r
 __global__ void __launch_bounds__(XBS) x_lstep(XDev X, int64_t r, double eps) {
   const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
   if (blockIdx.x == 0 && threadIdx.x == 0) X.cnt[2 * k1] = 0;  // nL of round r+1, filled by R(r)
@@ -180,38 +122,16 @@ __global__ void __launch_bounds__(XBS) x_lstep(XDev X, int64_t r, double eps) {
   const int32_t* list = (r & 1) ? X.actL1 : X.actL0;
   const int lane = threadIdx.x % XG;
   const int groups = gridDim.x * (XBS / XG);
+  int32_t* cr = &X.cnt[2 * k + 1];
+  int32_t* lr = X.actR;
   for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
     const int v = list[i];
-    int e = X.eL[v];
-    if (e <= 0) continue;
-    const int b = X.rp[v], en = X.rp[v + 1];
-    double p = X.pL[v];
-    while (e > 0) {
-      double bv;
-      int bs;
-      l_best(X, b, en, lane, bv, bs);
-      if (bs == INT_MAX) break;
-      const int w = X.nbr[bs];
-      if (!((X.c[bs] + p) - X.pR[w] < 0.0)) p = bv - eps;
-      if ((bs - b) % XG == lane) {
-        X.f[bs] = 1;
-        if (atomicAdd(&X.eR[w], 1) == 0) X.actR[atomicAdd(&X.cnt[2 * k + 1], 1)] = w;
-      }
-      --e;
-    }
-    {
-      double bv;
-      int bs;
-      l_best(X, b, en, lane, bv, bs);
-      if (bs != INT_MAX && !((X.c[bs] + p) - X.pR[X.nbr[bs]] < 0.0)) p = bv - eps;
-    }
-    if (lane == 0) {
-      X.pL[v] = p;
-      X.eL[v] = e;
-    }
+    x_discharge_l(X.S, v, X.rp[v], X.rp[v + 1], lane, eps, [&](int w) { lr[atomicAdd(cr, 1)] = w; });
   }
 }
 
+// R step of round r: every listed right node discharges; left nodes whose
+// excess crosses 0 -> 1 join the L list of round r+1
 __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
   const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
   const int na = X.cnt[2 * k + 1];
@@ -220,63 +140,26 @@ __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
     if (X.cnt[2 * k] + na > 0) atomicAdd(X.rounds, 1ull);
   }
   int32_t* next = (r & 1) ? X.actL0 : X.actL1;
+  int32_t* cl = &X.cnt[2 * k1];
   const int lane = threadIdx.x % XG;
   const int groups = gridDim.x * (XBS / XG);
   for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
     const int w = X.actR[i];
-    int e = X.eR[w];
-    if (e <= 0) continue;
-    const int b = X.rp[w], en = X.rp[w + 1];
-    double p = X.pR[w];
-    while (e > 0) {
-      double bt;
-      int bs;
-      r_best(X, b, en, lane, bt, bs);
-      if (bs == INT_MAX) break;
-      if (!(bt - p > 0.0)) p = bt - eps;
-      if ((bs - b) % XG == lane) {
-        const int v = X.nbr[bs];
-        X.f[X.mir[bs]] = 0;
-        if (atomicAdd(&X.eL[v], 1) == 0) next[atomicAdd(&X.cnt[2 * k1], 1)] = v;
-      }
-      --e;
-    }
-    {
-      double bt;
-      int bs;
-      r_best(X, b, en, lane, bt, bs);
-      if (bs != INT_MAX && !(bt - p > 0.0)) p = bt - eps;
-    }
-    if (lane == 0) {
-      X.pR[w] = p;
-      X.eR[w] = e;
-    }
+    x_discharge_r(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) { next[atomicAdd(cl, 1)] = v; });
   }
 }
 
-// per-node terms of P = sum c x and D = 2 sum y - sum max(0, y_u + y_w - c)
+// per-node terms of the duality gap certificate
 __global__ void x_cert_terms(XDev X, double* __restrict__ pv, double* __restrict__ dv) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= X.n) return;
-  const double yv = 0.5 * (X.pR[v] - X.pL[v]);
-  double a = 0.0, bsum = 0.0;
-  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
-    const int w = X.nbr[s];
-    if (w <= v) continue;
-    const double x = 0.5 * (double)(X.f[s] + X.f[X.mir[s]]);
-    a += X.c[s] * x;
-    const double yw = 0.5 * (X.pR[w] - X.pL[w]);
-    bsum += fmax(0.0, (yv + yw) - X.c[s]);
-  }
-  pv[v] = a;
-  dv[v] = 2.0 * yv - bsum;
+  if (v < X.n) x_cert_node(X.S, v, X.rp[v], X.rp[v + 1], pv[v], dv[v]);
 }
-// fixed-order sums: DET_T contiguous chunks, each summed in order, then the
-// partials in order (one block)
-__global__ void __launch_bounds__(DET_T) x_det_sum(int n, const double* __restrict__ pv,
-                                                   const double* __restrict__ dv, double* __restrict__ out) {
-  __shared__ double sp[DET_T], sd[DET_T];
-  const int t = threadIdx.x;
+// fixed-order sums: DET_T contiguous chunks, each summed in order (one
+// thread per chunk), then the chunk sums in order (one thread)
+constexpr int DET_B = 64;
+__global__ void __launch_bounds__(DET_B) x_det_part(int n, const double* __restrict__ pv,
+                                                    const double* __restrict__ dv, double* __restrict__ part) {
+  const int t = blockIdx.x * DET_B + threadIdx.x;
   const int64_t chunk = ((int64_t)n + DET_T - 1) / DET_T;
   const int64_t b = t * chunk;
   const int64_t e = b + chunk < n ? b + chunk : n;
@@ -285,14 +168,15 @@ __global__ void __launch_bounds__(DET_T) x_det_sum(int n, const double* __restri
     a += pv[i];
     d += dv[i];
   }
-  sp[t] = a;
-  sd[t] = d;
-  __syncthreads();
-  if (t == 0) {
+  part[t] = a;
+  part[DET_T + t] = d;
+}
+__global__ void x_det_final(const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
     double P = 0.0, D = 0.0;
     for (int k = 0; k < DET_T; ++k) {
-      P += sp[k];
-      D += sd[k];
+      P += part[k];
+      D += part[DET_T + k];
     }
     out[0] = P;
     out[1] = D;
@@ -305,11 +189,11 @@ __global__ void x_out(XDev X, double dmin, double* __restrict__ xc, uint8_t* __r
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= X.n) return;
   for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
-    const double x = 0.5 * (double)(X.f[s] + X.f[X.mir[s]]);
+    const double x = 0.5 * (double)(X.S.f[s] + X.S.f[X.S.mir[s]]);
     xc[s] = x;
     fc[s] = x > 0.0 ? 1 : 0;
   }
-  F[v] = (X.pL[v] - X.pR[v]) / dmin;
+  F[v] = (X.S.pL[v] - X.S.pR[v]) / dmin;
 }
 __global__ void x_uf_init(int n, int32_t* par) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,15 +203,15 @@ __global__ void x_half_union(XDev X, int32_t* par) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= X.n) return;
   for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
-    const int w = X.nbr[s];
-    if (w > v && X.f[s] + X.f[X.mir[s]] == 1) uf_unite(par, v, w);
+    const int w = X.S.nbr[s];
+    if (w > v && X.S.f[s] + X.S.f[X.S.mir[s]] == 1) uf_unite(par, v, w);
   }
 }
 __global__ void x_half_root(XDev X, int32_t* par, int32_t* __restrict__ root, int32_t* __restrict__ nroots) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= X.n) return;
   bool has = false;
-  for (int s = X.rp[v]; s < X.rp[v + 1] && !has; ++s) has = X.f[s] + X.f[X.mir[s]] == 1;
+  for (int s = X.rp[v]; s < X.rp[v + 1] && !has; ++s) has = X.S.f[s] + X.S.f[X.S.mir[s]] == 1;
   const int r = has ? uf_find(par, v) : -1;
   root[v] = r;
   if (r == v) atomicAdd(nroots, 1);
@@ -344,23 +228,24 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
                        int32_t* root, f2m_stats& z, int& rc) {
   F2M_HIP(c, w.xmir.reserve(m * 4));
   F2M_HIP(c, w.xf.reserve(m));
-  F2M_HIP(c, w.xp.reserve((int64_t)n * 8 * 4));  // pL, pR, certificate terms (2n)
+  F2M_HIP(c, w.xp.reserve((int64_t)n * 8 * 4 + 2 * DET_T * 8));  // pL, pR, terms (2n), chunk sums
   F2M_HIP(c, w.xe.reserve((int64_t)n * 4 * 2));
   F2M_HIP(c, w.xact.reserve((int64_t)n * 4 * 3));
   F2M_HIP(c, w.xcnt.reserve(64 * 8));
   XDev X;
   X.n = n;
   X.rp = rp;
-  X.nbr = nbr;
-  X.c = cost;
-  X.mir = w.xmir.as<int32_t>();
-  X.f = w.xf.as<uint8_t>();
-  X.pL = w.xp.as<double>();
-  X.pR = X.pL + n;
-  double* pv = X.pR + n;
+  X.S.nbr = nbr;
+  X.S.c = cost;
+  X.S.mir = w.xmir.as<int32_t>();
+  X.S.f = w.xf.as<uint8_t>();
+  X.S.pL = w.xp.as<double>();
+  X.S.pR = X.S.pL + n;
+  double* pv = X.S.pR + n;
   double* dv = pv + n;
-  X.eL = w.xe.as<int32_t>();
-  X.eR = X.eL + n;
+  double* part = dv + n;
+  X.S.eL = w.xe.as<int32_t>();
+  X.S.eR = X.S.eL + n;
   X.actL0 = w.xact.as<int32_t>();
   X.actL1 = X.actL0 + n;
   X.actR = X.actL1 + n;
@@ -380,7 +265,7 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   if (hp[0]) { rc = -1; return F2M_OK; }
   if (hp[1]) { rc = -2; return F2M_OK; }
   const bool integral = hp[2] == 0;
-  hipLaunchKernelGGL(x_prices, gn, bs, 0, st, n, F, dmin, X.pL, X.pR);
+  hipLaunchKernelGGL(x_prices, gn, bs, 0, st, n, F, dmin, X.S.pL, X.S.pR);
   hipLaunchKernelGGL((x_flow<false>), gn, bs, 0, st, X, 0.0);
   F2M_LAUNCH_CHECK(c);
   double eps = o.exact_eps0 * dmin;
@@ -397,7 +282,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   Ring* hr = static_cast<Ring*>(c->pinned);
   auto cert = [&]() -> f2m_status {
     hipLaunchKernelGGL(x_cert_terms, gn, bs, 0, st, X, pv, dv);
-    hipLaunchKernelGGL(x_det_sum, dim3(1), dim3(DET_T), 0, st, n, pv, dv, sums);
+    hipLaunchKernelGGL(x_det_part, dim3(DET_T / DET_B), dim3(DET_B), 0, st, n, pv, dv, part);
+    hipLaunchKernelGGL(x_det_final, dim3(1), dim3(64), 0, st, part, sums);
     F2M_LAUNCH_CHECK(c);
     double h[2];
     F2M_HIP(c, hipMemcpyAsync(h, sums, 16, hipMemcpyDeviceToHost, st));
@@ -409,7 +295,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   for (;;) {
     if (phases > 0) hipLaunchKernelGGL((x_flow<true>), gn, bs, 0, st, X, eps);
     hipLaunchKernelGGL(x_zero_ring, dim3(1), dim3(64), 0, st, X.cnt);
-    hipLaunchKernelGGL(x_excess, gn, bs, 0, st, X, r);
+    if (phases == 0) hipLaunchKernelGGL((x_excess<true>), gn, bs, 0, st, X, r);
+    else hipLaunchKernelGGL((x_excess<false>), gn, bs, 0, st, X, r);
     F2M_LAUNCH_CHECK(c);
     int chunk = 2;
     for (;;) {

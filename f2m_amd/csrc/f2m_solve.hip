@@ -26,6 +26,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "f2m_device.h"
 
 namespace f2m {
@@ -608,12 +610,19 @@ struct Runner {
       hipLaunchKernelGGL((s_sweep<LPN, false, true>), grid, block, 0, c->stream, n, eo, ef, fin, fout, nullptr, fz,
                          mslots);
   }
-  void launch_padded(bool write, bool frozen, unsigned long long* mslots) {
+  // ev0/ev1: when set, the runtime records them at the kernel's own start and
+  // end (hipExtLaunchKernelGGL), so sampled timings exclude event-marker latency
+  void launch_padded(bool write, bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
     dim3 grid(grid_for((int64_t)n * PLPN, BS)), block(BS);
     const uint8_t* fz = w.frozen.as<uint8_t>();
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, block, 0, c->stream, n, I(w.act), I(w.povo), w.psl.as<ESlot>(), F(), Fo(), fz,
-                         mslots);
+      if (ev0)
+        hipExtLaunchKernelGGL(kern, grid, block, 0, c->stream, ev0, ev1, 0, n, (const int32_t*)I(w.act),
+                              (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(), (const double*)F(), Fo(),
+                              fz, mslots);
+      else
+        hipLaunchKernelGGL(kern, grid, block, 0, c->stream, n, I(w.act), I(w.povo), w.psl.as<ESlot>(), F(), Fo(), fz,
+                           mslots);
     };
     if (!write) go(s_sweep_pad<false, false>);
     else if (frozen) go(s_sweep_pad<true, true>);
@@ -627,9 +636,10 @@ struct Runner {
         F2M_HIP(c, hipEventCreate(&e));
         c->prof_ev.push_back(e);
       }
-      F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
+      if (!padded) F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
     }
-    if (padded) launch_padded(write, frozen, mslots);
+    if (padded) launch_padded(write, frozen, mslots, timed ? c->prof_ev[2 * prof_used] : nullptr,
+                              timed ? c->prof_ev[2 * prof_used + 1] : nullptr);
     else switch (lpn) {
       case 1: launch_sweep_t<1>(write, frozen, mslots); break;
       case 2: launch_sweep_t<2>(write, frozen, mslots); break;
@@ -639,7 +649,7 @@ struct Runner {
     }
     F2M_LAUNCH_CHECK(c);
     if (timed) {
-      F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
+      if (!padded) F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
       ++prof_used;
     }
     if (write) {

@@ -157,8 +157,10 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
 /* ---- instrumentation (new; the reference prints gettimeofday deltas,
  * graph_cuda_define.hpp:29-34) -------------------------------------------------
  * With sample_every > 0, every sample_every-th sweep launch of f2m_solve is
- * bracketed by HIP events on the context stream; f2m_get_profile reports the
- * sampled average launch duration and the algorithmic bytes of all sweeps:
+ * timed by a pair of HIP events that the runtime records at the kernel's own
+ * start and end (hipExtLaunchKernelGGL, context stream); f2m_get_profile
+ * reports the sampled average kernel duration and the algorithmic bytes of
+ * all sweeps:
  * 12 B per active slot (D double + nbr int32) + 20 B per node (list
  * offset, F in, F out). */
 typedef struct f2m_profile {

@@ -1,10 +1,10 @@
 """GPU tests of the batched persistent solve and the differentiable bound.
 
 Bar: every instance of a batch gives the same graph, potentials, components,
-x and schedule counts as a plain f2m_solve of that instance, bit for bit
-(objective within 1e-12 relative: the per-instance sums are reduced in a
-different order); every lanes-per-node schedule gives identical bits; graph
-replays equal the eager result.  Backward: within 1e-4 relative of the FP64
+x and schedule counts as a plain f2m_solve of that instance, bit for bit,
+the exact recovery included (objective within 1e-12 relative: the
+per-instance sums are reduced in a different order); every lanes-per-node
+schedule gives identical bits; graph replays equal the eager result.  Backward: within 1e-4 relative of the FP64
 CPU gradient and of central finite differences (eps = 1e-4) on instances
 checked to be nondegenerate (same kNN graph and same x at +-eps).
 """
@@ -43,7 +43,7 @@ def _slices(node_off, rowptr, i):
 
 @pytest.mark.parametrize("mode", [W.COST_NINT, W.COST_RAW_FP64])
 def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
-    from f2m_amd.solver import Context, default_opts
+    from f2m_amd.solver import Context
     xs, ys = batch_case
     say("batch build", len(xs), mode)
     b = ctx.knn_build_batched(xs, ys, 16, mode)
@@ -59,14 +59,15 @@ def test_batch_matches_plain_solves(ctx, batch_case, mode, monkeypatch):
         assert np.array_equal(rowptr[a:e + 1] - sa, rp) and np.array_equal(nbr[sa:se], nb)
         assert np.array_equal(cost[sa:se], c)
         say("plain solve", i, len(xs[i]))
-        st = plain.solve(g, default_opts(exact=0))
+        st = plain.solve(g)
         r = plain.result()
         sb = sts[i]
         for key in ("sweeps", "glob_rounds", "outer_rounds", "n_combs", "n_irregular_a", "n_irregular_b",
-                    "status", "max_rank", "n_repairs"):
+                    "status", "max_rank", "n_repairs", "exact_ran", "exact_phases", "exact_rounds",
+                    "exact_certified", "exact_gap"):
             assert sb[key] == st[key], (i, key, sb[key], st[key])
         assert np.array_equal(res["F"][a:e], r["F"]), i
-        if st["n_irregular_a"] == 0:
+        if st["n_irregular_a"] == 0 or st["exact_ran"]:
             assert np.array_equal(res["x"][sa:se], r["x"]), i
         else:
             assert np.allclose(res["x"][sa:se], r["x"], atol=1e-6), i
@@ -143,7 +144,7 @@ def _nondegenerate_fd(orc, x, y, K, u, h):
         xx = x.copy()
         xx[u] += sgn * h
         rp, nb, c = orc.graph_build(xx, y, K, W.COST_RAW_FP64)
-        r = orc.solve(rp, nb, c, orc.default_opts(exact=0))
+        r = orc.solve(rp, nb, c)
         key = (rp.tobytes(), nb.tobytes(), r["x"].tobytes())
         if ref is None:
             ref = key
@@ -186,7 +187,7 @@ def test_bound_backward_fp64_and_fd(orc):
         gget = np.concatenate([gx[a:e], gy[a:e]])
         assert np.linalg.norm(gget - gref) <= 1e-4 * np.linalg.norm(gref), i
         # objective agrees with an FP64 oracle solve of the same RAW_FP32 graph
-        r = orc.solve(rp, nbr[sa:se], cost[sa:se], orc.default_opts(exact=0))
+        r = orc.solve(rp, nbr[sa:se], cost[sa:se])
         assert abs(r["stats"]["objective"] - V[i].item()) <= 1e-9 * abs(r["stats"]["objective"])
     # central finite differences on the first instance (FP64 costs), eps = 1e-4
     x0, y0 = xs[0].astype(np.float64), ys[0].astype(np.float64)
@@ -199,3 +200,38 @@ def test_bound_backward_fp64_and_fd(orc):
         assert abs(fd - gx[u]) <= 1e-4 * max(1.0, abs(fd)), (u, fd, gx[u])
     assert checked >= 3
     solver.close()
+
+
+def test_batch_exact_on_lattices(ctx, golden):
+    """Tie-heavy lattices: the reference recovery fails, every instance runs
+    the in-workgroup exact stage; bit for bit equal to plain solves, and the
+    n = 2000 lattice gives the HiGHS optimum exactly."""
+    from f2m_amd.solver import Context
+    sizes = (500, 2000, 1200, 3000, 800)
+    xs, ys = [], []
+    for n in sizes:
+        x, y, mode, K = W.c3(n=n)
+        xs.append(x)
+        ys.append(y)
+    say("batch lattices", sizes)
+    b = ctx.knn_build_batched(xs, ys, K, mode)
+    node_off, rowptr, nbr, cost = ctx.batch_csr(b)
+    sts = ctx.solve_batched(b)
+    res = ctx.batch_result(b)
+    plain = Context(0)
+    for i, n in enumerate(sizes):
+        a, e, sa, se = _slices(node_off, rowptr, i)
+        g = plain.knn_build(xs[i], ys[i], K, mode)
+        say("plain lattice", n)
+        st = plain.solve(g)
+        r = plain.result()
+        sb = sts[i]
+        assert sb["exact_ran"] == 1 and sb["exact_certified"] == 1 and sb["status"] == 0, (n, sb)
+        for key in ("sweeps", "n_combs", "exact_phases", "exact_rounds", "exact_gap", "max_rank"):
+            assert sb[key] == st[key], (n, key, sb[key], st[key])
+        assert np.array_equal(res["F"][a:e], r["F"]) and np.array_equal(res["x"][sa:se], r["x"]), n
+        assert sb["objective"] == st["objective"]  # half-integral x on integer costs: exact sums
+        if n == 2000:
+            opt = [l for l in golden["lattice"] if l["n"] == 2000][0]["lp_opt"]
+            assert sb["objective"] == opt
+    plain.close()
