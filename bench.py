@@ -6,8 +6,11 @@
 A step is one plain F2M solve (f2m_solve: dual descent with the reference
 schedule, tight components, primal recovery, FP64 objective) of the
 n=100,000 / K=20 clustered workload: 50-component Gaussian mixture, NINT
-costs, seed 0xF2A2.  The kNN graph is built once before the timed region,
-as the reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).
+costs, seed 0xF2A2.  Nodes are numbered along the Morton curve of the points
+(f2m_spatial_order, as the CLI does; the reference numbers nodes in hash
+order), and the kNN graph is built once before the timed region, as the
+reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).  The
+same solve in input order is timed as well (input_order).
 Prints one JSON line.  `value` = ms per solve (lower is better).
 
 roofline: the dominant kernel is the fused sweep (s_sweep).  Its algorithmic
@@ -61,6 +64,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB (counter calibration)")
     ap.add_argument("--no-fwd-bwd", action="store_true")
+    ap.add_argument("--no-input-order", action="store_true")
+    ap.add_argument("--batch-order", choices=["morton", "input"], default="morton")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -80,10 +85,15 @@ def main():
         log("calibration copy", args.calibrate_copy_mib, "MiB")
         del a, b
 
-    x, y, mode, K = W.c2()
-    n = len(x)
+    x0, y0, mode, K = W.c2()
+    n = len(x0)
     ctx = Context(0)
-    log("kNN graph build", n, K)
+    input_order = time_input_order(ctx, x0, y0, K, mode, args) if not args.no_input_order else None
+    log("spatial order + kNN graph build", n, K)
+    t = time.perf_counter()
+    perm = ctx.spatial_order(x0, y0)
+    x, y = x0[perm], y0[perm]
+    order_ms = (time.perf_counter() - t) * 1e3
     t = time.perf_counter()
     g = ctx.knn_build(x, y, K, mode)
     ctx.synchronize()
@@ -134,7 +144,7 @@ def main():
                          f"objective agrees: {ok}",
                "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1)}
 
-    batched = None if args.no_batched else bench_batched(ctx)
+    batched = None if args.no_batched else bench_batched(ctx, args.batch_order)
     fwd_bwd = None if args.no_fwd_bwd else bench_fwd_bwd()
 
     out = {
@@ -143,15 +153,32 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "plain F2M solve, n=100000 50-component Gaussian mixture, NINT costs, K=20",
                    "n": n, "k": K, "nslots": int(g.nslots), "sweeps": int(st["sweeps"]),
-                   "schedule": "reference", "parallelism": "single GPU"},
-        "solves_per_s": round(1e3 / ms, 3), "objective": st["objective"], "knn_build_ms": round(knn_ms, 3),
+                   "schedule": "reference", "node_order": "morton (f2m_spatial_order)",
+                   "recovery": "exact where the reference recovery is not", "parallelism": "single GPU"},
+        "solves_per_s": round(1e3 / ms, 3), "objective": st["objective"], "exact_ran": int(st["exact_ran"]),
+        "order_ms": round(order_ms, 3), "knn_build_ms": round(knn_ms, 3), "input_order": input_order,
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "fwd_bwd": fwd_bwd,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
 
 
-def bench_batched(ctx):
+def time_input_order(ctx, x, y, K, mode, args):
+    """The same solve with nodes in input order (for comparison)."""
+    g = ctx.knn_build(x, y, K, mode)
+    for _ in range(args.warmup):
+        ctx.solve(g)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = ctx.solve(g)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log("input order", ms, "ms/solve")
+    return {"ms": round(ms, 3), "objective": st["objective"], "sweeps": int(st["sweeps"])}
+
+
+def bench_batched(ctx, order):
     import numpy as np
 
     from f2m_amd import workloads as W
@@ -159,6 +186,10 @@ def bench_batched(ctx):
     insts, mode, K = W.c4()
     xs = [a for a, _ in insts]
     ys = [b for _, b in insts]
+    if order == "morton":
+        perms = ctx.spatial_order_batched(xs, ys)
+        xs = [a[p] for a, p in zip(xs, perms)]
+        ys = [b[p] for b, p in zip(ys, perms)]
     t = time.perf_counter()
     b = ctx.knn_build_batched(xs, ys, K, mode)
     knn_s = time.perf_counter() - t
@@ -183,7 +214,7 @@ def bench_batched(ctx):
     plain_ms = loop_s * 1e3 / len(idx)
     per_inst_ms = batched_s * 1e3 / len(xs)
     return {"workload": "4096 instances, n log-uniform in [100,20000], fp32 points x1e6, NINT costs, K=16",
-            "instances": len(xs), "sum_n": int(sum(len(a) for a in xs)), "nslots": int(b.nslots),
+            "node_order": order, "instances": len(xs), "sum_n": int(sum(len(a) for a in xs)), "nslots": int(b.nslots),
             "ms": round(batched_s * 1e3, 1), "per_instance_ms": round(per_inst_ms, 4),
             "plain_loop_per_instance_ms": round(plain_ms, 3),
             "speedup_per_instance": round(plain_ms / per_inst_ms, 2), "identical_optima_sampled": bool(same),

@@ -1,7 +1,14 @@
 // f2m_solver -- command-line F2M solver over the C ABI (counterpart of
 // RUN.cpp:12-31, `F2M.solver.exe <tsplib-file> <topK>`).
 //
-//   f2m_solver <tsplib-file> <topK> [--early-exit] [--quiet]
+//   f2m_solver <tsplib-file> <topK> [--early-exit] [--quiet] [--input-order]
+//                                  [--reference-recovery]
+//
+// Nodes are numbered along the Morton curve of the points (f2m_spatial_order;
+// --input-order keeps the input order), which only changes speed and, among
+// equal-cost candidates at the K-th rank, which one the kNN rule keeps.  The
+// exact recovery runs where the reference's recovery is not exact
+// (--reference-recovery reproduces the reference's output there instead).
 //
 // Writes next to the input, as the reference does (README.md:32-45):
 //   <file>.top<K>.F2M_solution  "label1 label2 x" per edge with x != 0, label1 <
@@ -38,15 +45,17 @@ static int die(f2m_ctx* ctx, f2m_status s, const char* what) {
 }
 
 int main(int argc, char** argv) {
-  bool early = false, quiet = false;
+  bool early = false, quiet = false, input_order = false, ref_recovery = false;
   std::vector<std::string> pos;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--early-exit")) early = true;
     else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
+    else if (!std::strcmp(argv[i], "--input-order")) input_order = true;
+    else if (!std::strcmp(argv[i], "--reference-recovery")) ref_recovery = true;
     else pos.push_back(argv[i]);
   }
   if (pos.size() != 2) {
-    std::cout << "usage: f2m_solver fileXY topK [--early-exit] [--quiet]\n"
+    std::cout << "usage: f2m_solver fileXY topK [--early-exit] [--quiet] [--input-order] [--reference-recovery]\n"
                  "fileXY == file in TSPLIB format\n"
                  "topK == the number of top K closest neighbors for each node\n";
     return 0;
@@ -78,6 +87,22 @@ int main(int argc, char** argv) {
 
   f2m_ctx* ctx = nullptr;
   if ((s = f2m_ctx_create(0, nullptr, &ctx))) return die(nullptr, s, "device");
+  // node v of the graph is input point perm[v]
+  std::vector<int32_t> perm(n);
+  for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+  if (!input_order) {
+    if ((s = f2m_spatial_order(ctx, (int32_t)n, x.data(), y.data(), perm.data()))) return die(ctx, s, "order");
+    std::vector<int64_t> l2(n);
+    std::vector<double> x2(n), y2(n);
+    for (int64_t i = 0; i < n; ++i) {
+      l2[i] = label[perm[i]];
+      x2[i] = x[perm[i]];
+      y2[i] = y[perm[i]];
+    }
+    label.swap(l2);
+    x.swap(x2);
+    y.swap(y2);
+  }
   f2m_graph g;
   if ((s = f2m_knn_build(ctx, (int32_t)n, x.data(), y.data(), K, F2M_COST_RAW_FP64, &g)))
     return die(ctx, s, "kNN graph");
@@ -86,11 +111,14 @@ int main(int argc, char** argv) {
   if ((s = f2m_graph_copy(ctx, &g, rowptr.data(), nbr.data(), cost.data()))) return die(ctx, s, "graph copy");
 
   const std::string stem = file + ".top" + std::to_string(K);
-  {  // .DM: kept edges in increasing (cost, v1, v2)
+  {  // .DM: kept edges in increasing (cost, v1, v2), input point indices
     std::vector<std::tuple<double, int32_t, int32_t>> e;
     for (int64_t v = 0; v < n; ++v)
       for (int32_t q = rowptr[v]; q < rowptr[v + 1]; ++q)
-        if (nbr[q] > v) e.emplace_back(cost[q], (int32_t)v, nbr[q]);
+        if (nbr[q] > v) {
+          const int32_t a = perm[v], b = perm[nbr[q]];
+          e.emplace_back(cost[q], std::min(a, b), std::max(a, b));
+        }
     std::sort(e.begin(), e.end());
     std::ofstream f(stem + ".DM");
     for (auto& t : e) f << std::get<1>(t) << " " << std::get<2>(t) << " " << std::get<0>(t) << "\n";
@@ -99,6 +127,7 @@ int main(int argc, char** argv) {
   f2m_opts o;
   f2m_default_opts(&o);
   o.early_exit = early ? 1 : 0;
+  if (ref_recovery) o.exact = 0;
   f2m_stats st;
   auto t0 = std::chrono::steady_clock::now();
   if ((s = f2m_solve(ctx, &g, &o, &st))) return die(ctx, s, "solve");

@@ -263,6 +263,7 @@ void f2m_ctx_destroy(f2m_ctx* c) {
                         &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk, &c->b_ef, &c->b_efoff};
   for (auto* b : all) b->release();
   for (auto& s : c->stage) s.release();
+  for (auto& s : c->o_buf) s.release();
   release_ws(c->ws);
   release_ws(c->bws);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);

@@ -65,6 +65,8 @@ struct f2m_ctx {
       k_deg, k_red, k_scan;
   // staging of host arrays
   f2m::DevBuf stage[6];
+  // spatial ordering scratch (f2m_order.hip)
+  f2m::DevBuf o_buf[5];
   f2m::SolveWs ws;
   // batched storage
   f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc, b_ef, b_efoff;

@@ -180,33 +180,38 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
 // then neighbour potentials) for every node with at most PP active slots.
 // (Pushing potentials into per-slot copies instead, to save the second
 // level, was measured slower: the scattered copy stores cost more than the
-// gather they replace.)
-constexpr int PP = 8;    // padded active slots per node
-constexpr int PLPN = 4;  // lanes per node of the padded sweep (PP = 2 * PLPN)
+// gather they replace.)  Stride and lanes per node are selectable
+// (F2M_PAD="stride,lanes[,g]"; g: guarded slot loads); default 4,2, the
+// fastest measured on the n = 100k Gaussian-mixture graph (5.6 us per sweep
+// against 6.2 us for 8,4, 5.8 us for 4,4 and 2,2, 6.5 us for 2,1).
+constexpr int PP_MAX = 8;  // largest padded stride (workspace sizing)
 
 __global__ void s_excess(int n, const int32_t* __restrict__ a, int p, int32_t* __restrict__ out) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) out[v] = a[v] > p ? a[v] - p : 0;
 }
-__global__ void s_pad_fill(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
+__global__ void s_pad_fill(int n, int pp, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                            const int32_t* __restrict__ ovo, const int32_t* __restrict__ nbr,
                            const double* __restrict__ D, ESlot* __restrict__ sl) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int b = rp[v], L = act[v], oo = ovo[v];
   for (int k = 0; k < L; ++k) {
-    const int64_t idx = k < PP ? (int64_t)v * PP + k : (int64_t)n * PP + oo + (k - PP);
+    const int64_t idx = k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + oo + (k - pp);
     sl[idx] = ESlot{D[b + k], nbr[b + k], 0};
   }
 }
 
-template <bool FROZEN, bool WRITE_F>
+// PPT padded slots per node, LPN lanes per node (PPT / LPN slots each); with
+// SPEC the padded slots are loaded before act[v] is known, else only those
+// below act[v].  Every variant selects the same three values: equal bits.
+template <int PPT, int LPN, bool SPEC, bool FROZEN, bool WRITE_F>
 __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restrict__ act,
                                                   const int32_t* __restrict__ ovo, const ESlot* __restrict__ sl,
                                                   const double* __restrict__ Fin, double* __restrict__ Fout,
                                                   const uint8_t* __restrict__ frozen,
                                                   unsigned long long* __restrict__ maxslots) {
-  constexpr int LPN = PLPN;
+  constexpr int R = PPT / LPN;
   const int gt = blockIdx.x * BS + threadIdx.x;
   const int v = gt / LPN, lane = gt % LPN;
   Top3 t;
@@ -216,12 +221,16 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
     const int L = act[v];
     const int oo = ovo[v];
     Fv = Fin[v];
-    const int64_t p = (int64_t)v * PP + lane;  // padded entries: loaded unconditionally
-    const ESlot s0 = sl[p], s1 = sl[p + LPN];
-    if (lane < L) t.push(dval(s0.D, Fv, Fin[s0.nbr], v, s0.nbr));
-    if (lane + LPN < L) t.push(dval(s1.D, Fv, Fin[s1.nbr], v, s1.nbr));
-    for (int k = lane + PP; k < L; k += LPN) {
-      const ESlot sk = sl[(int64_t)n * PP + oo + (k - PP)];
+    const int64_t p = (int64_t)v * PPT + lane;
+    ESlot sr[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (SPEC || lane + j * LPN < L) sr[j] = sl[p + j * LPN];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (lane + j * LPN < L) t.push(dval(sr[j].D, Fv, Fin[sr[j].nbr], v, sr[j].nbr));
+    for (int k = lane + PPT; k < L; k += LPN) {
+      const ESlot sk = sl[(int64_t)n * PPT + oo + (k - PPT)];
       t.push(dval(sk.D, Fv, Fin[sk.nbr], v, sk.nbr));
     }
   }
@@ -518,6 +527,8 @@ struct Runner {
   int64_t sweeps = 0;
   int lpn = 4;
   bool padded = true;  // padded layout (default) or compacted lists (F2M_SWEEP=compact)
+  int pad_cfg = 2;     // padded variant (stride,lanes): see launch_padded; default 4,2
+  int pp = 4;          // padded stride of pad_cfg
   // instrumentation: active slots of the current lists, totals
   unsigned long long cur_actsum = 0;
   int64_t active_slots = 0;
@@ -557,7 +568,7 @@ struct Runner {
     F2M_HIP(c, w.axb.reserve(m * 8 + (int64_t)n * 8 + (int64_t)(n + 1) * 4));
     F2M_HIP(c, w.ef.reserve(m * 16));
     F2M_HIP(c, w.efoff.reserve((n + 1) * 4));
-    F2M_HIP(c, w.psl.reserve(((int64_t)n * PP + m) * 16));
+    F2M_HIP(c, w.psl.reserve(((int64_t)n * PP_MAX + m) * 16));
     F2M_HIP(c, w.povo.reserve((n + 1) * 4));
     F2M_HIP(c, w.ptmp.reserve(n * 4));
     F2M_HIP(c, w.red.reserve(MAXS * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
@@ -589,6 +600,13 @@ struct Runner {
     lpn = env_int("F2M_LPN", 4);
     const char* v = std::getenv("F2M_SWEEP");
     padded = !(v && std::strcmp(v, "compact") == 0);
+    if (const char* pv = std::getenv("F2M_PAD")) {
+      static const char* names[] = {"8,4", "4,4", "4,2", "8,8", "8,4,g", "4,1", "2,2", "6,2", "4,2,g", "2,1", "3,1"};
+      for (int i = 0; i < 11; ++i)
+        if (std::strcmp(pv, names[i]) == 0) pad_cfg = i;
+    }
+    static const int strides[] = {8, 4, 4, 8, 8, 4, 2, 6, 4, 2, 3};
+    pp = strides[pad_cfg];
     return F2M_OK;
   }
 
@@ -612,8 +630,9 @@ struct Runner {
   }
   // ev0/ev1: when set, the runtime records them at the kernel's own start and
   // end (hipExtLaunchKernelGGL), so sampled timings exclude event-marker latency
-  void launch_padded(bool write, bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
-    dim3 grid(grid_for((int64_t)n * PLPN, BS)), block(BS);
+  template <int PPT, int LPN, bool SPEC>
+  void launch_pad_t(bool write, bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
+    dim3 grid(grid_for((int64_t)n * LPN, BS)), block(BS);
     const uint8_t* fz = w.frozen.as<uint8_t>();
     auto go = [&](auto kern) {
       if (ev0)
@@ -624,9 +643,26 @@ struct Runner {
         hipLaunchKernelGGL(kern, grid, block, 0, c->stream, n, I(w.act), I(w.povo), w.psl.as<ESlot>(), F(), Fo(), fz,
                            mslots);
     };
-    if (!write) go(s_sweep_pad<false, false>);
-    else if (frozen) go(s_sweep_pad<true, true>);
-    else go(s_sweep_pad<false, true>);
+    if (!write) go(s_sweep_pad<PPT, LPN, SPEC, false, false>);
+    else if (frozen) go(s_sweep_pad<PPT, LPN, SPEC, true, true>);
+    else go(s_sweep_pad<PPT, LPN, SPEC, false, true>);
+  }
+  // ev0/ev1: when set, the runtime records them at the kernel's own start and
+  // end (hipExtLaunchKernelGGL), so sampled timings exclude event-marker latency
+  void launch_padded(bool write, bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
+    switch (pad_cfg) {
+      case 1: launch_pad_t<4, 4, true>(write, frozen, mslots, ev0, ev1); break;
+      case 2: launch_pad_t<4, 2, true>(write, frozen, mslots, ev0, ev1); break;
+      case 3: launch_pad_t<8, 8, true>(write, frozen, mslots, ev0, ev1); break;
+      case 4: launch_pad_t<8, 4, false>(write, frozen, mslots, ev0, ev1); break;
+      case 5: launch_pad_t<4, 1, true>(write, frozen, mslots, ev0, ev1); break;
+      case 6: launch_pad_t<2, 2, true>(write, frozen, mslots, ev0, ev1); break;
+      case 7: launch_pad_t<6, 2, true>(write, frozen, mslots, ev0, ev1); break;
+      case 8: launch_pad_t<4, 2, false>(write, frozen, mslots, ev0, ev1); break;
+      case 9: launch_pad_t<2, 1, true>(write, frozen, mslots, ev0, ev1); break;
+      case 10: launch_pad_t<3, 1, true>(write, frozen, mslots, ev0, ev1); break;
+      default: launch_pad_t<8, 4, true>(write, frozen, mslots, ev0, ev1); break;
+    }
   }
   f2m_status launch_sweep(bool write, bool frozen, unsigned long long* mslots) {
     const bool timed = write && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
@@ -681,9 +717,9 @@ struct Runner {
     F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
-      hipLaunchKernelGGL(s_excess, gn, bs, 0, c->stream, n, I(w.act), PP, I(w.ptmp));
+      hipLaunchKernelGGL(s_excess, gn, bs, 0, c->stream, n, I(w.act), pp, I(w.ptmp));
       F2M_HIP(c, scan_i32_to_i32(c, I(w.ptmp), I(w.povo), n, nullptr));
-      hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
+      hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, pp, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
                          w.psl.as<ESlot>());
     } else {
       hipLaunchKernelGGL(s_ef_fill, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.efoff), I(w.nbr), w.D.as<double>(),

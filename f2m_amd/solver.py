@@ -121,6 +121,27 @@ class Context:
         self.graph = g
         return g
 
+    def spatial_order(self, x, y) -> np.ndarray:
+        """Morton (Z-order) permutation of the points (f2m_spatial_order):
+        perm[i] = input index of node i when the graph is built on
+        x[perm], y[perm]."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        perm = np.zeros(len(x), np.int32)
+        self._check(lib.f2m_spatial_order(self.h, len(x), _ptr(x), _ptr(y), _ptr(perm)))
+        return perm
+
+    def spatial_order_batched(self, xs, ys) -> list:
+        """Per-instance Morton permutations (local indices) of a list of
+        point sets, computed in one pass (f2m_spatial_order_batched)."""
+        off = np.zeros(len(xs) + 1, np.int64)
+        off[1:] = np.cumsum([len(a) for a in xs])
+        x = np.ascontiguousarray(np.concatenate(xs), np.float64)
+        y = np.ascontiguousarray(np.concatenate(ys), np.float64)
+        perm = np.zeros(off[-1], np.int64)
+        self._check(lib.f2m_spatial_order_batched(self.h, len(xs), _ptr(off), _ptr(x), _ptr(y), _ptr(perm)))
+        return [perm[off[i]:off[i + 1]] - off[i] for i in range(len(xs))]
+
     def graph_from_csr(self, rowptr, nbr, cost) -> L.Graph:
         rowptr = np.ascontiguousarray(rowptr, np.int32)
         nbr = np.ascontiguousarray(nbr, np.int32)

@@ -54,3 +54,29 @@ def certificate(rowptr, nbr, cost, x, F, d_min):
     P = float(np.dot(c, xe))
     D = float(2 * y.sum() - np.maximum(0.0, y[u] + y[w] - c).sum())
     return P, D
+
+
+def morton_order(x, y):
+    """numpy restatement of f2m_spatial_order: 16-bit quantisation over the
+    bounding box, x in the even bits, stable sort (ties in input order)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+
+    def quant(v):
+        mn, mx = v.min(), v.max()
+        ext = mx - mn
+        if not ext > 0:
+            return np.zeros(len(v), np.uint64)
+        q = np.floor(((v - mn) / ext) * 65535.0)
+        q = np.where(q > 0, q, 0)
+        return np.minimum(q, 65535).astype(np.uint64)
+
+    def spread(v):
+        v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x33333333)
+        v = (v | (v << np.uint64(1))) & np.uint64(0x55555555)
+        return v
+
+    key = spread(quant(x)) | (spread(quant(y)) << np.uint64(1))
+    return np.argsort(key, kind="stable").astype(np.int32)

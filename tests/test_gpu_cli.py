@@ -24,22 +24,28 @@ def _unpack(tmp, inst):
     return p
 
 
-@pytest.mark.parametrize("inst,K", [("xqf131", 10), ("dka1376", 20)])
-def test_cli_files_and_optimum(orc, inst, K):
+@pytest.mark.parametrize("inst,K,order", [("xqf131", 10, "morton"), ("dka1376", 20, "morton"),
+                                          ("dka1376", 20, "input")])
+def test_cli_files_and_optimum(orc, inst, K, order):
     from oracle import tsplib
+    from tests.helpers import morton_order
     tmp = tempfile.mkdtemp()
     try:
         p = _unpack(tmp, inst)
-        say("cli", inst, K)
-        r = subprocess.run([CLI, p, str(K), "--quiet"], capture_output=True, text=True, timeout=120)
+        say("cli", inst, K, order)
+        flags = ["--quiet"] + (["--input-order"] if order == "input" else [])
+        r = subprocess.run([CLI, p, str(K)] + flags, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         rep = open(p + f".top{K}.F2M_report").read().splitlines()
         assert rep[0] == "Optimal Found: Yes"
         m = re.match(r"Optimal Value: (\S+)$", rep[1])
         assert m and rep[2].startswith("Time: ") and rep[3].startswith("MAX edge rank: ")
         obj = float(m.group(1))
-        # same instance through the oracle (parse restatement + kNN + solve)
+        # same instance through the oracle (parse restatement + kNN + solve),
+        # nodes numbered as the CLI numbers them
         lab, x, y, _ = tsplib.read(p)
+        perm = morton_order(x, y) if order == "morton" else np.arange(len(x))
+        x, y, lab = x[perm], y[perm], lab[perm]
         rp, nb, c = orc.graph_build(x, y, K, 0)
         ro = orc.solve(rp, nb, c)
         assert abs(obj - ro["stats"]["objective"]) <= 1e-9 * obj + 1e-9
@@ -53,9 +59,13 @@ def test_cli_files_and_optimum(orc, inst, K):
             deg[int(b)] = deg.get(int(b), 0) + float(xv)
         assert set(deg) == set(lab.tolist())
         assert max(abs(v - 2) for v in deg.values()) < 1e-3
-        # .DM: one line per kept edge, 0-based indices
-        dm = open(p + f".top{K}.DM").read().splitlines()
+        # .DM: one line per kept edge, 0-based input indices
+        dm = [ln.split() for ln in open(p + f".top{K}.DM").read().splitlines()]
         assert len(dm) == len(nb) // 2
+        src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        mk = src < nb
+        want = {(min(perm[a], perm[b]), max(perm[a], perm[b])) for a, b in zip(src[mk], nb[mk])}
+        assert {(int(a), int(b)) for a, b, _ in dm} == want
         # usage message and exit code of the reference (exit(0))
         r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
         assert r.returncode == 0 and "usage" in r.stdout

@@ -167,3 +167,32 @@ def test_solve_is_deterministic(ctx):
     assert a == b
     for k in ra:
         assert np.array_equal(ra[k], rb[k])
+
+
+def test_spatial_order_matches_restatement(ctx):
+    from tests.helpers import morton_order
+    x, y, _, _ = W.c2()
+    say("spatial order C2")
+    assert np.array_equal(ctx.spatial_order(x, y), morton_order(x, y))
+    # collinear points (zero extent on one axis), repeated quantised keys, tiny sets
+    xl = np.arange(5000, dtype=float) * 0.5
+    yl = np.zeros(5000)
+    assert np.array_equal(ctx.spatial_order(xl, yl), morton_order(xl, yl))
+    xs, ys = W.uniform_int(3000, 10**9, 31)
+    assert np.array_equal(ctx.spatial_order(xs, ys), morton_order(xs, ys))
+    assert np.array_equal(ctx.spatial_order(np.array([3.0]), np.array([1.0])), [0])
+    # batched: each instance permuted within itself
+    insts = [W.uniform_int(n, 10**6, 40 + i) for i, n in enumerate((100, 2500, 7, 900))]
+    perms = ctx.spatial_order_batched([a for a, _ in insts], [b for _, b in insts])
+    for (a, b), p in zip(insts, perms):
+        assert np.array_equal(p, morton_order(a, b))
+
+
+def test_morton_numbering_same_optimum(ctx, orc, golden):
+    # C1 renumbered along the curve: the same LP optimum, bit-exact with the
+    # oracle on the renumbered points
+    from tests.helpers import morton_order
+    case = [c for c in golden["lp"] if c["name"] == "C1"][0]
+    x, y, mode, K = W.c1()
+    p = morton_order(x, y)
+    _full_compare(ctx, orc, x[p], y[p], K, mode, "C1 morton", lp_opt=case["lp_opt"])
