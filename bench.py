@@ -13,13 +13,13 @@ reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).  The
 same solve in input order is timed as well (input_order).
 Prints one JSON line.  `value` = ms per solve (lower is better).
 
-roofline: the dominant kernel is the fused sweep (s_sweep).  Its algorithmic
-bytes per launch are 12 B per active slot (nbr int32 + D double) + 20 B per
-node (list offset, F in, F out); the average launch duration comes
-from HIP events the library records on its stream around every 16th sweep
-launch of the timed solves.  traffic is read from profiles/traffic.json
-(rocprofv3 FETCH_SIZE + WRITE_SIZE per sweep launch, scaled by a copy
-kernel of known size) when present.
+roofline: the dominant kernel is the fused sweep (s_sweep_pad).  Its
+algorithmic bytes per launch are 12 B per active slot (D double + nbr int32)
++ 20 B per node (list offset, F in, F out); the average kernel duration comes
+from the HIP events the library records at the start and end of every 16th
+sweep kernel of the timed solves (hipExtLaunchKernelGGL, on its stream).
+traffic is read from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE
+per sweep launch, scaled by a copy kernel of known size) when present.
 
 cpu_baseline: the oracle (the CPU restatement of the reference schedule,
 oracle/f2m_oracle.cpp) solving the same graph with the host's threads.
@@ -126,7 +126,7 @@ def main():
             traffic = json.load(f).get("s_sweep_c2_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "s_sweep", "avg_launch_us": round(launch_ms * 1e3, 3),
+                "kernel": "s_sweep_pad", "avg_launch_us": round(launch_ms * 1e3, 3),
                 "alg_bytes_per_launch": round(bytes_per_launch, 1)}
 
     cpu = None
