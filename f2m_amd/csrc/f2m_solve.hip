@@ -171,6 +171,11 @@ __global__ void __launch_bounds__(BS) s_sweep(int n, const int32_t* __restrict__
   if (maxslots) block_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
 }
 
+// profiling: acc += (active slots of the current lists) x sweeps
+__global__ void s_acc_slots(const int32_t* __restrict__ total, unsigned long long* acc, unsigned long long sweeps) {
+  if (threadIdx.x == 0) *acc += (unsigned long long)(*total) * sweeps;
+}
+
 // ---- padded layout ----------------------------------------------------------
 // The compacted sweep above reads three dependent levels: list offset ->
 // slot -> neighbour potential.  The padded layout keeps each node's first PP
@@ -550,9 +555,10 @@ struct Runner {
   int pad_cfg = 2;     // padded variant (stride,lanes): see launch_padded; default 4,2
   int pp = 4;          // padded stride of pad_cfg
   // instrumentation: active slots of the current lists, totals
-  unsigned long long cur_actsum = 0;
-  int64_t active_slots = 0;
-  double alg_bytes = 0;
+  // with profiling on: sweeps run on the current lists not yet counted, and a
+  // device accumulator of active slots x sweeps (no host round trip per list)
+  int64_t pending = 0;
+  unsigned long long* acc = nullptr;
   size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
   double* part = nullptr;              // reduction partials
@@ -595,12 +601,14 @@ struct Runner {
     maxs = w.red.as<unsigned long long>();
     part = reinterpret_cast<double*>(maxs + MAXS);
     sums = part + 3 * RED_BLOCKS;
+    acc = reinterpret_cast<unsigned long long*>(sums + 4);
     ibuf = reinterpret_cast<int32_t*>(sums + 8);
     return F2M_OK;
   }
 
   f2m_status setup() {
     hipLaunchKernelGGL(s_zero_u64, dim3(4), dim3(256), 0, c->stream, maxs, (int64_t)MAXS, INF_BITS);
+    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, acc, (int64_t)1, 0ull);
     int nb = grid_for(m, BS);
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(s_min_pos, dim3(nb), dim3(BS), 0, c->stream, gcost, m, maxs);
@@ -711,13 +719,33 @@ struct Runner {
     if (write) {
       w.cur ^= 1;
       ++sweeps;
-      active_slots += cur_actsum;
-      alg_bytes += 12.0 * (double)cur_actsum + 20.0 * (double)n;
+      if (c->prof_every > 0) ++pending;
     }
     return F2M_OK;
   }
-  // harvest sampled sweep timings into ctx->prof (after a stream sync)
+  // adds (active slots of the current lists) x (sweeps run on them) on the device
+  f2m_status flush_slots() {
+    if (c->prof_every > 0 && pending > 0) {
+      hipLaunchKernelGGL(s_acc_slots, dim3(1), dim3(1), 0, c->stream, I(w.efoff) + n, acc,
+                         (unsigned long long)pending);
+      F2M_LAUNCH_CHECK(c);
+    }
+    pending = 0;
+    return F2M_OK;
+  }
+  // harvest sampled sweep timings and slot counts into ctx->prof
   f2m_status harvest() {
+    f2m_status st = flush_slots();
+    if (st) return st;
+    unsigned long long slots = 0;
+    if (c->prof_every > 0) {
+      unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+      F2M_HIP(c, hipMemcpyAsync(hp, acc, 8, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipStreamSynchronize(c->stream));
+      slots = hp[0];
+    }
+    const int64_t active_slots = (int64_t)slots;
+    const double alg_bytes = 12.0 * (double)slots + 20.0 * (double)n * (double)sweeps;
     double ms = 0;
     for (size_t i = 0; i < prof_used; ++i) {
       float t = 0;
@@ -732,8 +760,11 @@ struct Runner {
     return F2M_OK;
   }
   // update_EF (graph_cuda.cu:229-252): offsets = scan(act), copy the active
-  // prefixes.  With profiling on, the total is read back for the byte count.
+  // prefixes.  With profiling on, the sweeps run on the old lists are counted
+  // first (on the device).
   f2m_status ef_rebuild() {
+    f2m_status st0 = flush_slots();
+    if (st0) return st0;
     F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
@@ -746,12 +777,6 @@ struct Runner {
                          w.ef.as<ESlot>());
     }
     F2M_LAUNCH_CHECK(c);
-    if (c->prof_every > 0) {
-      int32_t* hp = static_cast<int32_t*>(c->pinned);
-      F2M_HIP(c, hipMemcpyAsync(hp, I(w.efoff) + n, 4, hipMemcpyDeviceToHost, c->stream));
-      F2M_HIP(c, hipStreamSynchronize(c->stream));
-      cur_actsum = (unsigned long long)hp[0];
-    }
     return F2M_OK;
   }
   // max over the 32 strided slots of region r (0: gradient, 1: early exit)
