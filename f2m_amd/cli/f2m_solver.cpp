@@ -2,13 +2,16 @@
 // RUN.cpp:12-31, `F2M.solver.exe <tsplib-file> <topK>`).
 //
 //   f2m_solver <tsplib-file> <topK> [--early-exit] [--quiet] [--input-order]
-//                                  [--reference-recovery]
+//                                  [--reference-recovery] [--adaptive]
 //
 // Nodes are numbered along the Morton curve of the points (f2m_spatial_order;
 // --input-order keeps the input order), which only changes speed and, among
 // equal-cost candidates at the K-th rank, which one the kNN rule keeps.  The
 // exact recovery runs where the reference's recovery is not exact
 // (--reference-recovery reproduces the reference's output there instead).
+// --adaptive: topK is the first K; while some node's K-th nearest cost is below
+// its dual bound (f2m_knn_certify), K doubles (at most 80) and the solve restarts
+// from the previous potentials; the report gains a line "kNN K: <K> (certified)".
 //
 // Writes next to the input, as the reference does (README.md:32-45):
 //   <file>.top<K>.F2M_solution  "label1 label2 x" per edge with x != 0, label1 <
@@ -45,17 +48,19 @@ static int die(f2m_ctx* ctx, f2m_status s, const char* what) {
 }
 
 int main(int argc, char** argv) {
-  bool early = false, quiet = false, input_order = false, ref_recovery = false;
+  bool early = false, quiet = false, input_order = false, ref_recovery = false, adaptive = false;
   std::vector<std::string> pos;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--early-exit")) early = true;
     else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
     else if (!std::strcmp(argv[i], "--input-order")) input_order = true;
     else if (!std::strcmp(argv[i], "--reference-recovery")) ref_recovery = true;
+    else if (!std::strcmp(argv[i], "--adaptive")) adaptive = true;
     else pos.push_back(argv[i]);
   }
   if (pos.size() != 2) {
-    std::cout << "usage: f2m_solver fileXY topK [--early-exit] [--quiet] [--input-order] [--reference-recovery]\n"
+    std::cout << "usage: f2m_solver fileXY topK [--early-exit] [--quiet] [--input-order] [--reference-recovery]"
+                 " [--adaptive]\n"
                  "fileXY == file in TSPLIB format\n"
                  "topK == the number of top K closest neighbors for each node\n";
     return 0;
@@ -103,9 +108,36 @@ int main(int argc, char** argv) {
     x.swap(x2);
     y.swap(y2);
   }
+  f2m_opts o;
+  f2m_default_opts(&o);
+  o.early_exit = early ? 1 : 0;
+  if (ref_recovery) o.exact = 0;
+  // solve with K; with --adaptive, while the kNN restriction is not certified
+  // (f2m_knn_certify > 0) double K (at most 80) and re-solve from the previous
+  // potentials (f2m_solve_warm)
   f2m_graph g;
-  if ((s = f2m_knn_build(ctx, (int32_t)n, x.data(), y.data(), K, F2M_COST_RAW_FP64, &g)))
-    return die(ctx, s, "kNN graph");
+  f2m_stats st;
+  int Kc = K;
+  int64_t viol = -1;
+  std::vector<double> Fw;
+  double secs = 0;
+  for (;;) {
+    if ((s = f2m_knn_build(ctx, (int32_t)n, x.data(), y.data(), Kc, F2M_COST_RAW_FP64, &g)))
+      return die(ctx, s, "kNN graph");
+    auto t0 = std::chrono::steady_clock::now();
+    s = Fw.empty() ? f2m_solve(ctx, &g, &o, &st) : f2m_solve_warm(ctx, &g, &o, Fw.data(), &st);
+    if (s) return die(ctx, s, "solve");
+    secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!adaptive) break;
+    if ((s = f2m_knn_certify(ctx, &g, &viol))) return die(ctx, s, "certificate");
+    if (!quiet) std::cout << "K " << Kc << ": objective " << std::setprecision(15) << st.objective
+                          << ", nodes failing the kNN certificate " << viol << "\n";
+    const int next = std::min<int64_t>(std::min(2 * Kc, 80), n - 1);
+    if (viol == 0 || next <= Kc) break;
+    Fw.resize(n);
+    if ((s = f2m_result_copy(ctx, Fw.data(), nullptr, nullptr))) return die(ctx, s, "results");
+    Kc = next;
+  }
   std::vector<int32_t> rowptr(n + 1), nbr(g.nslots);
   std::vector<double> cost(g.nslots);
   if ((s = f2m_graph_copy(ctx, &g, rowptr.data(), nbr.data(), cost.data()))) return die(ctx, s, "graph copy");
@@ -123,15 +155,6 @@ int main(int argc, char** argv) {
     std::ofstream f(stem + ".DM");
     for (auto& t : e) f << std::get<1>(t) << " " << std::get<2>(t) << " " << std::get<0>(t) << "\n";
   }
-
-  f2m_opts o;
-  f2m_default_opts(&o);
-  o.early_exit = early ? 1 : 0;
-  if (ref_recovery) o.exact = 0;
-  f2m_stats st;
-  auto t0 = std::chrono::steady_clock::now();
-  if ((s = f2m_solve(ctx, &g, &o, &st))) return die(ctx, s, "solve");
-  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::vector<double> xs(g.nslots);
   if ((s = f2m_result_copy(ctx, nullptr, xs.data(), nullptr))) return die(ctx, s, "results");
   f2m_ctx_destroy(ctx);
@@ -154,6 +177,7 @@ int main(int argc, char** argv) {
     rep << "Optimal Value: " << std::setprecision(15) << st.objective << "\n";
     rep << "Time: " << secs << "\n";
     rep << "MAX edge rank: " << st.max_rank << "\n";
+    if (adaptive) rep << "kNN K: " << Kc << (viol == 0 ? " (certified)" : " (not certified)") << "\n";
     if (!quiet) {
       std::cout << "Optimal Found: Yes\n";
       std::cout << "Optimal Value: " << std::setprecision(15) << st.objective << "\n";

@@ -86,6 +86,8 @@ struct f2m_ctx {
   void* pinned = nullptr;
   // last solved graph (for f2m_result_copy)
   int64_t last_n = 0, last_m = 0;
+  double last_dmin = 0;    // d_min of the last single-graph solve
+  int32_t last_k = 0;      // k of its graph (0: caller CSR)
   // instrumentation
   int32_t prof_every = 0;
   std::vector<hipEvent_t> prof_ev;  // pool of start/end pairs

@@ -176,6 +176,19 @@ __global__ void s_acc_slots(const int32_t* __restrict__ total, unsigned long lon
   if (threadIdx.x == 0) *acc += (unsigned long long)(*total) * sweeps;
 }
 
+// nodes whose K-th nearest cost r_v is below -d_min F_v: an edge left out of
+// the kNN graph costs at least max(r_u, r_w), so with no such node every
+// left-out edge has a nonnegative reduced cost (y = -d_min F / 2)
+__global__ void s_knn_certify(int n, const int32_t* __restrict__ rp, const double* __restrict__ cost, int K,
+                              double dmin, const double* __restrict__ F, unsigned long long* cnt) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int bad = 0;
+  if (v < n && rp[v + 1] - rp[v] >= K) bad = cost[rp[v] + K - 1] < -(dmin * F[v]) ? 1 : 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
+  if ((threadIdx.x & 63) == 0 && bad) atomicAdd(cnt, (unsigned long long)bad);
+}
+
 // ---- padded layout ----------------------------------------------------------
 // The compacted sweep above reads three dependent levels: list offset ->
 // slot -> neighbour potential.  The padded layout keeps each node's first PP
@@ -1062,17 +1075,25 @@ f2m_status Runner::count_roots(int32_t& out) {
 
 }  // namespace
 
-f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opts& o, f2m_stats* st) {
+f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opts& o, f2m_stats* st,
+                       const double* F_init) {
   Runner R{c, w, o, g->n, g->nslots, g->rowptr, g->nbr, g->cost};
   f2m_status s;
   w.n = g->n;
   w.m = g->nslots;
   if ((s = R.alloc())) return s;
   if ((s = R.setup())) return s;
+  c->last_dmin = R.dmin;
+  c->last_k = g->k;
   f2m_stats z;
   std::memset(&z, 0, sizeof z);
   z.d_min = R.dmin;
-  if ((s = R.block_glob(z.glob_rounds))) return s;
+  if (F_init) {  // warm start: the given potentials, schedule from check_OPT
+    F2M_HIP(c, hipMemcpyAsync(w.F0.p, F_init, (size_t)g->n * 8, hipMemcpyDeviceToDevice, c->stream));
+    w.cur = 0;
+  } else if ((s = R.block_glob(z.glob_rounds))) {
+    return s;
+  }
   double M = 0;
   if (o.stop_phase != 1)
     if ((s = R.run_start(M, z.outer_rounds))) return s;
@@ -1128,9 +1149,39 @@ f2m_status f2m_solve(f2m_ctx* c, const f2m_graph* g, const f2m_opts* o, f2m_stat
   f2m_opts oo;
   if (o) oo = *o; else f2m_default_opts(&oo);
   auto t0 = std::chrono::steady_clock::now();
-  f2m_status s = solve_graph(c, c->ws, g, oo, st);
+  f2m_status s = solve_graph(c, c->ws, g, oo, st, nullptr);
   c->prof.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return s;
+}
+
+f2m_status f2m_solve_warm(f2m_ctx* c, const f2m_graph* g, const f2m_opts* o, const double* F_init, f2m_stats* st) {
+  if (!c || !g || g->n < 2 || g->nslots < 1 || !F_init) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  f2m_opts oo;
+  if (o) oo = *o; else f2m_default_opts(&oo);
+  const double* fi = (const double*)stage_in(c, F_init, (size_t)g->n * 8, 3);
+  if (!fi) return F2M_ERR_HIP;
+  auto t0 = std::chrono::steady_clock::now();
+  f2m_status s = solve_graph(c, c->ws, g, oo, st, fi);
+  c->prof.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return s;
+}
+
+f2m_status f2m_knn_certify(f2m_ctx* c, const f2m_graph* g, int64_t* violations) {
+  if (!c || !g || !violations) return F2M_ERR_ARG;
+  if (g->k < 1) return fail(c, F2M_ERR_ARG, "the graph was not built by f2m_knn_build (k unknown)");
+  if (c->last_n != g->n || c->last_k != g->k) return fail(c, F2M_ERR_ARG, "solve this graph first");
+  F2M_HIP(c, hipSetDevice(c->device));
+  SolveWs& w = c->ws;
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(w.red.as<double>());
+  hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, cnt, (int64_t)1, 0ull);
+  hipLaunchKernelGGL(s_knn_certify, dim3(grid_for(g->n, BS)), dim3(BS), 0, c->stream, g->n, g->rowptr, g->cost,
+                     g->k, c->last_dmin, (w.cur ? w.F1 : w.F0).as<double>(), cnt);
+  F2M_LAUNCH_CHECK(c);
+  unsigned long long h = 0;
+  F2M_HIP(c, copy_out(c, &h, cnt, 8));
+  *violations = (int64_t)h;
+  return F2M_OK;
 }
 
 f2m_status f2m_ctx_set_profile(f2m_ctx* c, int32_t sample_every) {

@@ -161,11 +161,48 @@ class Context:
         return rowptr, nbr, cost
 
     # ---- solve -----------------------------------------------------------
-    def solve(self, g: L.Graph | None = None, opts: L.Opts | None = None) -> dict:
+    def solve(self, g: L.Graph | None = None, opts: L.Opts | None = None, F_init=None) -> dict:
+        """f2m_solve, or f2m_solve_warm from potentials F_init (normalised, as
+        result()["F"] returns them)."""
         g = g or self.graph
         st = L.Stats()
-        self._check(lib.f2m_solve(self.h, ctypes.byref(g), ctypes.byref(opts) if opts else None, ctypes.byref(st)))
+        o = ctypes.byref(opts) if opts else None
+        if F_init is None:
+            self._check(lib.f2m_solve(self.h, ctypes.byref(g), o, ctypes.byref(st)))
+        else:
+            if not hasattr(F_init, "data_ptr"):
+                F_init = np.ascontiguousarray(F_init, np.float64)
+            self._check(lib.f2m_solve_warm(self.h, ctypes.byref(g), o, _ptr(F_init), ctypes.byref(st)))
         return st.as_dict()
+
+    def knn_certify(self, g: L.Graph | None = None) -> int:
+        """Nodes whose k-th nearest cost is below -d_min F after the last
+        solve of g (f2m_knn_certify); 0 certifies that the optimum on the kNN
+        graph is the LP optimum over all pairs of points."""
+        g = g or self.graph
+        v = ctypes.c_int64(0)
+        self._check(lib.f2m_knn_certify(self.h, ctypes.byref(g), ctypes.byref(v)))
+        return int(v.value)
+
+    def solve_adaptive(self, x, y, k0: int, mode: int, k_max: int = 80, opts: L.Opts | None = None):
+        """Adaptive K: solve with k0, and while the kNN restriction is not
+        certified (knn_certify > 0) double k (at most k_max) and re-solve,
+        warm-started from the previous potentials (node ids are unchanged).
+        Returns (k, stats, result, history) with history = [(k, objective,
+        violations, sweeps)]."""
+        k = k0
+        F = None
+        hist = []
+        while True:
+            g = self.knn_build(x, y, k, mode)
+            st = self.solve(g, opts, F_init=F)
+            res = self.result()
+            viol = self.knn_certify(g)
+            hist.append((k, st["objective"], viol, st["sweeps"]))
+            if viol == 0 or k >= k_max or k >= len(x) - 1:
+                return k, st, res, hist
+            F = res["F"]
+            k = min(2 * k, k_max, len(x) - 1)
 
     def result(self, n: int | None = None, nslots: int | None = None):
         g = self.graph

@@ -222,6 +222,21 @@ f2m_status f2m_graph_copy(f2m_ctx* ctx, const f2m_graph* g, int32_t* rowptr, int
  * Dual descent with the reference schedule, tight components, primal
  * recovery, objective.  Results stay on the device; copy with f2m_result_copy. */
 f2m_status f2m_solve(f2m_ctx* ctx, const f2m_graph* g, const f2m_opts* o, f2m_stats* st);
+/* Warm start (new; the reference always starts from F = 0 in block_GLOB,
+ * graph_cuda.cu:363-413): the same solve from potentials F_init [dev|host] n
+ * (normalised, as f2m_result_copy returns them, e.g. of the same points with a
+ * smaller k), skipping block_GLOB (stats.glob_rounds = 0). */
+f2m_status f2m_solve_warm(f2m_ctx* ctx, const f2m_graph* g, const f2m_opts* o, const double* F_init,
+                          f2m_stats* st);
+/* kNN restriction certificate (new; the reference reports the MAX edge rank,
+ * graph_tour.hpp:169-180, as a hint).  After a solve of g (built by
+ * f2m_knn_build with k), counts the nodes v with r_v < -d_min F_v, r_v = cost of
+ * v's k-th nearest neighbour.  An edge left out of the graph costs at least
+ * max(r_u, r_w), so with 0 such nodes every left-out edge has a nonnegative
+ * reduced cost: the dual stays feasible over all pairs and the optimum on the
+ * graph is the LP optimum over the complete graph.  (Sufficient, not
+ * necessary: a graph can be optimal with nodes counted.) */
+f2m_status f2m_knn_certify(f2m_ctx* ctx, const f2m_graph* g, int64_t* violations);
 /* F [n] potentials, x [nslots] per canonical slot, comb_root [n] smallest node
  * id of the node's tight component or -1.  Any pointer may be NULL; [dev|host]. */
 f2m_status f2m_result_copy(f2m_ctx* ctx, double* F, double* x, int32_t* comb_root);

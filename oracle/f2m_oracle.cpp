@@ -643,6 +643,12 @@ int orc_axb(int m, const int32_t* row_ptr, const int32_t* col, const int32_t* b,
 int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* cost,
               const orc_opts* opts, int nthreads, double* F_out, double* x_out,
               int32_t* comb_root, orc_stats* st) {
+  return orc_solve_warm(n, rowptr, nbr, cost, opts, nthreads, nullptr, F_out, x_out, comb_root, st);
+}
+
+int orc_solve_warm(int n, const int32_t* rowptr, const int32_t* nbr, const double* cost,
+                   const orc_opts* opts, int nthreads, const double* F_init, double* F_out, double* x_out,
+                   int32_t* comb_root, orc_stats* st) {
   orc_opts o;
   if (opts) o = *opts; else orc_default_opts(&o);
   Pool pool(nthreads);
@@ -651,8 +657,13 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
   z.d_min = S.dmin;
   if (!(S.dmin > 0)) { if (st) *st = z; return -2; }
 
-  // graph::RUN (graph_cuda.cu:47-155)
-  z.glob_rounds = S.block_glob();
+  // graph::RUN (graph_cuda.cu:47-155); a warm start (new) begins at check_OPT
+  // from the given potentials instead of block_GLOB from zero
+  if (F_init) {
+    for (int v = 0; v < n; ++v) S.F[v] = F_init[v];
+  } else {
+    z.glob_rounds = S.block_glob();
+  }
   double M = 0;
   if (o.stop_phase != 1) z.outer_rounds = S.run_start(M);
   z.max_grad = M;
@@ -865,6 +876,21 @@ void orc_bound_backward(int n, const int32_t* rowptr, const int32_t* nbr, const 
     gx[u] = ax;
     gy[u] = ay;
   }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int64_t orc_knn_certify(int n, const int32_t* rowptr, const double* cost, int K, double dmin, const double* F) {
+  int64_t bad = 0;
+  for (int v = 0; v < n; ++v) {
+    const int32_t deg = rowptr[v + 1] - rowptr[v];
+    if (deg < K) continue;  // v keeps every other point: nothing left out
+    const double r = cost[rowptr[v] + K - 1];
+    if (r < -(dmin * F[v])) ++bad;
+  }
+  return bad;
 }
 
 }  // extern "C"

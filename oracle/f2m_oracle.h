@@ -113,6 +113,22 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
               const orc_opts* o, int nthreads, double* F_out, double* x_out,
               int32_t* comb_root, orc_stats* st);
 
+/* --- warm start (new): the schedule from potentials F_init (normalised, as
+ * F_out), skipping block_GLOB (graph_cuda.cu:363-413); glob_rounds = 0. */
+int orc_solve_warm(int n, const int32_t* rowptr, const int32_t* nbr, const double* cost,
+                   const orc_opts* o, int nthreads, const double* F_init, double* F_out, double* x_out,
+                   int32_t* comb_root, orc_stats* st);
+
+/* --- kNN restriction certificate (new; the reference reports the MAX edge
+ * rank, graph_tour.hpp:169-180, as a hint).  Counts the nodes v with
+ * r_v < -d_min F_v, r_v = cost of v's K-th nearest neighbour (slot K-1 of its
+ * (cost, id)-sorted row; nodes of degree < K are skipped).  An edge (u, w) left
+ * out of the symmetrised kNN graph has cost >= max(r_u, r_w); with y = -d_min
+ * F / 2 its reduced cost c - y_u - y_w is then >= 0 whenever no node is
+ * counted, so the dual stays feasible on the complete graph and the optimum on
+ * the graph is the optimum over all pairs. */
+int64_t orc_knn_certify(int n, const int32_t* rowptr, const double* cost, int K, double dmin, const double* F);
+
 /* --- least squares of the irregular combs (Ax_b.cpp:19-158) ----------------
  * rows: CSR over equations (row_ptr[m+1], col[nnz]); all coefficients 1.
  * x_out[nv]; returns iterations used (go_max) or 0 if not converged. */

@@ -121,6 +121,11 @@ def _setup(L):
     L.orc_exact.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Opts), _P, _P, _P,
                             ctypes.POINTER(Stats)]
     L.orc_exact.restype = ctypes.c_int
+    L.orc_solve_warm.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.POINTER(Opts), ctypes.c_int, _P, _P, _P, _P,
+                                 ctypes.POINTER(Stats)]
+    L.orc_solve_warm.restype = ctypes.c_int
+    L.orc_knn_certify.argtypes = [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_double, _P]
+    L.orc_knn_certify.restype = ctypes.c_int64
 
 
 def default_opts(**kw) -> Opts:
@@ -179,7 +184,7 @@ def qpart(d, thr, raise_):
     return p, d, idx
 
 
-def solve(rowptr, nbr, cst, opts: Opts | None = None, threads=8):
+def solve(rowptr, nbr, cst, opts: Opts | None = None, threads=8, F_init=None):
     n = len(rowptr) - 1
     m = len(nbr)
     o = opts or default_opts()
@@ -187,9 +192,10 @@ def solve(rowptr, nbr, cst, opts: Opts | None = None, threads=8):
     x = np.zeros(m)
     root = np.zeros(n, np.int32)
     st = Stats()
-    rc = lib().orc_solve(n, _ptr(np.ascontiguousarray(rowptr, np.int32)), _ptr(np.ascontiguousarray(nbr, np.int32)),
-                         _ptr(np.ascontiguousarray(cst, np.float64)), ctypes.byref(o), threads, _ptr(F), _ptr(x),
-                         _ptr(root), ctypes.byref(st))
+    fi = None if F_init is None else np.ascontiguousarray(F_init, np.float64)
+    rc = lib().orc_solve_warm(n, _ptr(np.ascontiguousarray(rowptr, np.int32)),
+                              _ptr(np.ascontiguousarray(nbr, np.int32)), _ptr(np.ascontiguousarray(cst, np.float64)),
+                              ctypes.byref(o), threads, _ptr(fi), _ptr(F), _ptr(x), _ptr(root), ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f"orc_solve failed: {rc}")
     return {"F": F, "x": x, "comb_root": root, "stats": st.as_dict()}
@@ -233,3 +239,12 @@ def exact(rowptr, nbr, cst, F, d_min=None, opts: Opts | None = None):
                          _ptr(cst), d_min, _ptr(np.ascontiguousarray(F, np.float64)), ctypes.byref(o), _ptr(Fo),
                          _ptr(x), _ptr(hr), ctypes.byref(st))
     return rc, {"F": Fo, "x": x, "half_root": hr, "stats": st.as_dict()}
+
+
+def knn_certify(rowptr, cst, K, d_min, F):
+    """Nodes whose K-th nearest cost is below -d_min F (0: the kNN graph's
+    optimum is the complete graph's)."""
+    n = len(rowptr) - 1
+    return int(lib().orc_knn_certify(n, _ptr(np.ascontiguousarray(rowptr, np.int32)),
+                                     _ptr(np.ascontiguousarray(cst, np.float64)), K, d_min,
+                                     _ptr(np.ascontiguousarray(F, np.float64))))

@@ -73,6 +73,29 @@ def test_cli_files_and_optimum(orc, inst, K, order):
         shutil.rmtree(tmp)
 
 
+def test_cli_adaptive_k(orc):
+    from oracle import tsplib
+    from tests.helpers import morton_order
+    tmp = tempfile.mkdtemp()
+    try:
+        p = _unpack(tmp, "xqf131")
+        say("cli adaptive xqf131 from K=3")
+        r = subprocess.run([CLI, p, "3", "--adaptive"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        rep = open(p + ".top3.F2M_report").read().splitlines()
+        assert rep[0] == "Optimal Found: Yes" and rep[4].startswith("kNN K: ") and rep[4].endswith("(certified)")
+        Kf = int(rep[4].split()[2])
+        assert Kf > 3 and "nodes failing the kNN certificate 0" in r.stdout
+        lab, x, y, _ = tsplib.read(p)
+        pm = morton_order(x, y)
+        rp, nb, c = orc.graph_build(x[pm], y[pm], Kf, 0)
+        ro = orc.solve(rp, nb, c)
+        assert abs(float(rep[1].split(": ")[1]) - ro["stats"]["objective"]) <= 1e-9 * ro["stats"]["objective"]
+        assert orc.knn_certify(rp, c, Kf, ro["stats"]["d_min"], ro["F"]) == 0
+    finally:
+        shutil.rmtree(tmp)
+
+
 def test_known_answer_sra104815_k50(ctx, orc, golden):
     from f2m_amd.solver import default_opts, read_tsplib
     known = [k for k in golden["known"] if k["instance"] == "sra104815"][0]

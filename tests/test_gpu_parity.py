@@ -196,3 +196,52 @@ def test_morton_numbering_same_optimum(ctx, orc, golden):
     x, y, mode, K = W.c1()
     p = morton_order(x, y)
     _full_compare(ctx, orc, x[p], y[p], K, mode, "C1 morton", lp_opt=case["lp_opt"])
+
+
+def test_warm_start_and_knn_certificate(ctx, orc, golden):
+    from f2m_amd.solver import default_opts
+    x, y, mode, K = W.c1()
+    opt = [c for c in golden["lp"] if c["name"] == "C1"][0]["lp_opt"]
+    g4 = ctx.knn_build(x, y, 4, mode)
+    say("solve K=4")
+    ctx.solve(g4)
+    F4 = ctx.result()["F"]
+    v4 = ctx.knn_certify(g4)
+    rp4, nb4, c4 = ctx.graph_csr(g4)
+    r4 = orc.solve(rp4, nb4, c4)
+    assert np.array_equal(F4, r4["F"])
+    assert v4 == orc.knn_certify(rp4, c4, 4, r4["stats"]["d_min"], r4["F"]) > 0
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    say("warm solve K=20 from K=4")
+    st = ctx.solve(g, F_init=F4)
+    res = ctx.result()
+    r = orc.solve(rp, nb, c, F_init=F4)
+    for k in ("sweeps", "glob_rounds", "outer_rounds", "n_combs", "status", "exact_ran", "exact_rounds"):
+        assert st[k] == r["stats"][k], k
+    assert st["glob_rounds"] == 0
+    assert np.array_equal(res["F"], r["F"]) and np.array_equal(res["x"], r["x"])
+    assert st["objective"] == opt
+    assert ctx.knn_certify(g) == orc.knn_certify(rp, c, K, r["stats"]["d_min"], r["F"]) == 0
+    # the warm entry point accepts device memory too
+    import torch
+    Fd = torch.tensor(F4, device="cuda")
+    st2 = ctx.solve(g, F_init=Fd)
+    assert st2["sweeps"] == st["sweeps"] and st2["objective"] == st["objective"]
+    with pytest.raises(RuntimeError):  # a caller CSR has no k
+        gc = ctx.graph_from_csr(rp, nb, c)
+        ctx.solve(gc)
+        ctx.knn_certify(gc)
+
+
+def test_solve_adaptive_certifies_c1(ctx, golden):
+    x, y, mode, K = W.c1()
+    opt = [c for c in golden["lp"] if c["name"] == "C1"][0]["lp_opt"]
+    say("adaptive K from 4")
+    k, st, res, hist = ctx.solve_adaptive(x, y, 4, mode)
+    ks = [h[0] for h in hist]
+    assert ks[0] == 4 and ks == sorted(ks) and hist[-1][2] == 0
+    assert all(h[2] > 0 for h in hist[:-1])
+    # certified: the optimum over all pairs of points, equal to the K=20 graph's
+    assert st["objective"] == opt and st["status"] == 0
+    assert all(h[3] > 0 for h in hist)
