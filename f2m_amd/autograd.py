@@ -1,6 +1,6 @@
 """Differentiable F2M bound: torch.autograd.Function over the C ABI.
 
-    V = F2MBound.apply(px, py, node_off, k, solver, opts)
+    V = F2MBound.apply(px, py, node_off, k, solver, opts, warm)
 
 px, py: float32 CUDA tensors holding the points of a batch of instances
 packed back to back (instance i = rows node_off[i]:node_off[i+1]).  Forward
@@ -15,6 +15,10 @@ constant), scaled by the upstream gradient of the instance.
 The solver context's batch buffers are reused by the next forward on the
 same context: call backward before the next forward, or use one context per
 pending graph.
+
+warm (optional, a WarmStart): keeps each forward's potentials and starts the
+next forward of the same instances from them (f2m_solve_batched_warm), which
+cuts the sweeps of a training loop whose points move a little per step.
 """
 from __future__ import annotations
 
@@ -24,9 +28,17 @@ import torch
 from .solver import COST_RAW_FP32, Context
 
 
+class WarmStart:
+    """Potentials of the last forward (device tensor), reused by the next
+    forward when the batch has the same number of points."""
+
+    def __init__(self):
+        self.F = None
+
+
 class F2MBound(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, px, py, node_off, k, solver: Context, opts=None):
+    def forward(ctx, px, py, node_off, k, solver: Context, opts=None, warm: WarmStart | None = None):
         if not (px.is_cuda and py.is_cuda and px.dtype == torch.float32 and py.dtype == torch.float32):
             raise TypeError("px, py must be float32 CUDA tensors")
         node_off = np.ascontiguousarray(np.asarray(node_off), np.int64)
@@ -34,9 +46,13 @@ class F2MBound(torch.autograd.Function):
         y64 = py.detach().to(torch.float64).contiguous()
         torch.cuda.current_stream().synchronize()
         b = solver.knn_build_batched((x64, node_off), y64, k, COST_RAW_FP32)
-        solver.solve_batched(b, opts)
+        F0 = warm.F if warm is not None and warm.F is not None and warm.F.numel() == b.n_total else None
+        solver.solve_batched(b, opts, F_init=F0)
         xs = torch.empty(b.nslots, dtype=torch.float64, device=px.device)
-        res = solver.batch_result(b, F=np.zeros(b.n_total), x=xs)
+        Fd = torch.empty(b.n_total, dtype=torch.float64, device=px.device)
+        res = solver.batch_result(b, F=Fd, x=xs)
+        if warm is not None:
+            warm.F = Fd
         ctx.save_for_backward(px.detach().contiguous(), py.detach().contiguous(), xs)
         ctx.solver = solver
         ctx.batch = b
@@ -51,10 +67,10 @@ class F2MBound(torch.autograd.Function):
         torch.cuda.current_stream().synchronize()
         ctx.solver.bound_backward(ctx.batch, xs, px, py, scale, gx, gy)
         ctx.solver.synchronize()
-        return gx, gy, None, None, None, None
+        return gx, gy, None, None, None, None, None
 
 
-def f2m_bound(px, py, node_off, k=20, solver: Context | None = None, opts=None):
+def f2m_bound(px, py, node_off, k=20, solver: Context | None = None, opts=None, warm: WarmStart | None = None):
     """Convenience wrapper; creates a context on px's device when none is given."""
     solver = solver or Context(px.device.index or 0)
-    return F2MBound.apply(px, py, node_off, k, solver, opts)
+    return F2MBound.apply(px, py, node_off, k, solver, opts, warm)

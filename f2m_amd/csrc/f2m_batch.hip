@@ -57,6 +57,7 @@ struct BArgs {
   // exact recovery: per slot mirror + flow; per node prices, excesses,
   // lists, certificate terms (arrays of n_total at these bases)
   int64_t n_total;
+  const double* F_init;  // warm start (n_total, normalised), or null
   int32_t* xmir;
   uint8_t* xf;
   double* xp;      // pL | pR | P terms | D terms
@@ -604,14 +605,15 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   }
   for (int v = threadIdx.x; v < I.n; v += BS) {
     I.deg[v] = (int)(I.rp[v + 1] - I.rp[v]);
-    I.F[0][v] = 0.0;
+    I.F[0][v] = a.F_init ? a.F_init[I.nb + v] : 0.0;
     I.frozen[v] = 0;
   }
   __syncthreads();
   z.d_min = dmin;
   Sched<BS, LPN> S{I, o, sh, shi};
   S.eps = o.err_grad * o.err_comb_mult;
-  z.glob_rounds = S.block_glob();
+  // a warm start begins at check_OPT from the given potentials
+  if (!a.F_init) z.glob_rounds = S.block_glob();
   double M = 0.0;
   if (o.stop_phase != 1) z.outer_rounds = S.run_start(M);
   z.max_grad = M;
@@ -880,7 +882,25 @@ f2m_status f2m_knn_build_batched(f2m_ctx* c, int32_t n_inst, const int64_t* node
   return F2M_OK;
 }
 
+static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, const double* F_init,
+                                     f2m_stats* st);
+
 f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, f2m_stats* st) {
+  return solve_batched_impl(c, b, o, nullptr, st);
+}
+
+f2m_status f2m_solve_batched_warm(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, const double* F_init,
+                                  f2m_stats* st) {
+  if (!c || !b || b->n_inst < 1 || !F_init) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  // copied into a context buffer: a stable address keeps the captured graph
+  F2M_HIP(c, c->b_finit.reserve((size_t)b->n_total * 8));
+  F2M_HIP(c, hipMemcpyAsync(c->b_finit.p, F_init, (size_t)b->n_total * 8, hipMemcpyDefault, c->stream));
+  return solve_batched_impl(c, b, o, c->b_finit.as<double>(), st);
+}
+
+static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, const double* F_init,
+                                     f2m_stats* st) {
   if (!c || !b || b->n_inst < 1) return F2M_ERR_ARG;
   F2M_HIP(c, hipSetDevice(c->device));
   f2m_opts oo;
@@ -937,6 +957,10 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   }
   if (const char* e = std::getenv("F2M_BATCH_BS")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
+  {  // the warm-start source is a captured kernel argument
+    const unsigned char* q = reinterpret_cast<const unsigned char*>(&F_init);
+    key.insert(key.end(), q, q + sizeof F_init);
+  }
   if (!c->b_plan) c->b_plan = new BatchPlan;
   BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
   const bool rebuild = P->key != key || !P->exec;
@@ -1008,6 +1032,7 @@ f2m_status f2m_solve_batched(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, 
   a.stats = dstats;
   a.cur_out = curs;
   a.n_total = N;
+  a.F_init = F_init;
   a.xmir = w.xmir.as<int32_t>();
   a.xf = w.xf.as<uint8_t>();
   a.xp = w.xp.as<double>();

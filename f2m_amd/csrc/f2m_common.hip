@@ -260,7 +260,7 @@ void f2m_ctx_destroy(f2m_ctx* c) {
                         &c->k_cstart, &c->k_order, &c->k_sx, &c->k_sy, &c->k_knn, &c->k_kcost,
                         &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->b_node_off,
                         &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_order, &c->b_stats,
-                        &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk, &c->b_ef, &c->b_efoff};
+                        &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk, &c->b_ef, &c->b_efoff, &c->b_finit};
   for (auto* b : all) b->release();
   for (auto& s : c->stage) s.release();
   for (auto& s : c->o_buf) s.release();

@@ -69,7 +69,7 @@ struct f2m_ctx {
   f2m::DevBuf o_buf[5];
   f2m::SolveWs ws;
   // batched storage
-  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc, b_ef, b_efoff;
+  f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc, b_ef, b_efoff, b_finit;
   f2m::SolveWs bws;
   void* b_plan = nullptr;                          // captured launch plan (f2m_batch.hip)
   struct TuneEntry {

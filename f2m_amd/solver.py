@@ -260,10 +260,18 @@ class Context:
         self._check(lib.f2m_batch_copy(self.h, ctypes.byref(b), _ptr(node_off), _ptr(rowptr), _ptr(nbr), _ptr(cost)))
         return node_off, rowptr, nbr, cost
 
-    def solve_batched(self, b: L.Batch | None = None, opts: L.Opts | None = None) -> list:
+    def solve_batched(self, b: L.Batch | None = None, opts: L.Opts | None = None, F_init=None) -> list:
+        """f2m_solve_batched, or f2m_solve_batched_warm from potentials
+        F_init (n_total, normalised; numpy or a CUDA tensor)."""
         b = b or self.batch
         st = (L.Stats * b.n_inst)()
-        self._check(lib.f2m_solve_batched(self.h, ctypes.byref(b), ctypes.byref(opts) if opts else None, st))
+        o = ctypes.byref(opts) if opts else None
+        if F_init is None:
+            self._check(lib.f2m_solve_batched(self.h, ctypes.byref(b), o, st))
+        else:
+            if not hasattr(F_init, "data_ptr"):
+                F_init = np.ascontiguousarray(F_init, np.float64)
+            self._check(lib.f2m_solve_batched_warm(self.h, ctypes.byref(b), o, _ptr(F_init), st))
         return [s.as_dict() for s in st]
 
     def batch_autotune(self, b: L.Batch | None = None, opts: L.Opts | None = None, sample: int = 256,

@@ -260,6 +260,11 @@ f2m_status f2m_knn_build_batched(f2m_ctx* ctx, int32_t n_inst, const int64_t* no
 /* Solves every instance with the plain schedule, all in one persistent launch
  * per phase.  st [host] n_inst stats (may be NULL). */
 f2m_status f2m_solve_batched(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, f2m_stats* st);
+/* Warm start of every instance (new): F_init [dev|host] n_total potentials
+ * (normalised, as f2m_batch_result_copy returns them, e.g. of the previous
+ * forward of a training loop), block_GLOB skipped as in f2m_solve_warm. */
+f2m_status f2m_solve_batched_warm(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, const double* F_init,
+                                  f2m_stats* st);
 /* Autotuner (new): times the schedules (lanes per node 1/2/4/8 x workgroup
  * 256/512/1024 threads) on the first `sample` instances (per-instance device
  * clocks, weighted by workgroup size), keeps the fastest per size class

@@ -235,3 +235,60 @@ def test_batch_exact_on_lattices(ctx, golden):
             opt = [l for l in golden["lattice"] if l["n"] == 2000][0]["lp_opt"]
             assert sb["objective"] == opt
     plain.close()
+
+
+def test_batch_warm_start_matches_plain_warm(ctx, batch_case):
+    from f2m_amd.solver import Context
+    xs, ys = batch_case
+    b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
+    node_off, rowptr, nbr, cost = ctx.batch_csr(b)
+    say("batch cold")
+    cold = ctx.solve_batched(b)
+    F = ctx.batch_result(b)["F"]
+    Fw = F + 0.01 * np.sin(np.arange(len(F)))  # a nearby starting point
+    say("batch warm")
+    warm = ctx.solve_batched(b, F_init=Fw)
+    res = ctx.batch_result(b)
+    plain = Context(0)
+    for i in range(0, len(xs), 5):
+        a, e, sa, se = _slices(node_off, rowptr, i)
+        g = plain.knn_build(xs[i], ys[i], 16, W.COST_NINT)
+        say("plain warm", i)
+        st = plain.solve(g, F_init=Fw[a:e])
+        r = plain.result()
+        assert warm[i]["glob_rounds"] == st["glob_rounds"] == 0
+        for k in ("sweeps", "outer_rounds", "n_combs", "status", "exact_ran", "exact_rounds"):
+            assert warm[i][k] == st[k], (i, k)
+        assert np.array_equal(res["F"][a:e], r["F"]) and np.array_equal(res["x"][sa:se], r["x"])
+        assert abs(warm[i]["objective"] - cold[i]["objective"]) <= 1e-9 * abs(cold[i]["objective"])
+    plain.close()
+    # replays of the warm graph with new starting values (same context buffer)
+    warm2 = ctx.solve_batched(b, F_init=F)
+    assert all(w["status"] == 0 for w in warm2)
+    assert sum(w["sweeps"] for w in warm2) < sum(c["sweeps"] for c in cold)
+
+
+def test_autograd_warm_start(orc):
+    import torch
+
+    from f2m_amd.autograd import F2MBound, WarmStart
+    from f2m_amd.solver import Context
+    solver = Context(0, stream=torch.cuda.current_stream().cuda_stream)
+    insts, _, K = W.c5(count=8, n=400)
+    xs = np.concatenate([a for a, _ in insts])
+    ys = np.concatenate([b for _, b in insts])
+    node_off = np.arange(0, 8 * 400 + 1, 400, dtype=np.int64)
+    warm = WarmStart()
+    rng = np.random.default_rng(3)
+    for step in range(3):
+        jx = (xs + step * 1e-4 * rng.standard_normal(len(xs))).astype(np.float32)
+        jy = (ys + step * 1e-4 * rng.standard_normal(len(ys))).astype(np.float32)
+        px = torch.tensor(jx, device="cuda", requires_grad=True)
+        py = torch.tensor(jy, device="cuda", requires_grad=True)
+        say("autograd warm step", step)
+        Vw = F2MBound.apply(px, py, node_off, K, solver, None, warm)
+        Vw.sum().backward()
+        Vc = F2MBound.apply(px.detach(), py.detach(), node_off, K, solver, None)
+        assert torch.allclose(Vw, Vc, rtol=1e-9, atol=0), step
+        assert warm.F is not None and warm.F.numel() == len(xs)
+    solver.close()
