@@ -32,7 +32,8 @@ identical optima checked on those 16).
 
 fwd_bwd: the differentiable bound on 256 instances of n=2000 (fp32 points
 in the unit square, RAW_FP32 costs, K=20): forward (kNN + batched solve) and
-backward (f2m_bound_backward through torch.autograd).
+backward (f2m_bound_backward through torch.autograd); then one training-loop
+step (points moved by ~1e-4) forwarded warm (WarmStart) and cold.
 """
 from __future__ import annotations
 
@@ -252,11 +253,32 @@ def bench_fwd_bwd():
         if rep and (best is None or t2 - t0 < best[0] + best[1]):
             best = (t1 - t0, t2 - t1)
     log("fwd+bwd", best)
+    # a training-loop step: points moved by ~1e-4, forward warm-started from
+    # the previous step's potentials (WarmStart)
+    from f2m_amd.autograd import WarmStart
+    warm = WarmStart()
+    F2MBound.apply(px.detach(), py.detach(), node_off, K, solver, None, warm)
+    rng = np.random.default_rng(0)
+    qx = (px.detach() + 1e-4 * torch.tensor(rng.standard_normal(px.numel()), dtype=torch.float32, device="cuda"))
+    qy = (py.detach() + 1e-4 * torch.tensor(rng.standard_normal(py.numel()), dtype=torch.float32, device="cuda"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Vw = F2MBound.apply(qx, qy, node_off, K, solver, None, warm)
+    torch.cuda.synchronize()
+    warm_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    Vc = F2MBound.apply(qx, qy, node_off, K, solver, None)
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    same = bool(torch.allclose(Vw, Vc, rtol=1e-9, atol=0))
+    log("training step: warm", warm_ms, "cold", cold_ms, "same optima", same)
     solver.close()
     return {"workload": "256 instances of n=2000, fp32 points in [0,1)^2, RAW_FP32 costs, K=20",
             "fwd_ms": round(best[0] * 1e3, 2), "bwd_ms": round(best[1] * 1e3, 3),
             "fwd_bwd_ms": round((best[0] + best[1]) * 1e3, 2),
-            "fwd_bwd_over_fwd": round((best[0] + best[1]) / best[0], 4)}
+            "fwd_bwd_over_fwd": round((best[0] + best[1]) / best[0], 4),
+            "step_fwd_warm_ms": round(warm_ms, 2), "step_fwd_cold_ms": round(cold_ms, 2),
+            "step_same_optima": same}
 
 
 if __name__ == "__main__":

@@ -612,8 +612,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   z.d_min = dmin;
   Sched<BS, LPN> S{I, o, sh, shi};
   S.eps = o.err_grad * o.err_comb_mult;
-  // a warm start begins at check_OPT from the given potentials
-  if (!a.F_init) z.glob_rounds = S.block_glob();
+  z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
   if (o.stop_phase != 1) z.outer_rounds = S.run_start(M);
   z.max_grad = M;

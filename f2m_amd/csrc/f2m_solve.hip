@@ -1088,12 +1088,11 @@ f2m_status solve_graph(f2m_ctx* c, SolveWs& w, const f2m_graph* g, const f2m_opt
   f2m_stats z;
   std::memset(&z, 0, sizeof z);
   z.d_min = R.dmin;
-  if (F_init) {  // warm start: the given potentials, schedule from check_OPT
+  if (F_init) {  // warm start: the same schedule from the given potentials
     F2M_HIP(c, hipMemcpyAsync(w.F0.p, F_init, (size_t)g->n * 8, hipMemcpyDeviceToDevice, c->stream));
     w.cur = 0;
-  } else if ((s = R.block_glob(z.glob_rounds))) {
-    return s;
   }
+  if ((s = R.block_glob(z.glob_rounds))) return s;
   double M = 0;
   if (o.stop_phase != 1)
     if ((s = R.run_start(M, z.outer_rounds))) return s;

@@ -162,8 +162,8 @@ class Context:
 
     # ---- solve -----------------------------------------------------------
     def solve(self, g: L.Graph | None = None, opts: L.Opts | None = None, F_init=None) -> dict:
-        """f2m_solve, or f2m_solve_warm from potentials F_init (normalised, as
-        result()["F"] returns them)."""
+        """f2m_solve, or f2m_solve_warm: the same schedule from potentials
+        F_init (normalised, as result()["F"] returns them) instead of zero."""
         g = g or self.graph
         st = L.Stats()
         o = ctypes.byref(opts) if opts else None

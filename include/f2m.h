@@ -223,9 +223,11 @@ f2m_status f2m_graph_copy(f2m_ctx* ctx, const f2m_graph* g, int32_t* rowptr, int
  * recovery, objective.  Results stay on the device; copy with f2m_result_copy. */
 f2m_status f2m_solve(f2m_ctx* ctx, const f2m_graph* g, const f2m_opts* o, f2m_stats* st);
 /* Warm start (new; the reference always starts from F = 0 in block_GLOB,
- * graph_cuda.cu:363-413): the same solve from potentials F_init [dev|host] n
- * (normalised, as f2m_result_copy returns them, e.g. of the same points with a
- * smaller k), skipping block_GLOB (stats.glob_rounds = 0). */
+ * graph_cuda.cu:363-413): the same schedule, block_GLOB included, from
+ * potentials F_init [dev|host] n (normalised, as f2m_result_copy returns them,
+ * e.g. of the same points with a smaller k, or moved slightly).  Skipping
+ * block_GLOB as well is not offered: from a perturbed optimum the Jacobi
+ * descent then failed to converge on some instances. */
 f2m_status f2m_solve_warm(f2m_ctx* ctx, const f2m_graph* g, const f2m_opts* o, const double* F_init,
                           f2m_stats* st);
 /* kNN restriction certificate (new; the reference reports the MAX edge rank,
@@ -262,7 +264,7 @@ f2m_status f2m_knn_build_batched(f2m_ctx* ctx, int32_t n_inst, const int64_t* no
 f2m_status f2m_solve_batched(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, f2m_stats* st);
 /* Warm start of every instance (new): F_init [dev|host] n_total potentials
  * (normalised, as f2m_batch_result_copy returns them, e.g. of the previous
- * forward of a training loop), block_GLOB skipped as in f2m_solve_warm. */
+ * forward of a training loop), as f2m_solve_warm. */
 f2m_status f2m_solve_batched_warm(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, const double* F_init,
                                   f2m_stats* st);
 /* Autotuner (new): times the schedules (lanes per node 1/2/4/8 x workgroup

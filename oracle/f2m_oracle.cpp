@@ -657,13 +657,14 @@ int orc_solve_warm(int n, const int32_t* rowptr, const int32_t* nbr, const doubl
   z.d_min = S.dmin;
   if (!(S.dmin > 0)) { if (st) *st = z; return -2; }
 
-  // graph::RUN (graph_cuda.cu:47-155); a warm start (new) begins at check_OPT
-  // from the given potentials instead of block_GLOB from zero
+  // graph::RUN (graph_cuda.cu:47-155); a warm start (new) runs the same
+  // schedule from the given potentials instead of zero.  (Skipping block_GLOB
+  // as well was tried: from a perturbed optimum the Jacobi descent then failed
+  // to reach max|g| < 1e-7 on some instances, running to the outer-round cap.)
   if (F_init) {
     for (int v = 0; v < n; ++v) S.F[v] = F_init[v];
-  } else {
-    z.glob_rounds = S.block_glob();
   }
+  z.glob_rounds = S.block_glob();
   double M = 0;
   if (o.stop_phase != 1) z.outer_rounds = S.run_start(M);
   z.max_grad = M;

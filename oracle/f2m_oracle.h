@@ -113,8 +113,9 @@ int orc_solve(int n, const int32_t* rowptr, const int32_t* nbr, const double* co
               const orc_opts* o, int nthreads, double* F_out, double* x_out,
               int32_t* comb_root, orc_stats* st);
 
-/* --- warm start (new): the schedule from potentials F_init (normalised, as
- * F_out), skipping block_GLOB (graph_cuda.cu:363-413); glob_rounds = 0. */
+/* --- warm start (new): the same schedule (block_GLOB included,
+ * graph_cuda.cu:363-413) from potentials F_init (normalised, as F_out)
+ * instead of zero. */
 int orc_solve_warm(int n, const int32_t* rowptr, const int32_t* nbr, const double* cost,
                    const orc_opts* o, int nthreads, const double* F_init, double* F_out, double* x_out,
                    int32_t* comb_root, orc_stats* st);

