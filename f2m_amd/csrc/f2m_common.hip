@@ -247,7 +247,7 @@ static void release_ws(f2m::SolveWs& w) {
   f2m::DevBuf* all[] = {&w.nbr, &w.D, &w.orig, &w.F0, &w.F1, &w.g, &w.act, &w.clen, &w.deg,
                         &w.A, &w.T, &w.root, &w.irr, &w.frozen, &w.xw, &w.xc, &w.red, &w.axb,
                         &w.sortk, &w.sortv, &w.sortk2, &w.sortv2, &w.sorttmp, &w.ef, &w.efoff, &w.psl, &w.povo, &w.ptmp,
-                        &w.xmir, &w.xf, &w.xp, &w.xe, &w.xact, &w.xcnt};
+                        &w.xmir, &w.xf, &w.xp, &w.xe, &w.xact, &w.xcnt, &w.thdr, &w.tnode, &w.tslot, &w.tovf};
   for (auto* b : all) b->release();
 }
 

@@ -47,6 +47,7 @@ struct SolveWs {
   DevBuf ef, efoff;             // compacted active lists (16-byte slots) + offsets
   DevBuf psl, povo, ptmp;       // padded active lists (v*8 + k, overflow after n*8) + offsets
   DevBuf xmir, xf, xp, xe, xact, xcnt;  // exact recovery: mirror slots, flows, prices, excesses, lists
+  DevBuf thdr, tnode, tslot, tovf;      // spatial tiles for multi-sweep launches (f2m_solve.hip)
   int cur = 0;                  // which F buffer holds the current potentials
   int64_t n = 0, m = 0;
 };
@@ -151,14 +152,23 @@ __device__ __forceinline__ double dval(double D, double Fv, double Fw, int v, in
   return t + 0.5 * b;
 }
 
-// Three smallest values, sentinels 1e6 (graph_cuda_grad.hpp:68-79).
+// Three smallest values, sentinels 1e6 (graph_cuda_grad.hpp:68-79).  The
+// reference inserts with a branch chain (v < a: shift both; else v < b; else
+// v < c); the min/max network below keeps the same three values: min and
+// max return one of their operands, and ties between equal values only
+// differ by the sign of a zero.  A reduced cost is never -0.0 (D >= +0, and a
+// round-to-nearest sum that is zero with a +0 or nonzero operand is +0) nor
+// NaN, so the bits equal the branch chain's.  5 v_min/v_max_f64 per value
+// instead of 3 compares and up to 6 selects of double pairs.
 struct Top3 {
   double a, b, c;
   __device__ __forceinline__ Top3() : a(1e6), b(1e6), c(1e6) {}
   __device__ __forceinline__ void push(double v) {
-    if (v < a) { c = b; b = a; a = v; }
-    else if (v < b) { c = b; b = v; }
-    else if (v < c) { c = v; }
+    const double ha = __builtin_fmax(a, v);
+    const double hb = __builtin_fmax(b, v);
+    a = __builtin_fmin(a, v);
+    c = __builtin_fmin(c, hb);
+    b = __builtin_fmin(b, ha);
   }
   __device__ __forceinline__ double grad() const { return -(b + c) / 2; }
 };

@@ -49,6 +49,9 @@ __global__ void s_zero_u8(uint8_t* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+__global__ void s_set_i32(int32_t* p, int32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
 __global__ void s_zero_f64(double* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -288,6 +291,329 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
     }
   }
   if (maxslots) block_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
+}
+
+// ---- spatial tiles: several Jacobi sweeps per launch -------------------------
+// A sweep launch costs ~4.5 us at n = 100k however little it moves (two
+// dependent HBM levels plus the wave ramp), so the schedule's sweeps are run S
+// at a time.  Tile t is the id range [t*TS, t*TS + TS): with Morton-numbered
+// nodes a compact region.  Its local set is the tile plus a halo: nodes
+// reachable from the tile over active slots in at most depth edges, numbered
+// breadth first (local index = id - t*TS inside the tile).  A launch of S
+// sweeps (S <= depth + 1) gathers the potentials of the local set's active
+// neighbours from HBM once, computes sweep 1 for every local node of depth
+// <= S-1, sweep 2 from shared memory for depth <= S-2, ..., sweep S for the
+// tile, and writes only the tile's final potentials.  The halo is recomputed
+// by every tile that needs it: the arithmetic per node and sweep is the one
+// of s_sweep_pad, so the potentials are bit-identical to S single sweeps.
+constexpr int TS = 256;      // nodes per tile
+constexpr int TW = 384;      // threads per tile workgroup (tile + typical halo)
+constexpr int TL = 2 * TW;   // local-node capacity (tile + halo)
+constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
+constexpr int TOV = 1024;    // overflow slots per tile (beyond the padded ones)
+constexpr int THB = 11;      // log2 of the halo hash size
+constexpr int THSZ = 1 << THB;
+constexpr int THDR = 16;     // header ints per tile
+constexpr int TMAXS = 8;     // most sweeps per launch
+constexpr int DEFAULT_FUSE = 6;  // sweeps per launch unless F2M_FUSE says otherwise
+static_assert((TOV & (TOV - 1)) == 0 && TOV >= TW, "TOV is a power of two, one speculative record per thread");
+static_assert(TL < 32768 && THSZ >= 2 * TL, "local indices fit a short; hash has room");
+
+// local copy of an active slot: loc = local index of nbr (-1: not local)
+struct TSlot {
+  double D;
+  int32_t nbr;
+  int32_t loc;
+};
+
+__device__ __forceinline__ unsigned t_hash(int w) { return ((unsigned)w * 2654435761u) >> (32 - THB); }
+// local index clamped into the buffers (-1 only occurs where it is not read)
+__device__ __forceinline__ unsigned t_loc(int loc) { return min((unsigned)loc, (unsigned)(TL - 1)); }
+
+__device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) {
+  __shared__ double wm[TW / 64];
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wm[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int k = 1; k < TW / 64; ++k) m = fmax(m, wm[k]);
+    atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// One workgroup per tile: breadth first from the tile over active slots, one
+// level per pass, up to `depth` levels; fit = the deepest level kept (levels
+// are kept while the local set fits TL and the hash has room).
+// hdr[t*THDR + d] = local nodes of level <= d (d <= fit); hdr[t*THDR + 8] =
+// overflow slots; hdr[t*THDR + 9] = fit (-1 when the overflow slots do not
+// fit).  tnode[t*TL + j] = {id, active slots, overflow offset, 0}; slots k <
+// tpp of local node j at tslot[(t*TL + j)*tpp + k], the others at
+// tovf[t*TOV + offset + k - tpp].  *fit_min = min over tiles of fit: a launch
+// may run up to fit_min + 1 sweeps.
+__global__ void __launch_bounds__(TW) s_tile_build(int n, int pp, int tpp, int depth, const int32_t* __restrict__ act,
+                                                   const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
+                                                   int32_t* __restrict__ hdr, int4* __restrict__ tnode,
+                                                   TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
+                                                   int32_t* __restrict__ fit_min) {
+  __shared__ int hkey[THSZ];
+  __shared__ short hval[THSZ];
+  __shared__ int lg[TL];
+  __shared__ int s_nl, s_full, s_ov;
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t0 = t * TS, c0 = min(TS, n - t0);
+  auto slot_at = [&](int v, int k) -> const ESlot& {
+    return psl[k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + povo[v] + (k - pp)];
+  };
+  for (int i = tid; i < THSZ; i += TW) {
+    hkey[i] = -1;
+    hval[i] = -1;
+  }
+  for (int j = tid; j < c0; j += TW) lg[j] = t0 + j;
+  if (tid == 0) {
+    s_nl = c0;
+    s_full = 0;
+    s_ov = 0;
+    hdr[t * THDR] = c0;
+  }
+  __syncthreads();
+  int lo = 0, hi = c0, fit = 0;
+  for (int d = 0; d < depth; ++d) {
+    for (int j = lo + tid; j < hi; j += TW) {
+      const int v = lg[j], L = act[v];
+      int wk[PP_MAX];  // the padded slots' neighbours, loaded together
+#pragma unroll
+      for (int k = 0; k < PP_MAX; ++k) wk[k] = (k < pp && k < L) ? psl[(int64_t)v * pp + k].nbr : 0;
+      for (int k = 0; k < L; ++k) {
+        int w = 0;
+#pragma unroll
+        for (int q = 0; q < PP_MAX; ++q)
+          if (q == k) w = wk[q];
+        if (k >= pp) w = slot_at(v, k).nbr;
+        if (w >= t0 && w < t0 + c0) continue;
+        unsigned h = t_hash(w);
+        int probe = 0;
+        for (; probe < THSZ; ++probe) {
+          const int prev = atomicCAS(&hkey[h], -1, w);
+          if (prev == -1) {
+            const int idx = atomicAdd(&s_nl, 1);
+            if (idx < TL) {
+              lg[idx] = w;
+              hval[h] = (short)idx;
+            } else {
+              s_full = 1;
+            }
+            break;
+          }
+          if (prev == w) break;
+          h = (h + 1) & (THSZ - 1);
+        }
+        if (probe == THSZ) s_full = 1;
+      }
+    }
+    __syncthreads();
+    const int nl = s_nl;
+    const bool full = s_full != 0;
+    __syncthreads();
+    if (full) break;  // level d+1 incomplete: keep levels <= d
+    lo = hi;
+    hi = nl;
+    fit = d + 1;
+    if (tid == 0) hdr[t * THDR + d + 1] = hi;
+  }
+  for (int j = tid; j < hi; j += TW) {
+    const int v = lg[j], L = act[v];
+    int ovo = 0;
+    if (L > tpp) ovo = atomicAdd(&s_ov, L - tpp);
+    tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, 0);
+    ESlot ek[PP_MAX];  // the padded slots, loaded together
+#pragma unroll
+    for (int k = 0; k < PP_MAX; ++k)
+      if (k < pp && k < L) ek[k] = psl[(int64_t)v * pp + k];
+    for (int k = 0; k < L; ++k) {
+      ESlot e;
+#pragma unroll
+      for (int q = 0; q < PP_MAX; ++q)
+        if (q == k) e = ek[q];
+      if (k >= pp) e = slot_at(v, k);
+      int loc = -1;
+      if (e.nbr >= t0 && e.nbr < t0 + c0) {
+        loc = e.nbr - t0;
+      } else {
+        unsigned h = t_hash(e.nbr);
+        for (int probe = 0; probe < THSZ; ++probe) {
+          const int key = hkey[h];
+          if (key == e.nbr) {
+            loc = hval[h];
+            break;
+          }
+          if (key == -1) break;
+          h = (h + 1) & (THSZ - 1);
+        }
+        if (loc >= hi) loc = -1;  // from an incomplete level (neighbour of the deepest kept level)
+      }
+      const TSlot ts{e.D, e.nbr, loc};
+      if (k < tpp) tslot[((int64_t)t * TL + j) * tpp + k] = ts;
+      else if (ovo + (k - tpp) < TOV) tovf[(int64_t)t * TOV + ovo + (k - tpp)] = ts;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int f = s_ov > TOV ? -1 : fit;
+    hdr[t * THDR + 8] = min(s_ov, TOV);
+    hdr[t * THDR + 9] = f;
+    atomicMin(fit_min, f);
+  }
+}
+
+// Reduced cost of slot (v -> w) from the halves hv = 0.5*F[v], hw = 0.5*F[w]:
+// the operations of dval (0.5*F is exact), so the same bits.
+__device__ __forceinline__ double dval_h(double D, double hv, double hw, int v, int w) {
+  const bool lo = v < w;
+  return (D + (lo ? hv : hw)) + (lo ? hw : hv);
+}
+// Top3 over the padded slots, every slot pushed: a slot past the list
+// pushes the sentinel 1e6, which leaves the three values unchanged (each is
+// <= 1e6), so no lane diverges.
+template <int TPP>
+__device__ __forceinline__ void push_padded(Top3& t3, const TSlot* sl, const double* hn, double hv, int v, int L) {
+#pragma unroll
+  for (int k = 0; k < TPP; ++k) {
+    const double d = dval_h(sl[k].D, hv, hn[k], v, sl[k].nbr);
+    t3.push(k < L ? d : 1e6);
+  }
+}
+
+// S sweeps over the tiles (S <= fit_min + 1); Fin -> Fout, intermediate
+// sweeps in shared memory.  maxslots: max |g| of the last sweep, as
+// s_sweep_pad.  Two dependent HBM levels: the records (the first TW local
+// nodes' before the header is read: the tile and most halos), then the
+// potentials of every slot.  Overflow slots (beyond TPP, which is chosen
+// above the lists' usual length) are staged in shared memory with their
+// potentials, the first TW of them loaded with the records; the rest (a
+// third level) only when a tile has more.  Sweeps 2..S read shared memory.
+template <int S, bool FROZEN, int TPP>
+__global__ void __launch_bounds__(TW) s_sweep_tile(const int32_t* __restrict__ hdr, const int4* __restrict__ tnode,
+                                                   const TSlot* __restrict__ tslot, const TSlot* __restrict__ tovf,
+                                                   const double* __restrict__ Fin, double* __restrict__ Fout,
+                                                   const uint8_t* __restrict__ frozen,
+                                                   unsigned long long* __restrict__ maxslots) {
+  constexpr int NP = TL / TW;
+  __shared__ double fb[2][TL];
+  __shared__ TSlot ov[TOV];
+  __shared__ double ovF[TOV];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int32_t* h = hdr + t * THDR;
+  const TSlot orec = tovf[(int64_t)t * TOV + tid];  // speculative: tid < TW <= TOV
+  int cnt[S];  // local nodes of level <= d, read once (kept in registers across the barriers)
+#pragma unroll
+  for (int d = 0; d < S; ++d) cnt[d] = h[d];
+  const int nloc = cnt[S - 1];
+  const int nov = h[8];
+  int4 nd[NP];
+  TSlot sl[NP][TPP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int j = tid + p * TW;
+    nd[p] = make_int4(0, 0, 0, 0);
+    if (p == 0 || j < nloc) {
+      const int64_t r = (int64_t)t * TL + j;
+      nd[p] = tnode[r];
+#pragma unroll
+      for (int k = 0; k < TPP; ++k) sl[p][k] = tslot[r * TPP + k];
+    }
+  }
+  double fv[NP], fw[NP][TPP];
+  bool fz[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int j = tid + p * TW;
+    fz[p] = false;
+    fv[p] = 0.0;
+    if (j < nloc) {
+      fv[p] = Fin[nd[p].x];
+      if (FROZEN) fz[p] = frozen[nd[p].x] != 0;
+#pragma unroll
+      for (int k = 0; k < TPP; ++k)
+        if (k < nd[p].y) fw[p][k] = Fin[sl[p][k].nbr];
+    }
+  }
+  double of = 0.0;
+  if (tid < nov) of = Fin[orec.nbr];
+  for (int i = tid + TW; i < nov; i += TW) {
+    const TSlot o = tovf[(int64_t)t * TOV + i];
+    ov[i] = o;
+    ovF[i] = Fin[o.nbr];
+  }
+  if (tid < nov) {
+    ov[tid] = orec;
+    ovF[tid] = of;
+  }
+  __syncthreads();
+  // sweep 1: every local node of level <= S-1
+  double gabs = 0.0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int j = tid + p * TW;
+    if (j < nloc) {
+      const int v = nd[p].x, L = nd[p].y;
+      const double hv = 0.5 * fv[p];
+      double hn[TPP];
+#pragma unroll
+      for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * (k < L ? fw[p][k] : 0.0);
+      Top3 t3;
+      push_padded<TPP>(t3, sl[p], hn, hv, v, L);
+#pragma unroll 1
+      for (int k = TPP; k < L; ++k) {
+        const int i = (nd[p].z + k - TPP) & (TOV - 1);
+        t3.push(dval(ov[i].D, fv[p], ovF[i], v, ov[i].nbr));
+      }
+      const double g = t3.grad();
+      const double f = (FROZEN && fz[p]) ? fv[p] : fv[p] + g;
+      if (S == 1) {
+        Fout[v] = f;
+        gabs = fmax(gabs, fabs(g));
+      } else {
+        fb[0][j] = f;
+      }
+    }
+  }
+  // sweeps 2..S from shared memory: sweep q for the local nodes of level <= S-q
+#pragma unroll
+  for (int q = 2; q <= S; ++q) {
+    __syncthreads();
+    const double* src = fb[(q - 2) & 1];
+    double* dst = fb[(q - 1) & 1];
+    const int cq = cnt[S - q];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int j = tid + p * TW;
+      if (j < cq) {
+        const int v = nd[p].x, L = nd[p].y;
+        const double f0 = src[j];
+        double hn[TPP];  // all reads first (slots past L read a clamped, unused entry)
+#pragma unroll
+        for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * src[t_loc(sl[p][k].loc)];
+        Top3 t3;
+        push_padded<TPP>(t3, sl[p], hn, 0.5 * f0, v, L);
+#pragma unroll 1
+        for (int k = TPP; k < L; ++k) {
+          const TSlot o = ov[(nd[p].z + k - TPP) & (TOV - 1)];
+          t3.push(dval(o.D, f0, src[t_loc(o.loc)], v, o.nbr));
+        }
+        const double g = t3.grad();
+        const double f = (FROZEN && fz[p]) ? f0 : f0 + g;
+        if (q == S) {
+          Fout[v] = f;
+          gabs = fmax(gabs, fabs(g));
+        } else {
+          dst[j] = f;
+        }
+      }
+    }
+  }
+  if (maxslots) tile_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
 }
 
 // cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
@@ -567,6 +893,15 @@ struct Runner {
   bool padded = true;  // padded layout (default) or compacted lists (F2M_SWEEP=compact)
   int pad_cfg = 2;     // padded variant (stride,lanes): see launch_padded; default 4,2
   int pp = 4;          // padded stride of pad_cfg
+  // multi-sweep launches over spatial tiles (padded layout only): up to `fuse`
+  // sweeps per launch (1: one sweep per launch, s_sweep_pad); the tiles are
+  // rebuilt with the lists, and usable once their overflow flag reads clear
+  int fuse = 1;
+  int ntiles = 0;
+  bool tiles_pending = false;
+  int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
+  int tpp = 4;    // padded slots per local node on the current lists
+  int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
   // with profiling on: sweeps run on the current lists not yet counted, and a
   // device accumulator of active slots x sweeps (no host round trip per list)
@@ -648,7 +983,20 @@ struct Runner {
     }
     static const int strides[] = {8, 4, 4, 8, 8, 4, 2, 6, 4, 2, 3};
     pp = strides[pad_cfg];
+    fuse = padded ? std::max(1, std::min(TMAXS, env_int("F2M_FUSE", DEFAULT_FUSE))) : 1;
+    if (fuse > 1) {
+      ntiles = (n + TS - 1) / TS;
+      F2M_HIP(c, w.thdr.reserve(((int64_t)ntiles * THDR + 16) * 4));
+      F2M_HIP(c, w.tnode.reserve((int64_t)ntiles * TL * 16));
+      F2M_HIP(c, w.tslot.reserve((int64_t)ntiles * TL * TPP_MAX * 16));
+      F2M_HIP(c, w.tovf.reserve((int64_t)ntiles * TOV * 16));
+    }
     return F2M_OK;
+  }
+  int32_t* tile_fit() const { return w.thdr.as<int32_t>() + (int64_t)ntiles * THDR; }
+  void set_width(int32_t fit_min) {
+    width = std::max(1, std::min(fuse, fit_min + 1));
+    tiles_pending = false;
   }
 
   template <int LPN>
@@ -706,7 +1054,8 @@ struct Runner {
     }
   }
   f2m_status launch_sweep(bool write, bool frozen, unsigned long long* mslots) {
-    const bool timed = write && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
+    const bool timed = write && fuse == 1 && c->prof_every > 0 && (sweeps % c->prof_every) == 0;
+    if (timed) ++sampled_sweeps;
     if (timed) {
       while (c->prof_ev.size() < 2 * (prof_used + 1)) {
         hipEvent_t e;
@@ -732,6 +1081,7 @@ struct Runner {
     if (write) {
       w.cur ^= 1;
       ++sweeps;
+      ++launches;
       if (c->prof_every > 0) ++pending;
     }
     return F2M_OK;
@@ -765,7 +1115,9 @@ struct Runner {
       F2M_HIP(c, hipEventElapsedTime(&t, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
       ms += t;
     }
-    c->prof.sweep_launches = sweeps;
+    c->prof.sweep_launches = launches;
+    c->prof.sweeps = sweeps;
+    c->prof.sampled_sweeps = sampled_sweeps;
     c->prof.sweep_samples = (int64_t)prof_used;
     c->prof.sweep_ms_sampled = ms;
     c->prof.sweep_alg_bytes = alg_bytes;
@@ -775,7 +1127,7 @@ struct Runner {
   // update_EF (graph_cuda.cu:229-252): offsets = scan(act), copy the active
   // prefixes.  With profiling on, the sweeps run on the old lists are counted
   // first (on the device).
-  f2m_status ef_rebuild() {
+  f2m_status ef_rebuild(bool long_lists = false) {
     f2m_status st0 = flush_slots();
     if (st0) return st0;
     F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
@@ -785,6 +1137,16 @@ struct Runner {
       F2M_HIP(c, scan_i32_to_i32(c, I(w.ptmp), I(w.povo), n, nullptr));
       hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, pp, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
                          w.psl.as<ESlot>());
+      if (fuse > 1) {
+        tpp = long_lists ? 6 : 4;
+        hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)(fuse - 1));
+        hipLaunchKernelGGL(s_tile_build, dim3(ntiles), dim3(TW), 0, c->stream, n, pp, tpp, fuse - 1,
+                           (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
+                           w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
+                           tile_fit());
+        tiles_pending = true;
+        width = 1;
+      }
     } else {
       hipLaunchKernelGGL(s_ef_fill, gn, bs, 0, c->stream, n, rp, I(w.act), I(w.efoff), I(w.nbr), w.D.as<double>(),
                          w.ef.as<ESlot>());
@@ -796,7 +1158,11 @@ struct Runner {
   f2m_status read_max(int r, double& out) {
     unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
     F2M_HIP(c, hipMemcpyAsync(hp, maxs + 512 * r, 512 * 8, hipMemcpyDeviceToHost, c->stream));
+    int32_t* hfit = reinterpret_cast<int32_t*>(hp + 1024);
+    const bool tp = tiles_pending;
+    if (tp) F2M_HIP(c, hipMemcpyAsync(hfit, tile_fit(), 4, hipMemcpyDeviceToHost, c->stream));
     F2M_HIP(c, hipStreamSynchronize(c->stream));
+    if (tp) set_width(*hfit);
     unsigned long long b = 0;
     for (int k = 0; k < 32; ++k) b = hp[16 * k] > b ? hp[16 * k] : b;
     std::memcpy(&out, &b, 8);
@@ -814,13 +1180,96 @@ struct Runner {
     if ((s = launch_sweep(false, false, maxs))) return s;
     return read_max(0, M);
   }
-  f2m_status sweep_n(int count, bool frozen) {
-    for (int i = 0; i < count; ++i) {
-      f2m_status s = launch_sweep(true, frozen, nullptr);
-      if (s) return s;
+  // sweeps per launch on the current lists (reads the tiles' fit when the
+  // read-back of a max has not already done so)
+  f2m_status tile_width(int& out) {
+    if (tiles_pending) {
+      int32_t* hfit = reinterpret_cast<int32_t*>(static_cast<unsigned long long*>(c->pinned) + 1024);
+      F2M_HIP(c, hipMemcpyAsync(hfit, tile_fit(), 4, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipStreamSynchronize(c->stream));
+      set_width(*hfit);
     }
+    out = width;
     return F2M_OK;
   }
+  template <int S>
+  void launch_tile_t(bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
+    auto go = [&](auto kern) {
+      if (ev0)
+        hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, ev0, ev1, 0,
+                              (const int32_t*)w.thdr.as<int32_t>(), (const int4*)w.tnode.as<int4>(),
+                              (const TSlot*)w.tslot.as<TSlot>(), (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(),
+                              Fo(), (const uint8_t*)w.frozen.as<uint8_t>(), mslots);
+      else
+        hipLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, (const int32_t*)w.thdr.as<int32_t>(),
+                           (const int4*)w.tnode.as<int4>(), (const TSlot*)w.tslot.as<TSlot>(),
+                           (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(), Fo(),
+                           (const uint8_t*)w.frozen.as<uint8_t>(), mslots);
+    };
+    if (tpp == 6) {
+      if (frozen) go(s_sweep_tile<S, true, 6>);
+      else go(s_sweep_tile<S, false, 6>);
+    } else {
+      if (frozen) go(s_sweep_tile<S, true, 4>);
+      else go(s_sweep_tile<S, false, 4>);
+    }
+  }
+  // S sweeps in one launch over the tiles; timed like launch_sweep (only
+  // launches of the lists' full width are sampled)
+  f2m_status launch_tile(int S, bool frozen, unsigned long long* mslots) {
+    const bool wide = S == width;
+    const bool timed = wide && c->prof_every > 0 && (wide_launches % c->prof_every) == 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+      while (c->prof_ev.size() < 2 * (prof_used + 1)) {
+        hipEvent_t e;
+        F2M_HIP(c, hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+      }
+      e0 = c->prof_ev[2 * prof_used];
+      e1 = c->prof_ev[2 * prof_used + 1];
+    }
+    switch (S) {
+      case 1: launch_tile_t<1>(frozen, mslots, e0, e1); break;
+      case 2: launch_tile_t<2>(frozen, mslots, e0, e1); break;
+      case 3: launch_tile_t<3>(frozen, mslots, e0, e1); break;
+      case 4: launch_tile_t<4>(frozen, mslots, e0, e1); break;
+      case 5: launch_tile_t<5>(frozen, mslots, e0, e1); break;
+      case 6: launch_tile_t<6>(frozen, mslots, e0, e1); break;
+      case 7: launch_tile_t<7>(frozen, mslots, e0, e1); break;
+      default: launch_tile_t<8>(frozen, mslots, e0, e1); break;
+    }
+    F2M_LAUNCH_CHECK(c);
+    if (timed) {
+      ++prof_used;
+      sampled_sweeps += S;
+    }
+    if (wide) ++wide_launches;
+    ++launches;
+    w.cur ^= 1;
+    sweeps += S;
+    if (c->prof_every > 0) pending += S;
+    return F2M_OK;
+  }
+  // `count` sweeps; mlast (when set) receives max |g| of the last one
+  f2m_status run_sweeps(int count, bool frozen, unsigned long long* mlast) {
+    f2m_status s;
+    int wd = 1;
+    if (fuse > 1 && count > 1)
+      if ((s = tile_width(wd))) return s;
+    if (wd > 1) {
+      for (int done = 0; done < count;) {
+        const int S = std::min(wd, count - done);
+        done += S;
+        if ((s = launch_tile(S, frozen, done == count ? mlast : nullptr))) return s;
+      }
+      return F2M_OK;
+    }
+    for (int i = 0; i < count; ++i)
+      if ((s = launch_sweep(true, frozen, i + 1 == count ? mlast : nullptr))) return s;
+    return F2M_OK;
+  }
+  f2m_status sweep_n(int count, bool frozen) { return run_sweeps(count, frozen, nullptr); }
 
   // block_GLOB (graph_cuda.cu:363-413)
   f2m_status block_glob(int32_t& rounds) {
@@ -828,13 +1277,14 @@ struct Runner {
     F2M_LAUNCH_CHECK(c);
     int go = 0;
     f2m_status s;
-    if ((s = ef_rebuild())) return s;
+    const bool long_lists = o.glob_k > 4;  // every list holds min(deg, glob_k) or more slots here
+    if ((s = ef_rebuild(long_lists))) return s;
     for (; go < o.glob_rounds; ++go) {
       if ((s = sweep_n(o.glob_sweeps, false))) return s;
       hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
                          o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
       F2M_LAUNCH_CHECK(c);
-      if ((s = ef_rebuild())) return s;
+      if ((s = ef_rebuild(long_lists))) return s;
       double M;
       if ((s = grad_max(M))) return s;
       if (M < o.glob_exit) { ++go; break; }
@@ -863,16 +1313,15 @@ struct Runner {
     int done = 0;
     while (done < N) {
       int chunk = std::min(o.check_every, N - done);
-      if ((s = sweep_n(chunk - 1, frozen))) return s;
-      if (chunk == o.check_every) {
+      if (chunk == o.check_every) {  // max |g| of the chunk's last sweep
         if ((s = zero_region(1))) return s;
-        if ((s = launch_sweep(true, frozen, maxs + 512))) return s;
+        if ((s = run_sweeps(chunk, frozen, maxs + 512))) return s;
         double MB;
         if ((s = read_max(1, MB))) return s;
         done += chunk;
         if (MB < o.err_grad) break;
       } else {
-        if ((s = launch_sweep(true, frozen, nullptr))) return s;
+        if ((s = run_sweeps(chunk, frozen, nullptr))) return s;
         done += chunk;
       }
     }
