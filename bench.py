@@ -13,16 +13,22 @@ reference's "Time:" excludes graph construction (graph_cuda.cu:51,131).  The
 same solve in input order is timed as well (input_order).
 Prints one JSON line.  `value` = ms per solve (lower is better).
 
-roofline: the dominant kernel is the fused sweep (s_sweep_pad).  Its
-algorithmic bytes per launch are 12 B per active slot (D double + nbr int32)
-+ 20 B per node (list offset, F in, F out); the average kernel duration comes
-from the HIP events the library records at the start and end of every 16th
-sweep kernel of the timed solves (hipExtLaunchKernelGGL, on its stream).
-traffic is read from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE
-per sweep launch, scaled by a copy kernel of known size) when present.
+The timed solves run without instrumentation.
+
+roofline: the dominant kernel is the multi-sweep launch over spatial tiles
+(s_sweep_tile, F2M_FUSE sweeps per launch, default 6).  Its algorithmic
+bytes per launch are, per sweep it runs, 12 B per active slot (D double +
+nbr int32) + 20 B per node (list offset, F in, F out), counted on the device
+for the timed launches; the average kernel duration comes from the HIP
+events the library records at the start and end of every 16th such launch
+(hipExtLaunchKernelGGL, on its stream) in two further, instrumented solves.
+traffic is read from profiles/traffic.json (rocprofv3 FETCH_SIZE +
+WRITE_SIZE per launch of that kernel, scaled by a copy kernel of known size)
+when it was measured for the same kernel.
 
 cpu_baseline: the oracle (the CPU restatement of the reference schedule,
-oracle/f2m_oracle.cpp) solving the same graph with the host's threads.
+oracle/f2m_oracle.cpp) solving the same graph with the host's threads, and
+the same solve with nodes in input order.
 
 batched: the 4096-instance workload (n log-uniform in [100, 20000], fp32
 points x 1e6, NINT costs, K=16) solved by one replay of the captured
@@ -102,33 +108,48 @@ def main():
     for i in range(args.warmup):
         st = ctx.solve(g)
         log("warmup", i, "sweeps", st["sweeps"], "objective", st["objective"])
-    ctx.set_profile(args.sample_every)
-    prof_tot = {"sweep_launches": 0, "sweep_samples": 0, "sweep_ms_sampled": 0.0, "sweep_alg_bytes": 0.0}
+    # timed solves: no instrumentation (sampling events cost ~10 us per timed launch)
     ctx.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = ctx.solve(g)
-        p = ctx.profile()
-        for k in prof_tot:
-            prof_tot[k] += p[k]
     ctx.synchronize()
     t1 = time.perf_counter()
-    ctx.set_profile(0)
     ms = (t1 - t0) * 1e3 / args.steps
     log("timed", args.steps, "solves:", ms, "ms/solve")
 
-    launch_ms = prof_tot["sweep_ms_sampled"] / max(1, prof_tot["sweep_samples"])
-    bytes_per_launch = prof_tot["sweep_alg_bytes"] / max(1, prof_tot["sweep_launches"])
+    # roofline of the dominant sweep kernel, from separate instrumented solves:
+    # every sample_every-th launch of the full width timed with kernel
+    # start/end events, its algorithmic bytes counted on the device
+    ctx.set_profile(args.sample_every)
+    prof_tot = {"sweep_launches": 0, "sweep_samples": 0, "sweep_ms_sampled": 0.0, "sweeps": 0,
+                "sampled_sweeps": 0, "sampled_alg_bytes": 0.0}
+    for i in range(2):
+        ctx.solve(g)
+        p = ctx.profile()
+        for k in prof_tot:
+            prof_tot[k] += p[k]
+    ctx.set_profile(0)
+    samples = max(1, prof_tot["sweep_samples"])
+    launch_ms = prof_tot["sweep_ms_sampled"] / samples
+    bytes_per_launch = prof_tot["sampled_alg_bytes"] / samples
+    sweeps_per_launch = prof_tot["sampled_sweeps"] / samples
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    fuse = int(os.environ.get("F2M_FUSE", "6"))
+    kernel = f"s_sweep_tile<{fuse},false,4>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("s_sweep_c2_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("kernel") == kernel:
+            traffic = tj.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "s_sweep_pad", "avg_launch_us": round(launch_ms * 1e3, 3),
-                "alg_bytes_per_launch": round(bytes_per_launch, 1)}
+                "kernel": kernel, "avg_launch_us": round(launch_ms * 1e3, 3),
+                "sweeps_per_launch": round(sweeps_per_launch, 2),
+                "alg_bytes_per_launch": round(bytes_per_launch, 1),
+                "launches_per_solve": prof_tot["sweep_launches"] / 2, "sweeps_per_solve": prof_tot["sweeps"] / 2}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -141,9 +162,20 @@ def main():
         cpu_ms = (time.perf_counter() - t) * 1e3
         ok = abs(r["stats"]["objective"] - st["objective"]) <= 1e-9 * abs(st["objective"])
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
-               "sample": f"one full solve of the same n={n} K={K} graph ({r['stats']['sweeps']} sweeps), "
-                         f"objective agrees: {ok}",
+               "sample": f"one full solve of the same n={n} K={K} graph, nodes in Morton order "
+                         f"({r['stats']['sweeps']} sweeps), objective agrees: {ok}",
                "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1)}
+        if input_order is not None:
+            # the same solve with nodes in input order (the reference numbers
+            # nodes arbitrarily): the CPU loses its cache locality too
+            rp0, nb0, c0 = O.graph_build(x0, y0, K, mode, threads=threads)
+            t = time.perf_counter()
+            r0 = O.solve(rp0, nb0, c0, threads=threads)
+            cpu0 = (time.perf_counter() - t) * 1e3
+            cpu["input_order_ms"] = round(cpu0, 1)
+            cpu["input_order_speedup"] = round(cpu0 / input_order["ms"], 1)
+            cpu["input_order_objective_agrees"] = bool(
+                abs(r0["stats"]["objective"] - input_order["objective"]) <= 1e-9 * abs(input_order["objective"]))
 
     batched = None if args.no_batched else bench_batched(ctx, args.batch_order)
     fwd_bwd = None if args.no_fwd_bwd else bench_fwd_bwd()

@@ -70,7 +70,7 @@ class Profile(ctypes.Structure):
     _fields_ = [
         ("sweep_launches", c_i64), ("sweep_samples", c_i64), ("sweep_ms_sampled", c_f64),
         ("sweep_alg_bytes", c_f64), ("active_slots", c_i64), ("solve_ms", c_f64),
-        ("sweeps", c_i64), ("sampled_sweeps", c_i64),
+        ("sweeps", c_i64), ("sampled_sweeps", c_i64), ("sampled_alg_bytes", c_f64),
     ]
 
     def as_dict(self):

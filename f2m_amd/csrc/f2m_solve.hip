@@ -956,7 +956,7 @@ struct Runner {
 
   f2m_status setup() {
     hipLaunchKernelGGL(s_zero_u64, dim3(4), dim3(256), 0, c->stream, maxs, (int64_t)MAXS, INF_BITS);
-    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, acc, (int64_t)1, 0ull);
+    hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, acc, (int64_t)2, 0ull);
     int nb = grid_for(m, BS);
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(s_min_pos, dim3(nb), dim3(BS), 0, c->stream, gcost, m, maxs);
@@ -1077,6 +1077,7 @@ struct Runner {
     if (timed) {
       if (!padded) F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
       ++prof_used;
+      sample_slots(1);
     }
     if (write) {
       w.cur ^= 1;
@@ -1085,6 +1086,11 @@ struct Runner {
       if (c->prof_every > 0) ++pending;
     }
     return F2M_OK;
+  }
+  // acc[1] += active slots of the current lists x sweeps of a timed launch
+  void sample_slots(int S) {
+    hipLaunchKernelGGL(s_acc_slots, dim3(1), dim3(1), 0, c->stream, I(w.efoff) + n, acc + 1,
+                       (unsigned long long)S);
   }
   // adds (active slots of the current lists) x (sweeps run on them) on the device
   f2m_status flush_slots() {
@@ -1100,12 +1106,13 @@ struct Runner {
   f2m_status harvest() {
     f2m_status st = flush_slots();
     if (st) return st;
-    unsigned long long slots = 0;
+    unsigned long long slots = 0, sslots = 0;
     if (c->prof_every > 0) {
       unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
-      F2M_HIP(c, hipMemcpyAsync(hp, acc, 8, hipMemcpyDeviceToHost, c->stream));
+      F2M_HIP(c, hipMemcpyAsync(hp, acc, 16, hipMemcpyDeviceToHost, c->stream));
       F2M_HIP(c, hipStreamSynchronize(c->stream));
       slots = hp[0];
+      sslots = hp[1];
     }
     const int64_t active_slots = (int64_t)slots;
     const double alg_bytes = 12.0 * (double)slots + 20.0 * (double)n * (double)sweeps;
@@ -1118,6 +1125,7 @@ struct Runner {
     c->prof.sweep_launches = launches;
     c->prof.sweeps = sweeps;
     c->prof.sampled_sweeps = sampled_sweeps;
+    c->prof.sampled_alg_bytes = 12.0 * (double)sslots + 20.0 * (double)n * (double)sampled_sweeps;
     c->prof.sweep_samples = (int64_t)prof_used;
     c->prof.sweep_ms_sampled = ms;
     c->prof.sweep_alg_bytes = alg_bytes;
@@ -1215,9 +1223,9 @@ struct Runner {
     }
   }
   // S sweeps in one launch over the tiles; timed like launch_sweep (only
-  // launches of the lists' full width are sampled)
+  // launches of the configured width, the dominant kernel, are sampled)
   f2m_status launch_tile(int S, bool frozen, unsigned long long* mslots) {
-    const bool wide = S == width;
+    const bool wide = S == fuse;
     const bool timed = wide && c->prof_every > 0 && (wide_launches % c->prof_every) == 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
@@ -1243,6 +1251,7 @@ struct Runner {
     if (timed) {
       ++prof_used;
       sampled_sweeps += S;
+      sample_slots(S);
     }
     if (wide) ++wide_launches;
     ++launches;

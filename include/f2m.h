@@ -157,12 +157,12 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
 /* ---- instrumentation (new; the reference prints gettimeofday deltas,
  * graph_cuda_define.hpp:29-34) -------------------------------------------------
  * With sample_every > 0, every sample_every-th sweep launch of f2m_solve
- * that runs the full number of sweeps per launch of its lists (several on
- * spatial tiles: up to F2M_FUSE, default 4, as far as the tiles' halos fit;
- * one with F2M_FUSE=1) is timed by a pair of HIP events that the runtime
+ * of the dominant kind (F2M_FUSE sweeps on spatial tiles, default 6; one
+ * sweep with F2M_FUSE=1) is timed by a pair of HIP events that the runtime
  * records at the kernel's own start and end (hipExtLaunchKernelGGL, context
- * stream); f2m_get_profile reports the sampled kernel time, the sweeps those
- * launches ran, and the algorithmic bytes of all sweeps:
+ * stream); f2m_get_profile reports the sampled kernel time, the sweeps and
+ * algorithmic bytes of those launches, and the algorithmic bytes of all
+ * sweeps:
  * 12 B per active slot (D double + nbr int32) + 20 B per node (list
  * offset, F in, F out). */
 typedef struct f2m_profile {
@@ -174,6 +174,7 @@ typedef struct f2m_profile {
   double solve_ms;          /* wall time of the last f2m_solve (host clock)    */
   int64_t sweeps;           /* sweeps executed (a launch may run several)      */
   int64_t sampled_sweeps;   /* sweeps run by the timed launches                */
+  double sampled_alg_bytes; /* algorithmic bytes of those sweeps               */
 } f2m_profile;
 f2m_status f2m_ctx_set_profile(f2m_ctx* ctx, int32_t sample_every);
 f2m_status f2m_get_profile(f2m_ctx* ctx, f2m_profile* out);
