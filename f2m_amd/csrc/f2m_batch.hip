@@ -318,7 +318,9 @@ struct Sched {
     irr_a = nirr;
     int irr_b = 0;
     LocalRP rpf{I.rp, I.sb};
-    for (int i = 0; i < nirr; ++i) {
+    // with exact recovery on, irregular combs go to the exact stage: their
+    // least squares (and the dual repairs it would start) are not run
+    for (int i = 0; o.exact == 0 && i < nirr; ++i) {
       const int r = I.aux[i];
       const int m = wg_compact<BS>(I.n, [&](int v) { return II.root[v] == r; }, I.list, shi);
       const int go = axb_comb(I.list, m, rpf, I.clen, I.A, I.T, eps, I.nbr, I.D, F, I.xw, I.xg, I.rn, sh);

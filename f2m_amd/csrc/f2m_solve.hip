@@ -1402,7 +1402,10 @@ struct Runner {
     irr_a = nirr;
     irr_b = 0;
     broots_h.clear();
-    if (nirr > 0) {
+    // with exact recovery on (opts.exact >= 1), irregular combs go to the
+    // exact stage: their least squares (and the dual repairs it would start)
+    // are not run
+    if (nirr > 0 && o.exact == 0) {
       int32_t tot = 0;
       F2M_HIP(c, scan_i32_to_i32(c, cnt(), off(), n, &tot));
       hipLaunchKernelGGL(s_root_list, gn, bs, 0, c->stream, n, cnt(), ridx(), roots());

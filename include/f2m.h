@@ -74,8 +74,12 @@ typedef struct f2m_opts {
    * x is then in {0, 1/2, 1} with degree exactly 2. */
   int32_t exact;          /* 1     0 never (the reference's output); 1 when the
                                    reference recovery is not exact (status 1, or
-                                   an irregular tight component, whose x comes
-                                   from float least squares); 2 always           */
+                                   an irregular tight component, whose x would
+                                   come from float least squares); 2 always.
+                                   With 1 or 2 the least squares of irregular
+                                   components and the dual repairs are not run
+                                   (n_irregular_b = n_repairs = 0): the exact
+                                   stage replaces them                           */
   int32_t exact_alpha;    /* 4     eps divisor between phases                     */
   double exact_eps0;      /* 0.02  first-phase eps, units of d_min                */
   double exact_tol;       /* 1e-10 certified relative duality gap                 */

@@ -711,6 +711,9 @@ int orc_solve_warm(int n, const int32_t* rowptr, const int32_t* nbr, const doubl
     for (int r = 0; r < n; ++r) {
       if (members[r].empty()) continue;
       ++na;
+      // with exact recovery on, irregular combs go to the exact stage: their
+      // least squares (and the dual repairs it would start) are not run
+      if (o.exact >= 1) continue;
       // variables: tight edges keyed by (min id, max id), first appearance
       // over nodes ascending, each node's tight edges by neighbour id
       std::vector<std::pair<int64_t, int32_t>> var;  // key -> var id

@@ -146,6 +146,28 @@ def test_full_solve_c2(ctx, orc):
     assert err < 1e-6
 
 
+def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
+    """Morton-numbered C2: the schedule's sweeps run several per launch on
+    spatial tiles (halo levels recomputed in shared memory).  The default
+    width equals the oracle bit for bit, and every width 1..8 gives the same
+    potentials and sweep count."""
+    x0, y0, mode, K = W.c2()
+    perm = ctx.spatial_order(x0, y0)
+    x, y = x0[perm], y0[perm]
+    st, res, rp, nb = _full_compare(ctx, orc, x, y, K, mode, "C2 Morton")
+    p = ctx.profile()
+    assert p["sweeps"] == st["sweeps"] and p["sweep_launches"] < st["sweeps"] / 3, p
+    g = ctx.knn_build(x, y, K, mode)
+    for w in (1, 2, 3, 4, 5, 7, 8):
+        monkeypatch.setenv("F2M_FUSE", str(w))
+        say("solve C2 Morton, F2M_FUSE", w)
+        s2 = ctx.solve(g)
+        p = ctx.profile()
+        assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], w
+        assert np.array_equal(ctx.result()["F"], res["F"]), w
+        assert (p["sweep_launches"] == st["sweeps"]) if w == 1 else (p["sweep_launches"] < st["sweeps"]), (w, p)
+
+
 def test_full_solve_random_raw(ctx, orc):
     for seed in (21, 22):
         x, y = W.uniform_int(20000, 10**7, seed)
