@@ -47,6 +47,8 @@ struct BArgs {
   uint8_t *fw, *fc;
   int32_t* efoff;  // per instance n+1 entries at node_off[i] + i
   ESlot* ef;       // compacted active lists, per instance at its slot base
+                   // (16 B per slot: D doubles, then nbr ints, see Inst)
+  int32_t fcap;    // doubles of dynamic shared memory for the potentials
   const int32_t* order;  // instance ids of this bucket, decreasing size
   int32_t n_order;
   int32_t* queue;
@@ -81,7 +83,8 @@ struct Inst {
   double *xw, *xc, *xg, *rn;
   uint8_t *fw, *fc;
   int32_t* efoff;
-  ESlot* ef;
+  double* efD;      // active slot k of node v at efoff[v] + k: its D ...
+  int32_t* efN;     // ... and its neighbour (split arrays, 12 B per slot)
   XState X;            // exact recovery state (instance-local ids and slots)
   double *xpv, *xdv;   // certificate terms
   int32_t *xactL, *xactR;
@@ -94,45 +97,117 @@ struct LocalRP {
   __device__ __forceinline__ int64_t operator()(int v) const { return rp[v] - sb; }
 };
 
+// Where a pass reads the potentials: FS_HBM from I.F[cur] (global memory),
+// LDS copies otherwise -- FS_ONE: one shared-memory buffer holding F[cur]
+// (after a writing pass it is refreshed from the new global values), FS_TWO:
+// two buffers indexed like I.F (Fs + cur*n holds F[cur]).
+enum { FS_HBM = 0, FS_ONE = 1, FS_TWO = 2 };
+constexpr int PASS_U = 4;  // active slots per node loaded together in the shared-memory pass
+
 // One pass over all nodes: gradient over the compacted active lists from
 // F[cur]; if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns
 // max |g| to every thread (barriers inside).
+//
+// FS_HBM: LPN lanes per node, three dependent global loads per node (list
+// bounds, slots, neighbour potentials).  Shared-memory modes: one thread per
+// node; the list bounds of the thread's next node are loaded one step ahead,
+// its first PASS_U slots together, the neighbour potentials from shared
+// memory, so a step waits on one global load.  The arithmetic (dval, Top3,
+// F + g) is the same in every mode and Top3 keeps the three smallest values
+// whatever the order they come in, so all modes give the same bits.
 template <int BS, int LPN>
-__device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, double* sh) {
+__device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, double* sh, double* Fs, int fmode) {
   const double* Fin = I.F[cur];
   double* Fout = I.F[cur ^ 1];
   double gmax = 0.0;
-  const int total = I.n * LPN;
-  for (int base = 0; base < total; base += BS) {
-    const int gt = base + threadIdx.x;
-    const int v = gt / LPN, lane = gt % LPN;
-    Top3 t;
-    double Fv = 0.0;
-    const bool ok = v < I.n;
-    if (ok) {
-      const int b = I.efoff[v], e = I.efoff[v + 1];
-      Fv = Fin[v];
-      for (int k = b + lane; k < e; k += LPN) {
-        const ESlot sl = I.ef[k];
-        t.push(dval(sl.D, Fv, Fin[sl.nbr], v, sl.nbr));
+  if (fmode == FS_HBM) {
+    const int total = I.n * LPN;
+    for (int base = 0; base < total; base += BS) {
+      const int gt = base + threadIdx.x;
+      const int v = gt / LPN, lane = gt % LPN;
+      Top3 t;
+      double Fv = 0.0;
+      const bool ok = v < I.n;
+      if (ok) {
+        const int b = I.efoff[v], e = I.efoff[v + 1];
+        Fv = Fin[v];
+        for (int k = b + lane; k < e; k += LPN) {
+          const int w = I.efN[k];
+          t.push(dval(I.efD[k], Fv, Fin[w], v, w));
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < LPN; off <<= 1) {
+        const double a2 = __shfl_xor(t.a, off, 64);
+        const double b2 = __shfl_xor(t.b, off, 64);
+        const double c2 = __shfl_xor(t.c, off, 64);
+        t.push(a2);
+        t.push(b2);
+        t.push(c2);
+      }
+      if (ok && lane == 0) {
+        const double g = t.grad();
+        if (write) Fout[v] = (use_frozen && I.frozen[v]) ? Fv : Fv + g;
+        gmax = fmax(gmax, fabs(g));
       }
     }
-#pragma unroll
-    for (int off = 1; off < LPN; off <<= 1) {
-      const double a2 = __shfl_xor(t.a, off, 64);
-      const double b2 = __shfl_xor(t.b, off, 64);
-      const double c2 = __shfl_xor(t.c, off, 64);
-      t.push(a2);
-      t.push(b2);
-      t.push(c2);
+    return block_max_all(gmax, sh);
+  }
+  const double* Fl = Fs + (fmode == FS_TWO ? cur * I.n : 0);
+  double* Fl2 = Fs + (cur ^ 1) * I.n;  // written in FS_TWO only
+  int b = 0, e = 0;
+  if ((int)threadIdx.x < I.n) {
+    b = I.efoff[threadIdx.x];
+    e = I.efoff[threadIdx.x + 1];
+  }
+  for (int base = 0; base < I.n; base += BS) {
+    const int v = base + threadIdx.x;
+    const int vn = v + BS;
+    int bn = 0, en = 0;
+    if (vn < I.n) {
+      bn = I.efoff[vn];
+      en = I.efoff[vn + 1];
     }
-    if (ok && lane == 0) {
+    if (v < I.n) {
+      const int L = e - b;
+      double Dk[PASS_U];
+      int wk[PASS_U];
+#pragma unroll
+      for (int k = 0; k < PASS_U; ++k) {
+        Dk[k] = 0.0;
+        wk[k] = v;  // a slot past the list reads v's own potential, unused
+        if (k < L) {
+          Dk[k] = I.efD[b + k];
+          wk[k] = I.efN[b + k];
+        }
+      }
+      const double Fv = Fl[v];
+      Top3 t;
+#pragma unroll
+      for (int k = 0; k < PASS_U; ++k) {
+        const double d = dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]);
+        t.push(k < L ? d : 1e6);  // the sentinel leaves the three values unchanged
+      }
+      for (int k = b + PASS_U; k < e; ++k) {
+        const int w = I.efN[k];
+        t.push(dval(I.efD[k], Fv, Fl[w], v, w));
+      }
       const double g = t.grad();
-      if (write) Fout[v] = (use_frozen && I.frozen[v]) ? Fv : Fv + g;
+      if (write) {
+        const double f = (use_frozen && I.frozen[v]) ? Fv : Fv + g;
+        Fout[v] = f;
+        if (fmode == FS_TWO) Fl2[v] = f;
+      }
       gmax = fmax(gmax, fabs(g));
     }
+    b = bn;
+    e = en;
   }
-  return block_max_all(gmax, sh);
+  if (write && fmode == FS_ONE) {
+    __syncthreads();  // every read of the old potentials is done
+    for (int v = threadIdx.x; v < I.n; v += BS) Fs[v] = Fout[v];  // this thread's own stores
+  }
+  return block_max_all(gmax, sh);  // its barriers publish the new shared values
 }
 
 // update_EF (graph_cuda.cu:229-252) inside the workgroup: efoff = exclusive
@@ -166,7 +241,10 @@ __device__ void wg_ef_rebuild(const Inst& I, int* shi) {
   for (int v = threadIdx.x; v < I.n; v += BS) {
     const int64_t b = I.b(v);
     const int o = I.efoff[v], L = I.act[v];
-    for (int k = 0; k < L; ++k) I.ef[o + k] = ESlot{I.D[b + k], I.nbr[b + k], 0};
+    for (int k = 0; k < L; ++k) {
+      I.efD[o + k] = I.D[b + k];
+      I.efN[o + k] = I.nbr[b + k];
+    }
   }
   __syncthreads();
 }
@@ -202,13 +280,28 @@ struct Sched {
   const f2m_opts& o;
   double* sh;
   int* shi;
+  double* Fs;             // shared memory for the potentials (FS_ONE / FS_TWO)
+  int fmode;              // FS_HBM, FS_ONE or FS_TWO
   int cur = 0;
   int64_t sweeps = 0;
   double eps;
+  bool fok = false;       // the shared copy holds F[cur]
 
-  __device__ double grad_act() { return wg_pass<BS, LPN>(I, cur, false, false, sh); }
+  // (re)load the shared copy after the potentials were set outside a pass
+  __device__ void fs_load() {
+    if (fmode == FS_HBM || fok) return;
+    double* dst = Fs + (fmode == FS_TWO ? cur * I.n : 0);
+    for (int v = threadIdx.x; v < I.n; v += BS) dst[v] = I.F[cur][v];
+    __syncthreads();
+    fok = true;
+  }
+  __device__ double grad_act() {
+    fs_load();
+    return wg_pass<BS, LPN>(I, cur, false, false, sh, Fs, fmode);
+  }
   __device__ double sweep(bool frozen) {
-    const double m = wg_pass<BS, LPN>(I, cur, true, frozen, sh);
+    fs_load();
+    const double m = wg_pass<BS, LPN>(I, cur, true, frozen, sh, Fs, fmode);
     cur ^= 1;
     ++sweeps;
     return m;
@@ -336,6 +429,7 @@ struct Sched {
   // irr == 2; type 1 shifts only the smallest violating node of each comb.
   __device__ void shift(double sv, int type) {
     double* F = I.F[cur];
+    fok = false;  // F changes below
     for (int v = threadIdx.x; v < I.n; v += BS) {
       I.frozen[v] = 0;
       I.list[v] = INT_MAX;
@@ -564,7 +658,8 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   I.fw = a.fw + I.sb;
   I.fc = a.fc + I.sb;
   I.efoff = a.efoff + I.nb + inst;
-  I.ef = a.ef + I.sb;
+  I.efD = reinterpret_cast<double*>(a.ef + I.sb);  // the instance's 16 B per slot hold
+  I.efN = reinterpret_cast<int32_t*>(I.efD + I.m);  // m doubles, then m ints
   I.X.nbr = I.gnbr;
   I.X.c = I.gcost;
   I.X.mir = a.xmir + I.sb;
@@ -612,7 +707,10 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   }
   __syncthreads();
   z.d_min = dmin;
-  Sched<BS, LPN> S{I, o, sh, shi};
+  // the potentials in shared memory when they fit (both buffers, else one);
+  // the same space holds xdet in the exact stage, after the last pass
+  const int fmode = 2 * I.n <= a.fcap ? FS_TWO : (I.n <= a.fcap ? FS_ONE : FS_HBM);
+  Sched<BS, LPN> S{I, o, sh, shi, xdet, fmode};
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
@@ -712,11 +810,13 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   __syncthreads();
 }
 
+// Dynamic shared memory: a.fcap doubles (at least 2*BS), the potentials of
+// the instance's passes, then the exact stage's xdet.
 template <int BS, int LPN>
 __global__ void __launch_bounds__(BS) b_persistent(BArgs a) {
+  extern __shared__ double xdet[];
   __shared__ double sh[BS / 64];
   __shared__ int shi[BS / 64 + 1];
-  __shared__ double xdet[2 * BS];
   __shared__ int xcnt[8];
   __shared__ double sdmin;
   for (;;) {
@@ -755,8 +855,9 @@ __global__ void b_localize(int64_t m, int n_inst, const int64_t* __restrict__ no
 }  // namespace
 
 // defined in f2m_knn.hip
-f2m_status knn_directed_into(f2m_ctx* c, int64_t n, const double* x, const double* y, int K, int mode,
-                             int32_t* out_id, double* out_c, int32_t id_offset);
+f2m_status knn_directed_batched(f2m_ctx* c, int n_inst, const int64_t* node_off_h, const int64_t* node_off_d,
+                                const double* x, const double* y, int K, int mode, int32_t* out_id,
+                                double* out_c);
 f2m_status symmetrise_i64(f2m_ctx* c, int64_t n, int K, const int32_t* knn, const double* kc, DevBuf& rpb,
                           DevBuf& nbb, DevBuf& cb, int64_t* nslots_out);
 
@@ -764,7 +865,8 @@ namespace {
 
 struct BatchPlan {
   std::vector<int32_t> order;  // all instances, bucketed, decreasing size
-  std::vector<int> bucket_start, bucket_lpn, bucket_bs;
+  std::vector<int> bucket_start, bucket_lpn, bucket_bs, bucket_grid;
+  std::vector<size_t> bucket_dyn;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<unsigned char> key;
@@ -790,29 +892,79 @@ void batch_cfg_for(const f2m_ctx* c, int64_t n, int n_inst, int ncu, int& lpn, i
 }
 
 template <int BS, int LPN>
-hipError_t launch_bucket(const BArgs& ak, int ncu, hipStream_t st) {
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_persistent<BS, LPN>, BS, 0);
+const void* bucket_kernel() {
+  return reinterpret_cast<const void*>(&b_persistent<BS, LPN>);
+}
+
+const void* bucket_kernel_any(int bs, int lpn) {
+  auto pick = [&](auto k1, auto k2, auto k4, auto k8) {
+    return lpn == 1 ? k1 : (lpn == 2 ? k2 : (lpn == 8 ? k8 : k4));
+  };
+  if (bs == 1024)
+    return pick(bucket_kernel<1024, 1>(), bucket_kernel<1024, 2>(), bucket_kernel<1024, 4>(),
+                bucket_kernel<1024, 8>());
+  if (bs == 512)
+    return pick(bucket_kernel<512, 1>(), bucket_kernel<512, 2>(), bucket_kernel<512, 4>(), bucket_kernel<512, 8>());
+  return pick(bucket_kernel<256, 1>(), bucket_kernel<256, 2>(), bucket_kernel<256, 4>(), bucket_kernel<256, 8>());
+}
+
+// Launch shape of a bucket whose largest instance has nmax nodes: dynamic
+// shared memory for the potentials -- two buffers of nmax doubles if they
+// fit beside the workgroups per CU that the registers allow, else one, else
+// what is left (smaller instances of the bucket may still fit) -- at least
+// the exact stage's 2*bs doubles; grid = co-resident workgroups, at most one
+// per instance.  F2M_BATCH_LDS=0 keeps the potentials in global memory, =1
+// allows one shared buffer only (both select paths for tests and timing).
+hipError_t bucket_shape(int device, int bs, int lpn, int64_t nmax, int n_order, int ncu, size_t& dyn, int& grid) {
+  const void* k = bucket_kernel_any(bs, lpn);
+  hipFuncAttributes fa;
+  hipError_t e = hipFuncGetAttributes(&fa, k);
   if (e != hipSuccess) return e;
-  const int grid = std::min(ak.n_order, std::max(1, per_cu) * ncu);
-  hipLaunchKernelGGL((b_persistent<BS, LPN>), dim3(grid), dim3(BS), 0, st, ak);
+  int lds_cu = 0;
+  e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device);
+  if (e != hipSuccess) return e;
+  const size_t base = (size_t)bs * 2 * 8;
+  int per_cu = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, base);
+  if (e != hipSuccess) return e;
+  per_cu = std::max(1, per_cu);
+  size_t room = (size_t)lds_cu / per_cu;
+  room = room > fa.sharedSizeBytes + 64 ? (room - fa.sharedSizeBytes - 64) & ~(size_t)255 : 0;
+  dyn = base;
+  const char* env = std::getenv("F2M_BATCH_LDS");
+  const int lds = env ? std::atoi(env) : 2;
+  if (lds > 0) {
+    const size_t two = (size_t)nmax * 16, one = (size_t)nmax * 8;
+    dyn = std::max(base, (two <= room && lds >= 2) ? two : (one <= room ? one : room));
+  }
+  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  if (e != hipSuccess) (void)hipGetLastError();  // the default limit may already allow it
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, dyn);
+  if (e != hipSuccess) return e;
+  grid = std::min(n_order, std::max(1, per_cu) * ncu);
+  return hipSuccess;
+}
+
+template <int BS, int LPN>
+hipError_t launch_bucket(const BArgs& ak, int grid, size_t dyn, hipStream_t st) {
+  hipLaunchKernelGGL((b_persistent<BS, LPN>), dim3(grid), dim3(BS), dyn, st, ak);
   return hipGetLastError();
 }
 
 template <int BS>
-hipError_t launch_bucket_bs(const BArgs& ak, int lpn, int ncu, hipStream_t st) {
+hipError_t launch_bucket_bs(const BArgs& ak, int lpn, int grid, size_t dyn, hipStream_t st) {
   switch (lpn) {
-    case 1: return launch_bucket<BS, 1>(ak, ncu, st);
-    case 2: return launch_bucket<BS, 2>(ak, ncu, st);
-    case 8: return launch_bucket<BS, 8>(ak, ncu, st);
-    default: return launch_bucket<BS, 4>(ak, ncu, st);
+    case 1: return launch_bucket<BS, 1>(ak, grid, dyn, st);
+    case 2: return launch_bucket<BS, 2>(ak, grid, dyn, st);
+    case 8: return launch_bucket<BS, 8>(ak, grid, dyn, st);
+    default: return launch_bucket<BS, 4>(ak, grid, dyn, st);
   }
 }
 
-hipError_t launch_bucket_any(const BArgs& ak, int bs, int lpn, int ncu, hipStream_t st) {
-  if (bs == 1024) return launch_bucket_bs<1024>(ak, lpn, ncu, st);
-  if (bs == 512) return launch_bucket_bs<512>(ak, lpn, ncu, st);
-  return launch_bucket_bs<256>(ak, lpn, ncu, st);
+hipError_t launch_bucket_any(const BArgs& ak, int bs, int lpn, int grid, size_t dyn, hipStream_t st) {
+  if (bs == 1024) return launch_bucket_bs<1024>(ak, lpn, grid, dyn, st);
+  if (bs == 512) return launch_bucket_bs<512>(ak, lpn, grid, dyn, st);
+  return launch_bucket_bs<256>(ak, lpn, grid, dyn, st);
 }
 
 }  // namespace
@@ -853,16 +1005,13 @@ f2m_status f2m_knn_build_batched(f2m_ctx* c, int32_t n_inst, const int64_t* node
   F2M_HIP(c, c->b_kc.reserve(n_total * k * 8ull));
   int32_t* knn = c->b_knn.as<int32_t>();
   double* kc = c->b_kc.as<double>();
-  for (int i = 0; i < n_inst; ++i) {
-    const int64_t o = node_off[i], ni = node_off[i + 1] - o;
-    f2m_status s = knn_directed_into(c, ni, dx + o, dy + o, k, cost_mode, knn + o * k, kc + o * k, (int32_t)o);
-    if (s) return s;
-  }
-  int64_t m = 0;
-  f2m_status s = symmetrise_i64(c, n_total, k, knn, kc, c->b_rowptr, c->b_nbr, c->b_cost, &m);
-  if (s) return s;
   F2M_HIP(c, c->b_node_off.reserve((n_inst + 1) * 8ull));
   F2M_HIP(c, hipMemcpyAsync(c->b_node_off.p, node_off, (n_inst + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+  f2m_status s = knn_directed_batched(c, n_inst, node_off, c->b_node_off.as<int64_t>(), dx, dy, k, cost_mode, knn, kc);
+  if (s) return s;
+  int64_t m = 0;
+  s = symmetrise_i64(c, n_total, k, knn, kc, c->b_rowptr, c->b_nbr, c->b_cost, &m);
+  if (s) return s;
   hipLaunchKernelGGL(b_localize, dim3(grid_for(m, 256)), dim3(256), 0, c->stream, m, n_inst,
                      c->b_node_off.as<int64_t>(), c->b_rowptr.as<int64_t>(), c->b_nbr.as<int32_t>());
   F2M_LAUNCH_CHECK(c);
@@ -957,6 +1106,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     key.insert(key.end(), q, q + sizeof t);
   }
   if (const char* e = std::getenv("F2M_BATCH_BS")) key.insert(key.end(), e, e + std::strlen(e));
+  if (const char* e = std::getenv("F2M_BATCH_LDS")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
   {  // the warm-start source is a captured kernel argument
     const unsigned char* q = reinterpret_cast<const unsigned char*>(&F_init);
@@ -993,6 +1143,17 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
       }
     }
     P->bucket_start.push_back(n_inst);
+    P->bucket_dyn.clear();
+    P->bucket_grid.clear();
+    for (size_t k = 0; k + 1 < P->bucket_start.size(); ++k) {
+      const int first = P->order[P->bucket_start[k]];  // the bucket's largest instance
+      size_t dyn = 0;
+      int grid = 0;
+      F2M_HIP(c, bucket_shape(c->device, P->bucket_bs[k], P->bucket_lpn[k], off[first + 1] - off[first],
+                              P->bucket_start[k + 1] - P->bucket_start[k], ncu, dyn, grid));
+      P->bucket_dyn.push_back(dyn);
+      P->bucket_grid.push_back(grid);
+    }
     F2M_HIP(c, hipMemcpyAsync(c->b_order.p, P->order.data(), n_inst * 4ull, hipMemcpyHostToDevice, c->stream));
     F2M_HIP(c, hipStreamSynchronize(c->stream));
   }
@@ -1029,6 +1190,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   a.fc = a.fw + M;
   a.efoff = c->b_efoff.as<int32_t>();
   a.ef = c->b_ef.as<ESlot>();
+  a.fcap = 0;  // set per bucket
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;
@@ -1054,7 +1216,9 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
       ak.order = c->b_order.as<int32_t>() + P->bucket_start[k];
       ak.n_order = P->bucket_start[k + 1] - P->bucket_start[k];
       ak.queue = queues + k;
-      F2M_HIP(c, launch_bucket_any(ak, P->bucket_bs[k], P->bucket_lpn[k], ncu, c->stream));
+      ak.fcap = (int32_t)(P->bucket_dyn[k] / 8);
+      F2M_HIP(c, launch_bucket_any(ak, P->bucket_bs[k], P->bucket_lpn[k], P->bucket_grid[k], P->bucket_dyn[k],
+                                   c->stream));
     }
     return F2M_OK;
   };

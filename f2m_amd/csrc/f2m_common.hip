@@ -258,7 +258,7 @@ void f2m_ctx_destroy(f2m_ctx* c) {
   f2m::release_batch_plan(c);
   f2m::DevBuf* all[] = {&c->g_rowptr, &c->g_nbr, &c->g_cost, &c->k_x, &c->k_y, &c->k_cell,
                         &c->k_cstart, &c->k_order, &c->k_sx, &c->k_sy, &c->k_knn, &c->k_kcost,
-                        &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->b_node_off,
+                        &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->k_geom, &c->b_node_off,
                         &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_order, &c->b_stats,
                         &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk, &c->b_ef, &c->b_efoff, &c->b_finit};
   for (auto* b : all) b->release();

@@ -63,7 +63,7 @@ struct f2m_ctx {
   f2m::DevBuf g_rowptr, g_nbr, g_cost;
   // kNN scratch
   f2m::DevBuf k_x, k_y, k_cell, k_cstart, k_order, k_sx, k_sy, k_knn, k_kcost, k_rev, k_cur,
-      k_deg, k_red, k_scan;
+      k_deg, k_red, k_scan, k_geom;
   // staging of host arrays
   f2m::DevBuf stage[6];
   // spatial ordering scratch (f2m_order.hip)

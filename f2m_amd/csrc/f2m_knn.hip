@@ -12,8 +12,10 @@
 //   -> per-point ring search, best-K list in LDS  (k_knn)
 //   -> reverse-edge count -> scan -> fill -> per-node sort by (cost, nbr).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "f2m_internal.h"
 
@@ -109,25 +111,19 @@ __global__ void k_scatter(const double* __restrict__ x, const double* __restrict
   sy[p] = y[i];
 }
 
-// One thread per point, visited in cell order so a wave scans nearby cells.
-// best-K list (cost, id) sorted ascending in LDS, column-major per thread.
+// Ring search of one point over the cells of its grid: the K nearest other
+// points by the key (cost, id), ascending, as (bw, bc) columns of this
+// thread (stride T) in shared memory.  cstart: cell starts of this grid
+// (cells row-major, gx per row); order/sx/sy: points in cell order.  The
+// search stops once every unvisited point lies beyond the K-th cost, so the
+// result does not depend on the cell size.
 template <int T>
-__global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom g,
-                                           const int32_t* __restrict__ cstart,
-                                           const int32_t* __restrict__ order,
-                                           const double* __restrict__ sx, const double* __restrict__ sy,
-                                           const int32_t* __restrict__ cell,
-                                           int32_t* __restrict__ out_id, double* __restrict__ out_c,
-                                           int32_t id_offset) {
-  extern __shared__ unsigned char smem[];
-  double* bc = reinterpret_cast<double*>(smem);
-  int32_t* bw = reinterpret_cast<int32_t*>(bc + (size_t)K * T);
-  int64_t p = (int64_t)blockIdx.x * T + threadIdx.x;
-  if (p >= n) return;
+__device__ __forceinline__ void knn_search(int K, int mode, const GridGeom& g, const int32_t* __restrict__ cstart,
+                                           const int32_t* __restrict__ order, const double* __restrict__ sx,
+                                           const double* __restrict__ sy, int32_t v, double xv, double yv,
+                                           int64_t cell_v, double* bc, int32_t* bw) {
   const int t = threadIdx.x;
-  const int32_t v = order[p];
-  const double xv = sx[p], yv = sy[p];
-  const int64_t cx = cell[v] % g.gx, cy = cell[v] / g.gx;
+  const int64_t cx = cell_v % g.gx, cy = cell_v / g.gx;
   int cnt = 0;
   const int64_t Rmax = g.gx > g.gy ? g.gx : g.gy;
   for (int64_t R = 0;; ++R) {
@@ -169,8 +165,143 @@ __global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom 
       if (bc[(K - 1) * T + t] < lb) break;
     }
   }
+}
+
+// One thread per point, visited in cell order so a wave scans nearby cells.
+// best-K list (cost, id) sorted ascending in LDS, column-major per thread.
+template <int T>
+__global__ void __launch_bounds__(T) k_knn(int64_t n, int K, int mode, GridGeom g,
+                                           const int32_t* __restrict__ cstart,
+                                           const int32_t* __restrict__ order,
+                                           const double* __restrict__ sx, const double* __restrict__ sy,
+                                           const int32_t* __restrict__ cell,
+                                           int32_t* __restrict__ out_id, double* __restrict__ out_c,
+                                           int32_t id_offset) {
+  extern __shared__ unsigned char smem[];
+  double* bc = reinterpret_cast<double*>(smem);
+  int32_t* bw = reinterpret_cast<int32_t*>(bc + (size_t)K * T);
+  int64_t p = (int64_t)blockIdx.x * T + threadIdx.x;
+  if (p >= n) return;
+  const int t = threadIdx.x;
+  const int32_t v = order[p];
+  knn_search<T>(K, mode, g, cstart, order, sx, sy, v, sx[p], sy[p], cell[v], bc, bw);
   for (int j = 0; j < K; ++j) {
     out_id[(int64_t)v * K + j] = bw[j * T + t] + id_offset;
+    out_c[(int64_t)v * K + j] = bc[j * T + t];
+  }
+}
+
+// ---- batched form: every instance of a batch in one pipeline ----------------
+// Points carry global ids (instance i = [node_off[i], node_off[i+1])); each
+// instance has its own grid, whose cells sit at cell_off[i] (at most 4n+16
+// cells per instance, so the offsets are known on the host).
+
+__device__ __forceinline__ int inst_of(int64_t p, const int64_t* __restrict__ node_off, int n_inst) {
+  int lo = 0, hi = n_inst;  // largest i with node_off[i] <= p
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (node_off[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// one workgroup per instance: bbox as ordered bits [min x, min y, max x, max y]
+__global__ void __launch_bounds__(256) k_bbox_b(const double* __restrict__ x, const double* __restrict__ y,
+                                                const int64_t* __restrict__ node_off,
+                                                unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long sm[4][4];
+  const int i = blockIdx.x;
+  unsigned long long m[4] = {0, 0, 0, 0};  // all as "max" keys
+  for (int64_t p = node_off[i] + threadIdx.x; p < node_off[i + 1]; p += 256) {
+    const unsigned long long bx = ordered_bits(x[p]), by = ordered_bits(y[p]);
+    m[0] = max(m[0], ~bx);
+    m[1] = max(m[1], ~by);
+    m[2] = max(m[2], bx);
+    m[3] = max(m[3], by);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (int off = 32; off > 0; off >>= 1)
+      m[k] = max(m[k], (unsigned long long)__shfl_xor((long long)m[k], off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) sm[wid][k] = m[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long r = sm[0][threadIdx.x];
+    for (int w = 1; w < 4; ++w) r = max(r, sm[w][threadIdx.x]);
+    out[4 * (int64_t)i + threadIdx.x] = r;
+  }
+}
+
+// Grid of n points in a w x h box: square cells of about one point each,
+// enlarged until at most 4n+16 cells (also the single-instance rule).
+__host__ __device__ inline GridGeom grid_geom(double mnx, double mny, double mxx, double mxy, int64_t n) {
+  GridGeom g;
+  g.mnx = mnx;
+  g.mny = mny;
+  const double w = mxx - mnx, h = mxy - mny;
+  double cs = sqrt(fmax(w * h, 1e-300) / (double)n);
+  cs = fmax(cs, fmax(w, h) / (double)n);
+  if (!(cs > 0 && cs <= DBL_MAX)) cs = 1.0;  // also catches inf and NaN
+  int64_t gx = (int64_t)(w / cs) + 1, gy = (int64_t)(h / cs) + 1;
+  while (gx * gy > 4 * n + 16) {
+    cs *= 1.5;
+    gx = (int64_t)(w / cs) + 1;
+    gy = (int64_t)(h / cs) + 1;
+  }
+  g.cs = cs;
+  g.gx = gx;
+  g.gy = gy;
+  return g;
+}
+
+__global__ void k_geom_b(int n_inst, const int64_t* __restrict__ node_off,
+                         const unsigned long long* __restrict__ bb, GridGeom* __restrict__ geom) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  const unsigned long long* b = bb + 4 * (int64_t)i;
+  geom[i] = grid_geom(from_ordered_bits(~b[0]), from_ordered_bits(~b[1]), from_ordered_bits(b[2]),
+                      from_ordered_bits(b[3]), node_off[i + 1] - node_off[i]);
+}
+
+__global__ void k_cell_b(int64_t n, int n_inst, const int64_t* __restrict__ node_off,
+                         const int64_t* __restrict__ cell_off, const GridGeom* __restrict__ geom,
+                         const double* __restrict__ x, const double* __restrict__ y, int32_t* __restrict__ cell,
+                         int32_t* __restrict__ cnt) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = inst_of(p, node_off, n_inst);
+  const GridGeom g = geom[i];
+  int64_t cx = (int64_t)((x[p] - g.mnx) / g.cs);
+  int64_t cy = (int64_t)((y[p] - g.mny) / g.cs);
+  cx = cx < 0 ? 0 : (cx >= g.gx ? g.gx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= g.gy ? g.gy - 1 : cy);
+  const int32_t c = (int32_t)(cell_off[i] + cy * g.gx + cx);
+  cell[p] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) k_knn_b(int64_t n, int n_inst, int K, int mode,
+                                             const int64_t* __restrict__ node_off,
+                                             const int64_t* __restrict__ cell_off,
+                                             const GridGeom* __restrict__ geom, const int32_t* __restrict__ cstart,
+                                             const int32_t* __restrict__ order, const double* __restrict__ sx,
+                                             const double* __restrict__ sy, const int32_t* __restrict__ cell,
+                                             int32_t* __restrict__ out_id, double* __restrict__ out_c) {
+  extern __shared__ unsigned char smem[];
+  double* bc = reinterpret_cast<double*>(smem);
+  int32_t* bw = reinterpret_cast<int32_t*>(bc + (size_t)K * T);
+  const int64_t p = (int64_t)blockIdx.x * T + threadIdx.x;
+  if (p >= n) return;
+  const int t = threadIdx.x;
+  const int32_t v = order[p];
+  const int i = inst_of(v, node_off, n_inst);
+  const int64_t co = cell_off[i];
+  knn_search<T>(K, mode, geom[i], cstart + co, order, sx, sy, v, sx[p], sy[p], cell[v] - co, bc, bw);
+  for (int j = 0; j < K; ++j) {
+    out_id[(int64_t)v * K + j] = bw[j * T + t];
     out_c[(int64_t)v * K + j] = bc[j * T + t];
   }
 }
@@ -250,24 +381,9 @@ f2m_status knn_directed_into(f2m_ctx* c, int64_t n, const double* x, const doubl
   unsigned long long hb[4];
   F2M_HIP(c, hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, c->stream));
   F2M_HIP(c, hipStreamSynchronize(c->stream));
-  GridGeom g;
-  g.mnx = from_ordered_bits(~hb[0]);
-  g.mny = from_ordered_bits(~hb[1]);
-  double mxx = from_ordered_bits(hb[2]), mxy = from_ordered_bits(hb[3]);
-  double w = mxx - g.mnx, h = mxy - g.mny;
-  double cs = std::sqrt(std::max(w * h, 1e-300) / (double)n);
-  cs = std::max(cs, std::max(w, h) / (double)n);
-  if (!(cs > 0) || !std::isfinite(cs)) cs = 1.0;
-  int64_t gx = (int64_t)(w / cs) + 1, gy = (int64_t)(h / cs) + 1;
-  while (gx * gy > 4 * n + 16) {
-    cs *= 1.5;
-    gx = (int64_t)(w / cs) + 1;
-    gy = (int64_t)(h / cs) + 1;
-  }
-  g.cs = cs;
-  g.gx = gx;
-  g.gy = gy;
-  int64_t ncell = gx * gy;
+  const GridGeom g = grid_geom(from_ordered_bits(~hb[0]), from_ordered_bits(~hb[1]), from_ordered_bits(hb[2]),
+                               from_ordered_bits(hb[3]), n);
+  int64_t ncell = g.gx * g.gy;
   F2M_HIP(c, c->k_cell.reserve(n * 4));
   F2M_HIP(c, c->k_cstart.reserve((ncell + 1) * 4));
   F2M_HIP(c, c->k_cur.reserve((ncell + 1) * 4 > n * 4 ? (ncell + 1) * 4 : n * 4));
@@ -296,6 +412,56 @@ f2m_status knn_directed_into(f2m_ctx* c, int64_t n, const double* x, const doubl
     hipLaunchKernelGGL(k_knn<64>, dim3(grid_for(n, 64)), dim3(64), lds, c->stream, n, K, mode, g,
                        c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(), c->k_sx.as<double>(),
                        c->k_sy.as<double>(), c->k_cell.as<int32_t>(), out_id, out_c, id_offset);
+  }
+  F2M_LAUNCH_CHECK(c);
+  return F2M_OK;
+}
+
+// Directed kNN of every instance of a batch into (out_id, out_c) [N*K], ids
+// global: one pipeline (bbox, grid geometry, cells, scan, scatter, ring
+// search) over all instances, no host round trip.  node_off_h / node_off_d:
+// the instance offsets on the host and on the device.
+f2m_status knn_directed_batched(f2m_ctx* c, int n_inst, const int64_t* node_off_h, const int64_t* node_off_d,
+                                const double* x, const double* y, int K, int mode, int32_t* out_id,
+                                double* out_c) {
+  const int64_t n = node_off_h[n_inst];
+  std::vector<int64_t> coff(n_inst + 1, 0);
+  for (int i = 0; i < n_inst; ++i) coff[i + 1] = coff[i] + 4 * (node_off_h[i + 1] - node_off_h[i]) + 16;
+  const int64_t ncell = coff[n_inst];
+  if (ncell >= (int64_t)INT32_MAX) return fail(c, F2M_ERR_LIMIT, "batch too large for 32-bit cell offsets");
+  F2M_HIP(c, c->k_red.reserve(n_inst * 32ull + 64));
+  F2M_HIP(c, c->k_geom.reserve(n_inst * (sizeof(GridGeom) + 8ull) + 64));
+  F2M_HIP(c, c->k_cell.reserve(n * 4));
+  F2M_HIP(c, c->k_cstart.reserve((ncell + 1) * 4));
+  F2M_HIP(c, c->k_cur.reserve((ncell + 1) * 4));
+  F2M_HIP(c, c->k_order.reserve(n * 4));
+  F2M_HIP(c, c->k_sx.reserve(n * 8));
+  F2M_HIP(c, c->k_sy.reserve(n * 8));
+  unsigned long long* bb = c->k_red.as<unsigned long long>();
+  GridGeom* geom = c->k_geom.as<GridGeom>();
+  int64_t* cell_off = reinterpret_cast<int64_t*>(geom + n_inst);
+  F2M_HIP(c, hipMemcpyAsync(cell_off, coff.data(), (n_inst + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_bbox_b, dim3(n_inst), dim3(256), 0, c->stream, x, y, node_off_d, bb);
+  hipLaunchKernelGGL(k_geom_b, dim3(grid_for(n_inst, 256)), dim3(256), 0, c->stream, n_inst, node_off_d, bb, geom);
+  int32_t* cnt = c->k_cur.as<int32_t>();
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, cnt, ncell);
+  hipLaunchKernelGGL(k_cell_b, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, n_inst, node_off_d, cell_off,
+                     geom, x, y, c->k_cell.as<int32_t>(), cnt);
+  F2M_LAUNCH_CHECK(c);
+  F2M_HIP(c, scan_i32_to_i32(c, cnt, c->k_cstart.as<int32_t>(), ncell, nullptr));
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, cnt, ncell);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, y, n,
+                     c->k_cell.as<int32_t>(), c->k_cstart.as<int32_t>(), cnt, c->k_order.as<int32_t>(),
+                     c->k_sx.as<double>(), c->k_sy.as<double>());
+  F2M_LAUNCH_CHECK(c);
+  if (K <= 32) {
+    hipLaunchKernelGGL(k_knn_b<128>, dim3(grid_for(n, 128)), dim3(128), (size_t)K * 128 * 12, c->stream, n, n_inst,
+                       K, mode, node_off_d, cell_off, geom, c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(),
+                       c->k_sx.as<double>(), c->k_sy.as<double>(), c->k_cell.as<int32_t>(), out_id, out_c);
+  } else {
+    hipLaunchKernelGGL(k_knn_b<64>, dim3(grid_for(n, 64)), dim3(64), (size_t)K * 64 * 12, c->stream, n, n_inst,
+                       K, mode, node_off_d, cell_off, geom, c->k_cstart.as<int32_t>(), c->k_order.as<int32_t>(),
+                       c->k_sx.as<double>(), c->k_sy.as<double>(), c->k_cell.as<int32_t>(), out_id, out_c);
   }
   F2M_LAUNCH_CHECK(c);
   return F2M_OK;

@@ -4,7 +4,8 @@ Bar: every instance of a batch gives the same graph, potentials, components,
 x and schedule counts as a plain f2m_solve of that instance, bit for bit,
 the exact recovery included (objective within 1e-12 relative: the
 per-instance sums are reduced in a different order); every lanes-per-node
-schedule gives identical bits; graph replays equal the eager result.  Backward: within 1e-4 relative of the FP64
+schedule gives identical bits, with the potentials of the sweeps in global or
+shared memory; graph replays equal the eager result.  Backward: within 1e-4 relative of the FP64
 CPU gradient and of central finite differences (eps = 1e-4) on instances
 checked to be nondegenerate (same kNN graph and same x at +-eps).
 """
@@ -92,10 +93,14 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
     xs, ys = batch_case
     b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
     ref = None
-    for lpn, bs in (("1", "256"), ("2", "256"), ("4", "512"), ("8", "1024"), ("2", "1024")):
+    # potentials in shared memory (two buffers where they fit, one buffer
+    # refreshed from global memory after each sweep) or in global memory
+    for lpn, bs, lds in (("1", "256", "2"), ("2", "256", "0"), ("4", "512", "1"), ("8", "1024", "0"),
+                         ("2", "1024", "2"), ("4", "256", "1")):
         monkeypatch.setenv("F2M_BATCH_LPN", lpn)
         monkeypatch.setenv("F2M_BATCH_BS", bs)
-        say("batch lpn", lpn, "threads", bs)
+        monkeypatch.setenv("F2M_BATCH_LDS", lds)
+        say("batch lpn", lpn, "threads", bs, "shared potentials", lds)
         sts = ctx.solve_batched(b)
         res = ctx.batch_result(b)
         if ref is None:
@@ -105,6 +110,7 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
             assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
     monkeypatch.delenv("F2M_BATCH_LPN")
     monkeypatch.delenv("F2M_BATCH_BS")
+    monkeypatch.delenv("F2M_BATCH_LDS")
     # captured graph: first call captures, later calls replay; eager work between
     outs = []
     for rep in range(3):
