@@ -104,9 +104,27 @@ struct LocalRP {
 enum { FS_HBM = 0, FS_ONE = 1, FS_TWO = 2 };
 constexpr int PASS_U = 4;  // active slots per node loaded together in the shared-memory pass
 
+// What a pass reads and writes, by value: the callers copy it into locals,
+// so the pointers stay in scalar registers (the instance view itself lives
+// in private memory, and a flat pointer to it would be re-read after every
+// global store).
+struct PView {
+  int n;
+  const int32_t* efoff;
+  const double* efD;
+  const int32_t* efN;
+  double* F0;
+  double* F1;
+  const uint8_t* frozen;
+};
+__device__ __forceinline__ PView pview(const Inst& I) {
+  return PView{I.n, I.efoff, I.efD, I.efN, I.F[0], I.F[1], I.frozen};
+}
+
 // One pass over all nodes: gradient over the compacted active lists from
 // F[cur]; if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns
-// max |g| to every thread (barriers inside).
+// max |g| to every thread (barriers inside); 0 when !want_max (shared-memory
+// modes: one barrier instead of the reduction).
 //
 // FS_HBM: LPN lanes per node, three dependent global loads per node (list
 // bounds, slots, neighbour potentials).  Shared-memory modes: one thread per
@@ -116,9 +134,10 @@ constexpr int PASS_U = 4;  // active slots per node loaded together in the share
 // F + g) is the same in every mode and Top3 keeps the three smallest values
 // whatever the order they come in, so all modes give the same bits.
 template <int BS, int LPN>
-__device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, double* sh, double* Fs, int fmode) {
-  const double* Fin = I.F[cur];
-  double* Fout = I.F[cur ^ 1];
+__device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bool use_frozen, double* sh, double* Fs,
+                                          int fmode, bool want_max = true) {
+  const double* Fin = cur ? I.F1 : I.F0;
+  double* Fout = cur ? I.F0 : I.F1;
   double gmax = 0.0;
   if (fmode == FS_HBM) {
     const int total = I.n * LPN;
@@ -155,32 +174,43 @@ __device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, d
   }
   const double* Fl = Fs + (fmode == FS_TWO ? cur * I.n : 0);
   double* Fl2 = Fs + (cur ^ 1) * I.n;  // written in FS_TWO only
-  int b = 0, e = 0;
-  if ((int)threadIdx.x < I.n) {
-    b = I.efoff[threadIdx.x];
-    e = I.efoff[threadIdx.x + 1];
-  }
+  // node u's list bounds and first PASS_U slots (past its list: D 0 and the
+  // node itself, never used); nodes past n have empty lists
+  auto bounds = [&](int u, int& bb, int& ee) {
+    bb = ee = 0;
+    if (u < I.n) {
+      bb = I.efoff[u];
+      ee = I.efoff[u + 1];
+    }
+  };
+  auto slots = [&](int u, int bb, int ee, double* Dk, int* wk) {
+#pragma unroll
+    for (int k = 0; k < PASS_U; ++k) {
+      Dk[k] = 0.0;
+      wk[k] = u;
+      if (k < ee - bb) {
+        Dk[k] = I.efD[bb + k];
+        wk[k] = I.efN[bb + k];
+      }
+    }
+  };
+  // software pipeline over this thread's nodes v, v+BS, ...: bounds two
+  // nodes ahead, slots one node ahead
+  int b, e, b1, e1;
+  bounds(threadIdx.x, b, e);
+  bounds(threadIdx.x + BS, b1, e1);
+  double Dk[PASS_U];
+  int wk[PASS_U];
+  slots(threadIdx.x, b, e, Dk, wk);
   for (int base = 0; base < I.n; base += BS) {
     const int v = base + threadIdx.x;
-    const int vn = v + BS;
-    int bn = 0, en = 0;
-    if (vn < I.n) {
-      bn = I.efoff[vn];
-      en = I.efoff[vn + 1];
-    }
+    int b2, e2;
+    bounds(v + 2 * BS, b2, e2);
+    double Dn[PASS_U];
+    int wn[PASS_U];
+    slots(v + BS, b1, e1, Dn, wn);
     if (v < I.n) {
       const int L = e - b;
-      double Dk[PASS_U];
-      int wk[PASS_U];
-#pragma unroll
-      for (int k = 0; k < PASS_U; ++k) {
-        Dk[k] = 0.0;
-        wk[k] = v;  // a slot past the list reads v's own potential, unused
-        if (k < L) {
-          Dk[k] = I.efD[b + k];
-          wk[k] = I.efN[b + k];
-        }
-      }
       const double Fv = Fl[v];
       Top3 t;
 #pragma unroll
@@ -200,12 +230,23 @@ __device__ double wg_pass(const Inst& I, int cur, bool write, bool use_frozen, d
       }
       gmax = fmax(gmax, fabs(g));
     }
-    b = bn;
-    e = en;
+    b = b1;
+    e = e1;
+    b1 = b2;
+    e1 = e2;
+#pragma unroll
+    for (int k = 0; k < PASS_U; ++k) {
+      Dk[k] = Dn[k];
+      wk[k] = wn[k];
+    }
   }
   if (write && fmode == FS_ONE) {
     __syncthreads();  // every read of the old potentials is done
     for (int v = threadIdx.x; v < I.n; v += BS) Fs[v] = Fout[v];  // this thread's own stores
+  }
+  if (!want_max) {
+    __syncthreads();  // publishes the new shared values
+    return 0.0;
   }
   return block_max_all(gmax, sh);  // its barriers publish the new shared values
 }
@@ -282,6 +323,7 @@ struct Sched {
   int* shi;
   double* Fs;             // shared memory for the potentials (FS_ONE / FS_TWO)
   int fmode;              // FS_HBM, FS_ONE or FS_TWO
+  PView pv;               // the passes' view of I
   int cur = 0;
   int64_t sweeps = 0;
   double eps;
@@ -297,13 +339,24 @@ struct Sched {
   }
   __device__ double grad_act() {
     fs_load();
-    return wg_pass<BS, LPN>(I, cur, false, false, sh, Fs, fmode);
+    return wg_pass<BS, LPN>(pv, cur, false, false, sh, Fs, fmode);
   }
-  __device__ double sweep(bool frozen) {
+  // cnt Jacobi sweeps; returns max |g| of the last one when want_max (else
+  // 0).  The loop runs on local copies of the state.
+  __device__ double sweep_n(int cnt, bool frozen, bool want_max = false) {
     fs_load();
-    const double m = wg_pass<BS, LPN>(I, cur, true, frozen, sh, Fs, fmode);
-    cur ^= 1;
-    ++sweeps;
+    const PView P = pv;
+    double* const fs = Fs;
+    double* const shm = sh;
+    const int fm = fmode;
+    int c = cur;
+    double m = 0.0;
+    for (int i = 0; i < cnt; ++i) {
+      m = wg_pass<BS, LPN>(P, c, true, frozen, shm, fs, fm, want_max && i == cnt - 1);
+      c ^= 1;
+    }
+    cur = c;
+    sweeps += cnt;
     return m;
   }
   __device__ int block_glob() {
@@ -312,7 +365,7 @@ struct Sched {
     wg_ef_rebuild<BS>(I, shi);
     int go = 0;
     for (; go < o.glob_rounds; ++go) {
-      for (int s = 0; s < o.glob_sweeps; ++s) sweep(false);
+      sweep_n(o.glob_sweeps, false);
       const double* F = I.F[cur];
       for (int v = threadIdx.x; v < I.n; v += BS)
         I.act[v] = part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
@@ -335,14 +388,13 @@ struct Sched {
     const double MA = grad_act();
     if (!o.early_exit) {
       const int cnt = (MA < o.err_grad) ? min(N, o.check_every) : N;
-      for (int i = 0; i < cnt; ++i) sweep(frozen);
+      sweep_n(cnt, frozen);
       return;
     }
     int done = 0;
     while (done < N) {
       const int chunk = min(o.check_every, N - done);
-      double MB = 0.0;
-      for (int i = 0; i < chunk; ++i) MB = sweep(frozen);
+      const double MB = sweep_n(chunk, frozen, true);
       done += chunk;
       if (chunk == o.check_every && MB < o.err_grad) break;
     }
@@ -710,7 +762,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   // the potentials in shared memory when they fit (both buffers, else one);
   // the same space holds xdet in the exact stage, after the last pass
   const int fmode = 2 * I.n <= a.fcap ? FS_TWO : (I.n <= a.fcap ? FS_ONE : FS_HBM);
-  Sched<BS, LPN> S{I, o, sh, shi, xdet, fmode};
+  Sched<BS, LPN> S{I, o, sh, shi, xdet, fmode, pview(I)};
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
@@ -732,7 +784,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
     auto attempt = [&](double sv, int type) {
       ++z.n_repairs;
       S.shift(sv, type);
-      for (int k = 0; k < o.frozen_sweeps; ++k) S.sweep(true);
+      S.sweep_n(o.frozen_sweeps, true);
       for (int v = threadIdx.x; v < I.n; v += BS) I.frozen[v] = 0;
       __syncthreads();
       M = S.run_fix();
@@ -812,8 +864,11 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
 
 // Dynamic shared memory: a.fcap doubles (at least 2*BS), the potentials of
 // the instance's passes, then the exact stage's xdet.
+// 256-thread workgroups: at least 3 waves per SIMD (<= 168 VGPRs), three
+// workgroups per CU; the larger ones run one workgroup per CU whatever their
+// register count below 256.
 template <int BS, int LPN>
-__global__ void __launch_bounds__(BS) b_persistent(BArgs a) {
+__global__ void __launch_bounds__(BS, BS == 256 ? 3 : 1) b_persistent(BArgs a) {
   extern __shared__ double xdet[];
   __shared__ double sh[BS / 64];
   __shared__ int shi[BS / 64 + 1];
