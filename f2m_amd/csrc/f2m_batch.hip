@@ -100,7 +100,7 @@ struct LocalRP {
 // Where a pass reads the potentials: FS_HBM from I.F[cur] (global memory),
 // LDS copies otherwise -- FS_ONE: one shared-memory buffer holding F[cur]
 // (after a writing pass it is refreshed from the new global values), FS_TWO:
-// two buffers indexed like I.F (Fs + cur*n holds F[cur]).
+// two buffers indexed like I.F (b_dyn + cur*n holds F[cur]).
 enum { FS_HBM = 0, FS_ONE = 1, FS_TWO = 2 };
 constexpr int PASS_U = 4;  // active slots per node loaded together in the shared-memory pass
 
@@ -116,10 +116,17 @@ struct PView {
   double* F0;
   double* F1;
   const uint8_t* frozen;
+  int lofs;  // lists in shared memory at b_dyn + lofs (efD, efoff, efN), or -1: in global memory
+  int ltot;  // their slots
 };
 __device__ __forceinline__ PView pview(const Inst& I) {
-  return PView{I.n, I.efoff, I.efD, I.efN, I.F[0], I.F[1], I.frozen};
+  return PView{I.n, I.efoff, I.efD, I.efN, I.F[0], I.F[1], I.frozen, -1, 0};
 }
+
+// The workgroup's dynamic shared memory (b_persistent's launch size): the
+// potentials and active lists of the passes, later the exact stage's sums.
+// Addressed from this symbol, so that the compiler emits LDS instructions.
+extern __shared__ double b_dyn[];
 
 // One pass over all nodes: gradient over the compacted active lists from
 // F[cur]; if write, F[cur^1] = F[cur] + g (frozen nodes keep F).  Returns
@@ -133,9 +140,9 @@ __device__ __forceinline__ PView pview(const Inst& I) {
 // memory, so a step waits on one global load.  The arithmetic (dval, Top3,
 // F + g) is the same in every mode and Top3 keeps the three smallest values
 // whatever the order they come in, so all modes give the same bits.
-template <int BS, int LPN>
-__device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bool use_frozen, double* sh, double* Fs,
-                                          int fmode, bool want_max = true) {
+template <int BS, int LPN, bool LL>
+__device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bool use_frozen, double* sh, int fmode,
+                                          bool want_max = true) {
   const double* Fin = cur ? I.F1 : I.F0;
   double* Fout = cur ? I.F0 : I.F1;
   double gmax = 0.0;
@@ -172,15 +179,23 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
     }
     return block_max_all(gmax, sh);
   }
-  const double* Fl = Fs + (fmode == FS_TWO ? cur * I.n : 0);
-  double* Fl2 = Fs + (cur ^ 1) * I.n;  // written in FS_TWO only
+  const double* Fl = b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
+  double* Fl2 = b_dyn + (cur ^ 1) * I.n;  // written in FS_TWO only
+  const int32_t* eO = I.efoff;
+  const double* eD = I.efD;
+  const int32_t* eN = I.efN;
+  if (LL) {  // lists in shared memory
+    eD = b_dyn + I.lofs;
+    eO = reinterpret_cast<const int32_t*>(eD + I.ltot);
+    eN = eO + I.n + 1;
+  }
   // node u's list bounds and first PASS_U slots (past its list: D 0 and the
   // node itself, never used); nodes past n have empty lists
   auto bounds = [&](int u, int& bb, int& ee) {
     bb = ee = 0;
     if (u < I.n) {
-      bb = I.efoff[u];
-      ee = I.efoff[u + 1];
+      bb = eO[u];
+      ee = eO[u + 1];
     }
   };
   auto slots = [&](int u, int bb, int ee, double* Dk, int* wk) {
@@ -189,8 +204,8 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
       Dk[k] = 0.0;
       wk[k] = u;
       if (k < ee - bb) {
-        Dk[k] = I.efD[bb + k];
-        wk[k] = I.efN[bb + k];
+        Dk[k] = eD[bb + k];
+        wk[k] = eN[bb + k];
       }
     }
   };
@@ -219,8 +234,8 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
         t.push(k < L ? d : 1e6);  // the sentinel leaves the three values unchanged
       }
       for (int k = b + PASS_U; k < e; ++k) {
-        const int w = I.efN[k];
-        t.push(dval(I.efD[k], Fv, Fl[w], v, w));
+        const int w = eN[k];
+        t.push(dval(eD[k], Fv, Fl[w], v, w));
       }
       const double g = t.grad();
       if (write) {
@@ -242,7 +257,7 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   }
   if (write && fmode == FS_ONE) {
     __syncthreads();  // every read of the old potentials is done
-    for (int v = threadIdx.x; v < I.n; v += BS) Fs[v] = Fout[v];  // this thread's own stores
+    for (int v = threadIdx.x; v < I.n; v += BS) b_dyn[v] = Fout[v];  // this thread's own stores
   }
   if (!want_max) {
     __syncthreads();  // publishes the new shared values
@@ -252,9 +267,13 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 }
 
 // update_EF (graph_cuda.cu:229-252) inside the workgroup: efoff = exclusive
-// scan of act, then copy each node's active prefix into ef.
+// scan of act (global memory), then each node's active prefix copied into the
+// split lists -- in shared memory at b_dyn + lofs (efD doubles, an efoff
+// copy, efN ints) when they fit in room bytes (lofs >= 0), else in global
+// memory.  Returns in (lofs, ltot) the shared offset (-1: global) and the
+// slot count.
 template <int BS>
-__device__ void wg_ef_rebuild(const Inst& I, int* shi) {
+__device__ void wg_ef_rebuild(const Inst& I, int* shi, int lofs, int room, int& lofs_out, int& ltot) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int carry = 0;
   for (int base = 0; base < I.n; base += BS) {
@@ -279,15 +298,34 @@ __device__ void wg_ef_rebuild(const Inst& I, int* shi) {
   }
   if (threadIdx.x == 0) I.efoff[I.n] = carry;
   __syncthreads();
-  for (int v = threadIdx.x; v < I.n; v += BS) {
-    const int64_t b = I.b(v);
-    const int o = I.efoff[v], L = I.act[v];
-    for (int k = 0; k < L; ++k) {
-      I.efD[o + k] = I.D[b + k];
-      I.efN[o + k] = I.nbr[b + k];
+  // carry (the total) is the same in every thread
+  const bool ll = lofs >= 0 && (int64_t)carry * 12 + 4 * ((int64_t)I.n + 1) <= room;
+  if (ll) {
+    double* dD = b_dyn + lofs;
+    int32_t* dO = reinterpret_cast<int32_t*>(dD + carry);
+    int32_t* dN = dO + I.n + 1;
+    for (int v = threadIdx.x; v <= I.n; v += BS) dO[v] = I.efoff[v];
+    for (int v = threadIdx.x; v < I.n; v += BS) {
+      const int64_t b = I.b(v);
+      const int o = I.efoff[v], L = I.act[v];
+      for (int k = 0; k < L; ++k) {
+        dD[o + k] = I.D[b + k];
+        dN[o + k] = I.nbr[b + k];
+      }
+    }
+  } else {
+    for (int v = threadIdx.x; v < I.n; v += BS) {
+      const int64_t b = I.b(v);
+      const int o = I.efoff[v], L = I.act[v];
+      for (int k = 0; k < L; ++k) {
+        I.efD[o + k] = I.D[b + k];
+        I.efN[o + k] = I.nbr[b + k];
+      }
     }
   }
   __syncthreads();
+  lofs_out = ll ? lofs : -1;
+  ltot = carry;
 }
 
 // Ascending ids v in [0,n) with pred(v) into out; returns the count.
@@ -321,9 +359,9 @@ struct Sched {
   const f2m_opts& o;
   double* sh;
   int* shi;
-  double* Fs;             // shared memory for the potentials (FS_ONE / FS_TWO)
   int fmode;              // FS_HBM, FS_ONE or FS_TWO
-  PView pv;               // the passes' view of I
+  PView pv;               // the passes' view of I (lists in shared or global memory)
+  int dynb;               // bytes of shared memory at b_dyn
   int cur = 0;
   int64_t sweeps = 0;
   double eps;
@@ -332,28 +370,50 @@ struct Sched {
   // (re)load the shared copy after the potentials were set outside a pass
   __device__ void fs_load() {
     if (fmode == FS_HBM || fok) return;
-    double* dst = Fs + (fmode == FS_TWO ? cur * I.n : 0);
+    double* dst = b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
     for (int v = threadIdx.x; v < I.n; v += BS) dst[v] = I.F[cur][v];
     __syncthreads();
     fok = true;
   }
+  // the current potentials for the partitions: the shared copy when there is one
+  __device__ const double* fcur() {
+    fs_load();
+    return fmode == FS_HBM ? I.F[cur] : b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
+  }
+  // active lists rebuilt after a partition; into the shared memory after the
+  // potentials when they fit
+  __device__ void rebuild() {
+    int lofs = -1, room = 0;
+    if (fmode != FS_HBM) {
+      lofs = (fmode == FS_TWO ? 2 : 1) * I.n;  // doubles of potentials before the lists
+      room = dynb - lofs * 8;
+    }
+    wg_ef_rebuild<BS>(I, shi, lofs, room, pv.lofs, pv.ltot);
+  }
   __device__ double grad_act() {
     fs_load();
-    return wg_pass<BS, LPN>(pv, cur, false, false, sh, Fs, fmode);
+    if (pv.lofs >= 0) return wg_pass<BS, LPN, true>(pv, cur, false, false, sh, fmode);
+    return wg_pass<BS, LPN, false>(pv, cur, false, false, sh, fmode);
   }
   // cnt Jacobi sweeps; returns max |g| of the last one when want_max (else
   // 0).  The loop runs on local copies of the state.
   __device__ double sweep_n(int cnt, bool frozen, bool want_max = false) {
     fs_load();
     const PView P = pv;
-    double* const fs = Fs;
     double* const shm = sh;
     const int fm = fmode;
     int c = cur;
     double m = 0.0;
-    for (int i = 0; i < cnt; ++i) {
-      m = wg_pass<BS, LPN>(P, c, true, frozen, shm, fs, fm, want_max && i == cnt - 1);
-      c ^= 1;
+    if (P.lofs >= 0) {
+      for (int i = 0; i < cnt; ++i) {
+        m = wg_pass<BS, LPN, true>(P, c, true, frozen, shm, fm, want_max && i == cnt - 1);
+        c ^= 1;
+      }
+    } else {
+      for (int i = 0; i < cnt; ++i) {
+        m = wg_pass<BS, LPN, false>(P, c, true, frozen, shm, fm, want_max && i == cnt - 1);
+        c ^= 1;
+      }
     }
     cur = c;
     sweeps += cnt;
@@ -362,26 +422,26 @@ struct Sched {
   __device__ int block_glob() {
     for (int v = threadIdx.x; v < I.n; v += BS) I.act[v] = I.deg[v] > o.glob_k ? o.glob_k : I.deg[v];
     __syncthreads();
-    wg_ef_rebuild<BS>(I, shi);
+    rebuild();
     int go = 0;
     for (; go < o.glob_rounds; ++go) {
       sweep_n(o.glob_sweeps, false);
-      const double* F = I.F[cur];
+      const double* F = fcur();
       for (int v = threadIdx.x; v < I.n; v += BS)
         I.act[v] = part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
       __syncthreads();
-      wg_ef_rebuild<BS>(I, shi);
+      rebuild();
       const double M = grad_act();
       if (M < o.glob_exit) { ++go; break; }
     }
     return go;
   }
   __device__ double check_opt() {
-    const double* F = I.F[cur];
+    const double* F = fcur();
     for (int v = threadIdx.x; v < I.n; v += BS)
       I.act[v] = part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
     __syncthreads();
-    wg_ef_rebuild<BS>(I, shi);
+    rebuild();
     return grad_act();
   }
   __device__ void block_ef(int N, bool frozen) {
@@ -762,7 +822,7 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   // the potentials in shared memory when they fit (both buffers, else one);
   // the same space holds xdet in the exact stage, after the last pass
   const int fmode = 2 * I.n <= a.fcap ? FS_TWO : (I.n <= a.fcap ? FS_ONE : FS_HBM);
-  Sched<BS, LPN> S{I, o, sh, shi, xdet, fmode, pview(I)};
+  Sched<BS, LPN> S{I, o, sh, shi, fmode, pview(I), a.fcap * 8};
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
@@ -862,14 +922,15 @@ __device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, d
   __syncthreads();
 }
 
-// Dynamic shared memory: a.fcap doubles (at least 2*BS), the potentials of
-// the instance's passes, then the exact stage's xdet.
+// Dynamic shared memory: a.fcap doubles (at least 2*BS) holding the
+// potentials of the instance's passes and, where they fit, its active lists;
+// after the last pass, the exact stage's xdet.
 // 256-thread workgroups: at least 3 waves per SIMD (<= 168 VGPRs), three
 // workgroups per CU; the larger ones run one workgroup per CU whatever their
 // register count below 256.
 template <int BS, int LPN>
 __global__ void __launch_bounds__(BS, BS == 256 ? 3 : 1) b_persistent(BArgs a) {
-  extern __shared__ double xdet[];
+  double* const xdet = b_dyn;
   __shared__ double sh[BS / 64];
   __shared__ int shi[BS / 64 + 1];
   __shared__ int xcnt[8];
@@ -968,8 +1029,10 @@ const void* bucket_kernel_any(int bs, int lpn) {
 // fit beside the workgroups per CU that the registers allow, else one, else
 // what is left (smaller instances of the bucket may still fit) -- at least
 // the exact stage's 2*bs doubles; grid = co-resident workgroups, at most one
-// per instance.  F2M_BATCH_LDS=0 keeps the potentials in global memory, =1
-// allows one shared buffer only (both select paths for tests and timing).
+// per instance.  By default the workgroup gets all the shared memory its
+// occupancy allows, so that the active lists can follow the potentials.
+// F2M_BATCH_LDS=0 keeps the potentials in global memory, =1 allows one shared
+// buffer only, =2 the potentials only (paths for tests and timing).
 hipError_t bucket_shape(int device, int bs, int lpn, int64_t nmax, int n_order, int ncu, size_t& dyn, int& grid) {
   const void* k = bucket_kernel_any(bs, lpn);
   hipFuncAttributes fa;
@@ -987,10 +1050,11 @@ hipError_t bucket_shape(int device, int bs, int lpn, int64_t nmax, int n_order, 
   room = room > fa.sharedSizeBytes + 64 ? (room - fa.sharedSizeBytes - 64) & ~(size_t)255 : 0;
   dyn = base;
   const char* env = std::getenv("F2M_BATCH_LDS");
-  const int lds = env ? std::atoi(env) : 2;
+  const int lds = env ? std::atoi(env) : 3;
   if (lds > 0) {
     const size_t two = (size_t)nmax * 16, one = (size_t)nmax * 8;
-    dyn = std::max(base, (two <= room && lds >= 2) ? two : (one <= room ? one : room));
+    if (lds >= 3) dyn = std::max(base, room);  // potentials, then the lists where they fit
+    else dyn = std::max(base, (two <= room && lds >= 2) ? two : (one <= room ? one : room));
   }
   e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   if (e != hipSuccess) (void)hipGetLastError();  // the default limit may already allow it

@@ -94,9 +94,10 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
     b = ctx.knn_build_batched(xs, ys, 16, W.COST_NINT)
     ref = None
     # potentials in shared memory (two buffers where they fit, one buffer
-    # refreshed from global memory after each sweep) or in global memory
+    # refreshed from global memory after each sweep) or in global memory;
+    # active lists in shared memory where they fit after the potentials (3)
     for lpn, bs, lds in (("1", "256", "2"), ("2", "256", "0"), ("4", "512", "1"), ("8", "1024", "0"),
-                         ("2", "1024", "2"), ("4", "256", "1")):
+                         ("2", "1024", "3"), ("4", "256", "1"), ("4", "256", "3"), ("4", "1024", "2")):
         monkeypatch.setenv("F2M_BATCH_LPN", lpn)
         monkeypatch.setenv("F2M_BATCH_BS", bs)
         monkeypatch.setenv("F2M_BATCH_LDS", lds)
