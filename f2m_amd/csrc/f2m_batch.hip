@@ -375,10 +375,20 @@ struct Sched {
     __syncthreads();
     fok = true;
   }
-  // the current potentials for the partitions: the shared copy when there is one
-  __device__ const double* fcur() {
+  // act[v] = part(v, F) for every node, F = the current potentials: the
+  // shared copy when there is one (two call sites, so that each reads its
+  // memory with its own instructions)
+  template <class P>
+  __device__ void partition(P part) {
     fs_load();
-    return fmode == FS_HBM ? I.F[cur] : b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
+    if (fmode == FS_HBM) {
+      const double* F = I.F[cur];
+      for (int v = threadIdx.x; v < I.n; v += BS) I.act[v] = part(v, F);
+    } else {
+      const double* F = b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
+      for (int v = threadIdx.x; v < I.n; v += BS) I.act[v] = part(v, F);
+    }
+    __syncthreads();
   }
   // active lists rebuilt after a partition; into the shared memory after the
   // potentials when they fit
@@ -426,10 +436,9 @@ struct Sched {
     int go = 0;
     for (; go < o.glob_rounds; ++go) {
       sweep_n(o.glob_sweeps, false);
-      const double* F = fcur();
-      for (int v = threadIdx.x; v < I.n; v += BS)
-        I.act[v] = part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
-      __syncthreads();
+      partition([&](int v, const double* F) {
+        return part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
+      });
       rebuild();
       const double M = grad_act();
       if (M < o.glob_exit) { ++go; break; }
@@ -437,10 +446,9 @@ struct Sched {
     return go;
   }
   __device__ double check_opt() {
-    const double* F = fcur();
-    for (int v = threadIdx.x; v < I.n; v += BS)
-      I.act[v] = part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
-    __syncthreads();
+    partition([&](int v, const double* F) {
+      return part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
+    });
     rebuild();
     return grad_act();
   }
