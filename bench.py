@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-fwd-bwd", action="store_true")
     ap.add_argument("--no-input-order", action="store_true")
     ap.add_argument("--batch-order", choices=["morton", "input"], default="morton")
+    ap.add_argument("--only", choices=["batched", "fwd_bwd"], default=None,
+                    help="run only this part and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -91,6 +93,15 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy", args.calibrate_copy_mib, "MiB")
         del a, b
+
+    if args.only:
+        if args.only == "batched":
+            c = Context(0)
+            print(json.dumps({"batched": bench_batched(c, args.batch_order)}), flush=True)
+            c.close()
+        else:
+            print(json.dumps({"fwd_bwd": bench_fwd_bwd()}), flush=True)
+        return
 
     x0, y0, mode, K = W.c2()
     n = len(x0)
