@@ -296,17 +296,21 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
 // ---- spatial tiles: several Jacobi sweeps per launch -------------------------
 // A sweep launch costs ~4.5 us at n = 100k however little it moves (two
 // dependent HBM levels plus the wave ramp), so the schedule's sweeps are run S
-// at a time.  Tile t is the id range [t*TS, t*TS + TS): with Morton-numbered
-// nodes a compact region.  Its local set is the tile plus a halo: nodes
-// reachable from the tile over active slots in at most depth edges, numbered
-// breadth first (local index = id - t*TS inside the tile).  A launch of S
+// at a time.  Tile t is the id range [t*ts, t*ts + ts) (ts = TS, or TS_GLOB
+// for the longer block_GLOB lists): with Morton-numbered nodes a compact
+// region.  Its local set is the tile plus a halo: nodes reachable from the
+// tile over active slots in at most depth edges, numbered breadth first
+// (local index = id - t*ts inside the tile).  A launch of S
 // sweeps (S <= depth + 1) gathers the potentials of the local set's active
 // neighbours from HBM once, computes sweep 1 for every local node of depth
 // <= S-1, sweep 2 from shared memory for depth <= S-2, ..., sweep S for the
 // tile, and writes only the tile's final potentials.  The halo is recomputed
 // by every tile that needs it: the arithmetic per node and sweep is the one
 // of s_sweep_pad, so the potentials are bit-identical to S single sweeps.
-constexpr int TS = 256;      // nodes per tile
+constexpr int TS = 256;      // nodes per tile (lists of the block_EF epochs)
+constexpr int TS_GLOB = 256; // nodes per tile while every list holds >= 5 slots.  128 keeps deeper
+                             // halos within TL on C2 (5 sweeps per launch instead of 3) but the
+                             // doubled halo work made the solve slower (42.7 against 38.7 ms)
 constexpr int TW = 384;      // threads per tile workgroup (tile + typical halo)
 constexpr int TL = 2 * TW;   // local-node capacity (tile + halo)
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
@@ -352,7 +356,7 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // tpp of local node j at tslot[(t*TL + j)*tpp + k], the others at
 // tovf[t*TOV + offset + k - tpp].  *fit_min = min over tiles of fit: a launch
 // may run up to fit_min + 1 sweeps.
-__global__ void __launch_bounds__(TW) s_tile_build(int n, int pp, int tpp, int depth, const int32_t* __restrict__ act,
+__global__ void __launch_bounds__(TW) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
                                                    const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
@@ -362,7 +366,7 @@ __global__ void __launch_bounds__(TW) s_tile_build(int n, int pp, int tpp, int d
   __shared__ int lg[TL];
   __shared__ int s_nl, s_full, s_ov;
   const int t = blockIdx.x, tid = threadIdx.x;
-  const int t0 = t * TS, c0 = min(TS, n - t0);
+  const int t0 = t * ts, c0 = min(ts, n - t0);
   auto slot_at = [&](int v, int k) -> const ESlot& {
     return psl[k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + povo[v] + (k - pp)];
   };
@@ -897,7 +901,8 @@ struct Runner {
   // sweeps per launch (1: one sweep per launch, s_sweep_pad); the tiles are
   // rebuilt with the lists, and usable once their overflow flag reads clear
   int fuse = 1;
-  int ntiles = 0;
+  int ntiles = 0;                    // tiles of the current lists
+  int ts_ef = TS, ts_glob = TS_GLOB;  // nodes per tile (F2M_TILE, F2M_TILE_GLOB)
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
   int tpp = 4;    // padded slots per local node on the current lists
@@ -985,11 +990,13 @@ struct Runner {
     pp = strides[pad_cfg];
     fuse = padded ? std::max(1, std::min(TMAXS, env_int("F2M_FUSE", DEFAULT_FUSE))) : 1;
     if (fuse > 1) {
-      ntiles = (n + TS - 1) / TS;
-      F2M_HIP(c, w.thdr.reserve(((int64_t)ntiles * THDR + 16) * 4));
-      F2M_HIP(c, w.tnode.reserve((int64_t)ntiles * TL * 16));
-      F2M_HIP(c, w.tslot.reserve((int64_t)ntiles * TL * TPP_MAX * 16));
-      F2M_HIP(c, w.tovf.reserve((int64_t)ntiles * TOV * 16));
+      ts_ef = std::max(32, std::min(TS, env_int("F2M_TILE", TS)));
+      ts_glob = std::max(32, std::min(TS, env_int("F2M_TILE_GLOB", TS_GLOB)));
+      const int64_t mt = (n + std::min(ts_ef, ts_glob) - 1) / std::min(ts_ef, ts_glob);  // most tiles
+      F2M_HIP(c, w.thdr.reserve((mt * THDR + 16) * 4));
+      F2M_HIP(c, w.tnode.reserve(mt * TL * 16));
+      F2M_HIP(c, w.tslot.reserve(mt * TL * TPP_MAX * 16));
+      F2M_HIP(c, w.tovf.reserve(mt * TOV * 16));
     }
     return F2M_OK;
   }
@@ -1147,8 +1154,10 @@ struct Runner {
                          w.psl.as<ESlot>());
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
+        const int ts = long_lists ? ts_glob : ts_ef;
+        ntiles = (n + ts - 1) / ts;
         hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)(fuse - 1));
-        hipLaunchKernelGGL(s_tile_build, dim3(ntiles), dim3(TW), 0, c->stream, n, pp, tpp, fuse - 1,
+        hipLaunchKernelGGL(s_tile_build, dim3(ntiles), dim3(TW), 0, c->stream, n, ts, pp, tpp, fuse - 1,
                            (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
                            w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
                            tile_fit());

@@ -150,7 +150,7 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     """Morton-numbered C2: the schedule's sweeps run several per launch on
     spatial tiles (halo levels recomputed in shared memory).  The default
     width equals the oracle bit for bit, and every width 1..8 gives the same
-    potentials and sweep count."""
+    potentials and sweep count; so do other tile sizes."""
     x0, y0, mode, K = W.c2()
     perm = ctx.spatial_order(x0, y0)
     x, y = x0[perm], y0[perm]
@@ -166,6 +166,14 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], w
         assert np.array_equal(ctx.result()["F"], res["F"]), w
         assert (p["sweep_launches"] == st["sweeps"]) if w == 1 else (p["sweep_launches"] < st["sweeps"]), (w, p)
+    monkeypatch.delenv("F2M_FUSE")
+    for tg, te in (("256", "256"), ("64", "128")):  # nodes per tile: block_GLOB lists, block_EF lists
+        monkeypatch.setenv("F2M_TILE_GLOB", tg)
+        monkeypatch.setenv("F2M_TILE", te)
+        say("solve C2 Morton, tiles", tg, te)
+        s2 = ctx.solve(g)
+        assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], (tg, te)
+        assert np.array_equal(ctx.result()["F"], res["F"]), (tg, te)
 
 
 def test_full_solve_random_raw(ctx, orc):
