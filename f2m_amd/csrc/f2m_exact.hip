@@ -149,6 +149,70 @@ __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
   }
 }
 
+// Rounds r0, r0+1, ... of a phase inside one workgroup, used once the active
+// lists have shrunk (a round of x_lstep + x_rstep is then mostly launch
+// overhead): the same steps, barriers for the launch boundaries, the round's
+// counters in shared memory, the lists in the same global arrays.  Stops
+// after the first round without an active node, after max_rounds rounds with
+// one, or when the next L list reaches `small` nodes (back to the grid
+// steps).  On exit the ring holds the counts of the last round and of the
+// next one (as the grid steps leave it), X.rounds is advanced and *ran =
+// rounds run.  The steps' outcome does not depend on the order in which a
+// step's nodes are processed, so the bits equal the grid steps'.
+constexpr int XWG = 1024;
+__global__ void __launch_bounds__(XWG) x_rounds_wg(XDev X, int64_t r0, double eps, int64_t max_rounds, int small,
+                                                   int32_t* ran) {
+  __shared__ int sc[3];  // nL(r), nR(r), nL(r+1)
+  const int lane = threadIdx.x % XG, gid = threadIdx.x / XG;
+  constexpr int NG = XWG / XG;
+  if (threadIdx.x == 0) {
+    sc[0] = X.cnt[2 * (int)(r0 % RING)];
+    sc[1] = X.cnt[2 * (int)(r0 % RING) + 1];
+    sc[2] = 0;
+  }
+  __syncthreads();
+  int64_t r = r0, counted = 0;
+  for (;;) {
+    const int nl = sc[0];
+    const int32_t* list = (r & 1) ? X.actL1 : X.actL0;
+    for (int i = gid; i < nl; i += NG) {
+      const int v = list[i];
+      x_discharge_l(X.S, v, X.rp[v], X.rp[v + 1], lane, eps, [&](int w) { X.actR[atomicAdd(&sc[1], 1)] = w; });
+    }
+    __syncthreads();
+    const int nr = sc[1];
+    int32_t* next = (r & 1) ? X.actL0 : X.actL1;
+    for (int i = gid; i < nr; i += NG) {
+      const int w = X.actR[i];
+      x_discharge_r(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) { next[atomicAdd(&sc[2], 1)] = v; });
+    }
+    __syncthreads();
+    const int nl1 = sc[2];
+    const bool active = nl + nr > 0;
+    if (active) ++counted;
+    ++r;
+    if (!active || counted >= max_rounds || nl1 >= small) {
+      if (threadIdx.x == 0) {
+        const int kl = (int)((r - 1) % RING), kn = (int)(r % RING);
+        X.cnt[2 * kl] = nl;
+        X.cnt[2 * kl + 1] = nr;
+        X.cnt[2 * kn] = nl1;
+        X.cnt[2 * kn + 1] = 0;
+        atomicAdd(X.rounds, (unsigned long long)counted);
+        *ran = (int32_t)(r - r0);
+      }
+      break;
+    }
+    __syncthreads();  // every thread has read the counters
+    if (threadIdx.x == 0) {
+      sc[0] = nl1;
+      sc[1] = 0;
+      sc[2] = 0;
+    }
+    __syncthreads();
+  }
+}
+
 // per-node terms of the duality gap certificate
 __global__ void x_cert_terms(XDev X, double* __restrict__ pv, double* __restrict__ dv) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,7 +342,11 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     int32_t cnt[8];
     int32_t flag[8];
     unsigned long long rounds;
+    int32_t ran;  // rounds run by the last x_rounds_wg
   };
+  int32_t* ran = reinterpret_cast<int32_t*>(X.rounds + 1);
+  // lists below this many nodes run their rounds inside one workgroup
+  const int small = env_int("F2M_EXACT_WG", 512);
   Ring* hr = static_cast<Ring*>(c->pinned);
   auto cert = [&]() -> f2m_status {
     hipLaunchKernelGGL(x_cert_terms, gn, bs, 0, st, X, pv, dv);
@@ -299,9 +367,25 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     else hipLaunchKernelGGL((x_excess<false>), gn, bs, 0, st, X, r);
     F2M_LAUNCH_CHECK(c);
     int chunk = 2;
+    bool in_wg = false;
     for (;;) {
       const int64_t left = o.exact_max_rounds - done;
       if (left <= 0) { stuck = true; break; }
+      if (in_wg) {
+        // at most 65536 rounds per launch, so that no launch runs for seconds
+        hipLaunchKernelGGL(x_rounds_wg, dim3(1), dim3(XWG), 0, st, X, r, eps, std::min<int64_t>(left, 65536), small,
+                           ran);
+        F2M_LAUNCH_CHECK(c);
+        F2M_HIP(c, hipMemcpyAsync(hr, X.cnt, sizeof(Ring), hipMemcpyDeviceToHost, st));
+        F2M_HIP(c, hipStreamSynchronize(st));
+        r += hr->ran;
+        done = (int64_t)hr->rounds;
+        const int kl = (int)((r - 1) % RING);
+        if (hr->cnt[2 * kl] + hr->cnt[2 * kl + 1] == 0) break;
+        in_wg = hr->cnt[2 * (int)(r % RING)] < small;  // else the lists grew: grid steps
+        chunk = 2;
+        continue;
+      }
       const int64_t kk = std::min<int64_t>(chunk, left);
       for (int64_t i = 0; i < kk; ++i, ++r) {
         hipLaunchKernelGGL(x_lstep, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
@@ -314,6 +398,7 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
       const int kl = (int)((r - 1) % RING);
       if (hr->cnt[2 * kl] + hr->cnt[2 * kl + 1] == 0) break;
       chunk = std::min(chunk * 2, 64);
+      in_wg = small > 0 && hr->cnt[2 * (int)(r % RING)] < small;
     }
     ++phases;
     if (stuck) break;

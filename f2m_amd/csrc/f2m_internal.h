@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <utility>
 #include <vector>
@@ -125,6 +126,12 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
                        int32_t* root, f2m_stats& z, int& rc);
 
 inline int grid_for(int64_t threads, int block) { return (int)((threads + block - 1) / block); }
+
+// Integer tuning switch from the environment (dflt when unset).
+inline int env_int(const char* name, int dflt) {
+  const char* s = std::getenv(name);
+  return s ? std::atoi(s) : dflt;
+}
 
 }  // namespace f2m
 

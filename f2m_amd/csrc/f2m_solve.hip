@@ -873,11 +873,6 @@ __global__ void s_maxrank(int n, const int32_t* __restrict__ rp, const int32_t* 
 // ===========================================================================
 namespace {
 
-int env_int(const char* name, int dflt) {
-  const char* s = std::getenv(name);
-  return s ? std::atoi(s) : dflt;
-}
-
 // device max slots: [0, 512) gradient max (32 slots x 16), [512, 1024)
 // early-exit max of the last sweep of a chunk
 constexpr int MAXS = 1024;
@@ -1145,7 +1140,9 @@ struct Runner {
   f2m_status ef_rebuild(bool long_lists = false) {
     f2m_status st0 = flush_slots();
     if (st0) return st0;
-    F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
+    // offsets of the compacted lists: the padded layout needs them only for
+    // the instrumentation's slot counts (efoff[n])
+    if (!padded || c->prof_every > 0) F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
       hipLaunchKernelGGL(s_excess, gn, bs, 0, c->stream, n, I(w.act), pp, I(w.ptmp));
