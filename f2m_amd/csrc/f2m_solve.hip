@@ -311,17 +311,28 @@ constexpr int TS = 256;      // nodes per tile (lists of the block_EF epochs)
 constexpr int TS_GLOB = 256; // nodes per tile while every list holds >= 5 slots.  128 keeps deeper
                              // halos within TL on C2 (5 sweeps per launch instead of 3) but the
                              // doubled halo work made the solve slower (42.7 against 38.7 ms)
-constexpr int TW = 384;      // threads per tile workgroup (tile + typical halo)
-constexpr int TL = 2 * TW;   // local-node capacity (tile + halo)
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
-constexpr int TOV = 1024;    // overflow slots per tile (beyond the padded ones)
-constexpr int THB = 11;      // log2 of the halo hash size
-constexpr int THSZ = 1 << THB;
 constexpr int THDR = 16;     // header ints per tile
 constexpr int TMAXS = 8;     // most sweeps per launch
 constexpr int DEFAULT_FUSE = 6;  // sweeps per launch unless F2M_FUSE says otherwise
-static_assert((TOV & (TOV - 1)) == 0 && TOV >= TW, "TOV is a power of two, one speculative record per thread");
-static_assert(TL < 32768 && THSZ >= 2 * TL, "local indices fit a short; hash has room");
+// Tile workgroup shapes: W threads, local-node capacity TL = 2W (tile +
+// halo), a halo hash of THSZ >= 2*TL entries, TOV overflow slots.  W = 384 by
+// default.  768 (F2M_TILE_W_GLOB) lets the block_GLOB lists, whose halos grow
+// faster, fit five levels instead of two on C2 (6 sweeps per launch instead
+// of 3), but the larger recomputed halos made the solve slower (39.1 against
+// 37.2 ms): a launch's time follows its node updates more than its count.
+template <int W>
+struct TCfg {
+  static constexpr int TW = W;
+  static constexpr int TL = 2 * W;
+  static constexpr int THB = W <= 384 ? 11 : 12;  // log2 of the halo hash size
+  static constexpr int THSZ = 1 << THB;
+  static constexpr int TOV = W <= 384 ? 1024 : 2048;
+  static_assert((TOV & (TOV - 1)) == 0 && TOV >= TW, "TOV is a power of two, one speculative record per thread");
+  static_assert(TL < 32768 && THSZ >= 2 * TL, "local indices fit a short; hash has room");
+};
+constexpr int TW_EF = 384, TW_GLOB = 768;
+constexpr int TL_MAX = TCfg<TW_GLOB>::TL, TOV_MAX = TCfg<TW_GLOB>::TOV;
 
 // local copy of an active slot: loc = local index of nbr (-1: not local)
 struct TSlot {
@@ -330,11 +341,19 @@ struct TSlot {
   int32_t loc;
 };
 
-__device__ __forceinline__ unsigned t_hash(int w) { return ((unsigned)w * 2654435761u) >> (32 - THB); }
+template <int W>
+__device__ __forceinline__ unsigned t_hash(int w) {
+  return ((unsigned)w * 2654435761u) >> (32 - TCfg<W>::THB);
+}
 // local index clamped into the buffers (-1 only occurs where it is not read)
-__device__ __forceinline__ unsigned t_loc(int loc) { return min((unsigned)loc, (unsigned)(TL - 1)); }
+template <int W>
+__device__ __forceinline__ unsigned t_loc(int loc) {
+  return min((unsigned)loc, (unsigned)(TCfg<W>::TL - 1));
+}
 
+template <int W>
 __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) {
+  constexpr int TW = W;
   __shared__ double wm[TW / 64];
   v = wave_max(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -356,11 +375,13 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // tpp of local node j at tslot[(t*TL + j)*tpp + k], the others at
 // tovf[t*TOV + offset + k - tpp].  *fit_min = min over tiles of fit: a launch
 // may run up to fit_min + 1 sweeps.
-__global__ void __launch_bounds__(TW) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
+template <int W>
+__global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
                                                    const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
                                                    int32_t* __restrict__ fit_min) {
+  constexpr int TW = W, TL = TCfg<W>::TL, THSZ = TCfg<W>::THSZ, TOV = TCfg<W>::TOV;
   __shared__ int hkey[THSZ];
   __shared__ short hval[THSZ];
   __shared__ int lg[TL];
@@ -396,7 +417,7 @@ __global__ void __launch_bounds__(TW) s_tile_build(int n, int ts, int pp, int tp
           if (q == k) w = wk[q];
         if (k >= pp) w = slot_at(v, k).nbr;
         if (w >= t0 && w < t0 + c0) continue;
-        unsigned h = t_hash(w);
+        unsigned h = t_hash<W>(w);
         int probe = 0;
         for (; probe < THSZ; ++probe) {
           const int prev = atomicCAS(&hkey[h], -1, w);
@@ -445,7 +466,7 @@ __global__ void __launch_bounds__(TW) s_tile_build(int n, int ts, int pp, int tp
       if (e.nbr >= t0 && e.nbr < t0 + c0) {
         loc = e.nbr - t0;
       } else {
-        unsigned h = t_hash(e.nbr);
+        unsigned h = t_hash<W>(e.nbr);
         for (int probe = 0; probe < THSZ; ++probe) {
           const int key = hkey[h];
           if (key == e.nbr) {
@@ -497,12 +518,13 @@ __device__ __forceinline__ void push_padded(Top3& t3, const TSlot* sl, const dou
 // above the lists' usual length) are staged in shared memory with their
 // potentials, the first TW of them loaded with the records; the rest (a
 // third level) only when a tile has more.  Sweeps 2..S read shared memory.
-template <int S, bool FROZEN, int TPP>
-__global__ void __launch_bounds__(TW) s_sweep_tile(const int32_t* __restrict__ hdr, const int4* __restrict__ tnode,
+template <int S, bool FROZEN, int TPP, int W>
+__global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hdr, const int4* __restrict__ tnode,
                                                    const TSlot* __restrict__ tslot, const TSlot* __restrict__ tovf,
                                                    const double* __restrict__ Fin, double* __restrict__ Fout,
                                                    const uint8_t* __restrict__ frozen,
                                                    unsigned long long* __restrict__ maxslots) {
+  constexpr int TW = W, TL = TCfg<W>::TL, TOV = TCfg<W>::TOV;
   constexpr int NP = TL / TW;
   __shared__ double fb[2][TL];
   __shared__ TSlot ov[TOV];
@@ -598,13 +620,13 @@ __global__ void __launch_bounds__(TW) s_sweep_tile(const int32_t* __restrict__ h
         const double f0 = src[j];
         double hn[TPP];  // all reads first (slots past L read a clamped, unused entry)
 #pragma unroll
-        for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * src[t_loc(sl[p][k].loc)];
+        for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * src[t_loc<W>(sl[p][k].loc)];
         Top3 t3;
         push_padded<TPP>(t3, sl[p], hn, 0.5 * f0, v, L);
 #pragma unroll 1
         for (int k = TPP; k < L; ++k) {
           const TSlot o = ov[(nd[p].z + k - TPP) & (TOV - 1)];
-          t3.push(dval(o.D, f0, src[t_loc(o.loc)], v, o.nbr));
+          t3.push(dval(o.D, f0, src[t_loc<W>(o.loc)], v, o.nbr));
         }
         const double g = t3.grad();
         const double f = (FROZEN && fz[p]) ? f0 : f0 + g;
@@ -617,7 +639,7 @@ __global__ void __launch_bounds__(TW) s_sweep_tile(const int32_t* __restrict__ h
       }
     }
   }
-  if (maxslots) tile_max_to(gabs, maxslots + (blockIdx.x & 31) * 16);
+  if (maxslots) tile_max_to<W>(gabs, maxslots + (blockIdx.x & 31) * 16);
 }
 
 // cuda_partition_v4 (graph_cuda_partition.hpp:89-115)
@@ -898,6 +920,7 @@ struct Runner {
   int fuse = 1;
   int ntiles = 0;                    // tiles of the current lists
   int ts_ef = TS, ts_glob = TS_GLOB;  // nodes per tile (F2M_TILE, F2M_TILE_GLOB)
+  int tw = TW_EF, tw_glob = TW_EF;   // tile workgroup shape of the current lists, of block_GLOB's
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
   int tpp = 4;    // padded slots per local node on the current lists
@@ -989,9 +1012,10 @@ struct Runner {
       ts_glob = std::max(32, std::min(TS, env_int("F2M_TILE_GLOB", TS_GLOB)));
       const int64_t mt = (n + std::min(ts_ef, ts_glob) - 1) / std::min(ts_ef, ts_glob);  // most tiles
       F2M_HIP(c, w.thdr.reserve((mt * THDR + 16) * 4));
-      F2M_HIP(c, w.tnode.reserve(mt * TL * 16));
-      F2M_HIP(c, w.tslot.reserve(mt * TL * TPP_MAX * 16));
-      F2M_HIP(c, w.tovf.reserve(mt * TOV * 16));
+      F2M_HIP(c, w.tnode.reserve(mt * TL_MAX * 16));
+      F2M_HIP(c, w.tslot.reserve(mt * TL_MAX * TPP_MAX * 16));
+      F2M_HIP(c, w.tovf.reserve(mt * TOV_MAX * 16));
+      tw_glob = env_int("F2M_TILE_W_GLOB", TW_EF) == TW_GLOB ? TW_GLOB : TW_EF;
     }
     return F2M_OK;
   }
@@ -1152,12 +1176,17 @@ struct Runner {
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
         const int ts = long_lists ? ts_glob : ts_ef;
+        tw = long_lists ? tw_glob : TW_EF;
         ntiles = (n + ts - 1) / ts;
         hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)(fuse - 1));
-        hipLaunchKernelGGL(s_tile_build, dim3(ntiles), dim3(TW), 0, c->stream, n, ts, pp, tpp, fuse - 1,
-                           (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
-                           w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
-                           tile_fit());
+        auto build = [&](auto kern, int threads) {
+          hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, fuse - 1,
+                             (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
+                             w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
+                             tile_fit());
+        };
+        if (tw == TW_GLOB) build(s_tile_build<TW_GLOB>, TW_GLOB);
+        else build(s_tile_build<TW_EF>, TW_EF);
         tiles_pending = true;
         width = 1;
       }
@@ -1208,6 +1237,7 @@ struct Runner {
   }
   template <int S>
   void launch_tile_t(bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
+    const int TW = tw;
     auto go = [&](auto kern) {
       if (ev0)
         hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, ev0, ev1, 0,
@@ -1220,18 +1250,21 @@ struct Runner {
                            (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(), Fo(),
                            (const uint8_t*)w.frozen.as<uint8_t>(), mslots);
     };
-    if (tpp == 6) {
-      if (frozen) go(s_sweep_tile<S, true, 6>);
-      else go(s_sweep_tile<S, false, 6>);
+    if (tpp == 6 && tw == TW_GLOB) {
+      if (frozen) go(s_sweep_tile<S, true, 6, TW_GLOB>);
+      else go(s_sweep_tile<S, false, 6, TW_GLOB>);
+    } else if (tpp == 6) {
+      if (frozen) go(s_sweep_tile<S, true, 6, TW_EF>);
+      else go(s_sweep_tile<S, false, 6, TW_EF>);
     } else {
-      if (frozen) go(s_sweep_tile<S, true, 4>);
-      else go(s_sweep_tile<S, false, 4>);
+      if (frozen) go(s_sweep_tile<S, true, 4, TW_EF>);
+      else go(s_sweep_tile<S, false, 4, TW_EF>);
     }
   }
   // S sweeps in one launch over the tiles; timed like launch_sweep (only
   // launches of the configured width, the dominant kernel, are sampled)
   f2m_status launch_tile(int S, bool frozen, unsigned long long* mslots) {
-    const bool wide = S == fuse;
+    const bool wide = S == fuse && tpp == 4;  // the dominant kernel: block_EF lists at full width
     const bool timed = wide && c->prof_every > 0 && (wide_launches % c->prof_every) == 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {

@@ -167,10 +167,12 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
         assert np.array_equal(ctx.result()["F"], res["F"]), w
         assert (p["sweep_launches"] == st["sweeps"]) if w == 1 else (p["sweep_launches"] < st["sweeps"]), (w, p)
     monkeypatch.delenv("F2M_FUSE")
-    for tg, te in (("256", "256"), ("64", "128")):  # nodes per tile: block_GLOB lists, block_EF lists
+    # nodes per tile of the block_GLOB and block_EF lists; block_GLOB's workgroup shape
+    for tg, te, wg in (("256", "256", "768"), ("64", "128", "384")):
         monkeypatch.setenv("F2M_TILE_GLOB", tg)
         monkeypatch.setenv("F2M_TILE", te)
-        say("solve C2 Morton, tiles", tg, te)
+        monkeypatch.setenv("F2M_TILE_W_GLOB", wg)
+        say("solve C2 Morton, tiles", tg, te, wg)
         s2 = ctx.solve(g)
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], (tg, te)
         assert np.array_equal(ctx.result()["F"], res["F"]), (tg, te)
