@@ -116,7 +116,7 @@ struct PView {
   double* F0;
   double* F1;
   const uint8_t* frozen;
-  int lofs;  // lists in shared memory at b_dyn + lofs (efD, efoff, 16-bit efN), or -1: in global memory
+  int lofs;  // lists in shared memory at b_dyn + lofs (efD, 16-bit efN; offsets efoff), or -1: global
   int ltot;  // their slots
 };
 __device__ __forceinline__ PView pview(const Inst& I) {
@@ -185,10 +185,9 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   const double* eD = I.efD;
   const int32_t* eN = I.efN;
   const uint16_t* eN16 = nullptr;
-  if (LL) {  // lists in shared memory, neighbours as 16 bits
+  if (LL) {  // lists in shared memory, neighbours as 16 bits; offsets in global memory (prefetched)
     eD = b_dyn + I.lofs;
-    eO = reinterpret_cast<const int32_t*>(eD + I.ltot);
-    eN16 = reinterpret_cast<const uint16_t*>(eO + I.n + 1);
+    eN16 = reinterpret_cast<const uint16_t*>(eD + I.ltot);
   }
   auto nbr_at = [&](int k) -> int { return LL ? (int)eN16[k] : eN[k]; };
   // node u's list bounds and first PASS_U slots (past its list: D 0 and the
@@ -270,9 +269,9 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 
 // update_EF (graph_cuda.cu:229-252) inside the workgroup: efoff = exclusive
 // scan of act (global memory), then each node's active prefix copied into the
-// split lists -- in shared memory at b_dyn + lofs (efD doubles, an efoff
-// copy, efN as 16 bits) when they fit in room bytes (lofs >= 0, n <= 65536),
-// else in global memory.  Returns in (lofs, ltot) the shared offset (-1: global) and the
+// split lists -- in shared memory at b_dyn + lofs (efD doubles, then efN as
+// 16 bits; the offsets stay in global memory) when they fit in room bytes
+// (lofs >= 0, n <= 65536), else in global memory.  Returns in (lofs, ltot) the shared offset (-1: global) and the
 // slot count.
 template <int BS>
 __device__ void wg_ef_rebuild(const Inst& I, int* shi, int lofs, int room, int& lofs_out, int& ltot) {
@@ -301,12 +300,10 @@ __device__ void wg_ef_rebuild(const Inst& I, int* shi, int lofs, int room, int& 
   if (threadIdx.x == 0) I.efoff[I.n] = carry;
   __syncthreads();
   // carry (the total) is the same in every thread
-  const bool ll = lofs >= 0 && I.n <= 65536 && (int64_t)carry * 10 + 4 * ((int64_t)I.n + 1) <= room;
+  const bool ll = lofs >= 0 && I.n <= 65536 && (int64_t)carry * 10 <= room;
   if (ll) {
     double* dD = b_dyn + lofs;
-    int32_t* dO = reinterpret_cast<int32_t*>(dD + carry);
-    uint16_t* dN = reinterpret_cast<uint16_t*>(dO + I.n + 1);
-    for (int v = threadIdx.x; v <= I.n; v += BS) dO[v] = I.efoff[v];
+    uint16_t* dN = reinterpret_cast<uint16_t*>(dD + carry);
     for (int v = threadIdx.x; v < I.n; v += BS) {
       const int64_t b = I.b(v);
       const int o = I.efoff[v], L = I.act[v];
