@@ -140,9 +140,25 @@ extern __shared__ double b_dyn[];
 // memory, so a step waits on one global load.  The arithmetic (dval, Top3,
 // F + g) is the same in every mode and Top3 keeps the three smallest values
 // whatever the order they come in, so all modes give the same bits.
+// pre: list bounds of the thread's first two nodes (tid, tid + BS) as
+// (b, e, b, e), loaded once per run of sweeps on the same lists
+__device__ __forceinline__ int4 first_bounds(const PView& I, int bs) {
+  int4 p = make_int4(0, 0, 0, 0);
+  const int u = threadIdx.x;
+  if (u < I.n) {
+    p.x = I.efoff[u];
+    p.y = I.efoff[u + 1];
+  }
+  if (u + bs < I.n) {
+    p.z = I.efoff[u + bs];
+    p.w = I.efoff[u + bs + 1];
+  }
+  return p;
+}
+
 template <int BS, int LPN, bool LL>
 __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bool use_frozen, double* sh, int fmode,
-                                          bool want_max = true) {
+                                          int4 pre, bool want_max = true) {
   const double* Fin = cur ? I.F1 : I.F0;
   double* Fout = cur ? I.F0 : I.F1;
   double gmax = 0.0;
@@ -212,9 +228,7 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   };
   // software pipeline over this thread's nodes v, v+BS, ...: bounds two
   // nodes ahead, slots one node ahead
-  int b, e, b1, e1;
-  bounds(threadIdx.x, b, e);
-  bounds(threadIdx.x + BS, b1, e1);
+  int b = pre.x, e = pre.y, b1 = pre.z, e1 = pre.w;
   double Dk[PASS_U];
   int wk[PASS_U];
   slots(threadIdx.x, b, e, Dk, wk);
@@ -401,8 +415,9 @@ struct Sched {
   }
   __device__ double grad_act() {
     fs_load();
-    if (pv.lofs >= 0) return wg_pass<BS, LPN, true>(pv, cur, false, false, sh, fmode);
-    return wg_pass<BS, LPN, false>(pv, cur, false, false, sh, fmode);
+    const int4 pre = first_bounds(pv, BS);
+    if (pv.lofs >= 0) return wg_pass<BS, LPN, true>(pv, cur, false, false, sh, fmode, pre);
+    return wg_pass<BS, LPN, false>(pv, cur, false, false, sh, fmode, pre);
   }
   // cnt Jacobi sweeps; returns max |g| of the last one when want_max (else
   // 0).  The loop runs on local copies of the state.
@@ -413,14 +428,15 @@ struct Sched {
     const int fm = fmode;
     int c = cur;
     double m = 0.0;
+    const int4 pre = first_bounds(P, BS);  // the lists do not change during these sweeps
     if (P.lofs >= 0) {
       for (int i = 0; i < cnt; ++i) {
-        m = wg_pass<BS, LPN, true>(P, c, true, frozen, shm, fm, want_max && i == cnt - 1);
+        m = wg_pass<BS, LPN, true>(P, c, true, frozen, shm, fm, pre, want_max && i == cnt - 1);
         c ^= 1;
       }
     } else {
       for (int i = 0; i < cnt; ++i) {
-        m = wg_pass<BS, LPN, false>(P, c, true, frozen, shm, fm, want_max && i == cnt - 1);
+        m = wg_pass<BS, LPN, false>(P, c, true, frozen, shm, fm, pre, want_max && i == cnt - 1);
         c ^= 1;
       }
     }
@@ -745,7 +761,7 @@ __device__ int wg_exact(const Inst& I, const f2m_opts& o, double dmin, double* F
 }
 
 template <int BS, int LPN>
-__device__ void solve_instance(const BArgs& a, int inst, double* sh, int* shi, double* xdet, int* xcnt,
+__device__ __attribute__((noinline)) void solve_instance(const BArgs& a, int inst, double* sh, int* shi, double* xdet, int* xcnt,
                                double* sdmin) {
   Inst I;
   I.nb = a.node_off[inst];
