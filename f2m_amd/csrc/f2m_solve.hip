@@ -296,8 +296,8 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
 // ---- spatial tiles: several Jacobi sweeps per launch -------------------------
 // A sweep launch costs ~4.5 us at n = 100k however little it moves (two
 // dependent HBM levels plus the wave ramp), so the schedule's sweeps are run S
-// at a time.  Tile t is the id range [t*ts, t*ts + ts) (ts = TS, or TS_GLOB
-// for the longer block_GLOB lists): with Morton-numbered nodes a compact
+// at a time.  Tile t is the id range [t*ts, t*ts + ts) (ts <= TS, chosen per
+// graph, see Runner::setup): with Morton-numbered nodes a compact
 // region.  Its local set is the tile plus a halo: nodes reachable from the
 // tile over active slots in at most depth edges, numbered breadth first
 // (local index = id - t*ts inside the tile).  A launch of S
@@ -307,10 +307,10 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
 // tile, and writes only the tile's final potentials.  The halo is recomputed
 // by every tile that needs it: the arithmetic per node and sweep is the one
 // of s_sweep_pad, so the potentials are bit-identical to S single sweeps.
-constexpr int TS = 256;      // nodes per tile (lists of the block_EF epochs)
-constexpr int TS_GLOB = 256; // nodes per tile while every list holds >= 5 slots.  128 keeps deeper
-                             // halos within TL on C2 (5 sweeps per launch instead of 3) but the
-                             // doubled halo work made the solve slower (42.7 against 38.7 ms)
+constexpr int TS = 256;      // most nodes per tile (the default is chosen per graph, <= TS).  On C2,
+                             // 128-node tiles for the block_GLOB lists keep deeper halos within TL
+                             // (5 sweeps per launch instead of 3) but the doubled halo work made
+                             // the solve slower (42.7 against 38.7 ms)
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
 constexpr int THDR = 16;     // header ints per tile
 constexpr int TMAXS = 8;     // most sweeps per launch
@@ -919,7 +919,7 @@ struct Runner {
   // rebuilt with the lists, and usable once their overflow flag reads clear
   int fuse = 1;
   int ntiles = 0;                    // tiles of the current lists
-  int ts_ef = TS, ts_glob = TS_GLOB;  // nodes per tile (F2M_TILE, F2M_TILE_GLOB)
+  int ts_ef = TS, ts_glob = TS;  // nodes per tile of the block_EF / block_GLOB lists
   int tw = TW_EF, tw_glob = TW_EF;   // tile workgroup shape of the current lists, of block_GLOB's
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
@@ -1008,8 +1008,17 @@ struct Runner {
     pp = strides[pad_cfg];
     fuse = padded ? std::max(1, std::min(TMAXS, env_int("F2M_FUSE", DEFAULT_FUSE))) : 1;
     if (fuse > 1) {
-      ts_ef = std::max(32, std::min(TS, env_int("F2M_TILE", TS)));
-      ts_glob = std::max(32, std::min(TS, env_int("F2M_TILE_GLOB", TS_GLOB)));
+      // default tile size: the same whole number of tiles on every CU, as
+      // large as that allows up to TS (n = 100k on 256 CUs: 196 nodes, 511
+      // tiles, two per CU).  A launch lasts as long as its busiest CU: with
+      // TS = 256 (391 tiles) half the CUs run two tiles, the others one.
+      int ncu = 0;
+      F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+      ncu = std::max(1, ncu);
+      const int64_t per_cu = (n + (int64_t)TS * ncu - 1) / ((int64_t)TS * ncu);
+      const int ts0 = (int)std::max<int64_t>(64, std::min<int64_t>(TS, (n + per_cu * ncu - 1) / (per_cu * ncu)));
+      ts_ef = std::max(32, std::min(TS, env_int("F2M_TILE", ts0)));
+      ts_glob = std::max(32, std::min(TS, env_int("F2M_TILE_GLOB", ts0)));
       const int64_t mt = (n + std::min(ts_ef, ts_glob) - 1) / std::min(ts_ef, ts_glob);  // most tiles
       F2M_HIP(c, w.thdr.reserve((mt * THDR + 16) * 4));
       F2M_HIP(c, w.tnode.reserve(mt * TL_MAX * 16));
