@@ -242,12 +242,11 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
     if (v < I.n) {
       const int L = e - b;
       const double Fv = Fl[v];
-      Top3 t;
+      static_assert(PASS_U == 4, "the first slots go through top3_of4");
+      double d[PASS_U];
 #pragma unroll
-      for (int k = 0; k < PASS_U; ++k) {
-        const double d = dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]);
-        t.push(k < L ? d : 1e6);  // the sentinel leaves the three values unchanged
-      }
+      for (int k = 0; k < PASS_U; ++k) d[k] = k < L ? dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]) : 1e6;  // sentinel: no change
+      Top3 t = top3_of4(d[0], d[1], d[2], d[3]);
       for (int k = b + PASS_U; k < e; ++k) {
         const int w = nbr_at(k);
         t.push(dval(eD[k], Fv, Fl[w], v, w));

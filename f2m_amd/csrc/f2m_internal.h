@@ -180,6 +180,22 @@ struct Top3 {
   __device__ __forceinline__ double grad() const { return -(b + c) / 2; }
 };
 
+// A fresh Top3 after pushing x0..x3: the three smallest of x0..x3 and the
+// sentinels.  A selection network gives the three smallest of four values
+// (pairs sorted; the least of the lows, then the larger low and the smaller
+// high in order): 9 min/max, and 3 against the sentinel, instead of 20 for
+// four pushes.  min and max return an operand, so the bits are the pushes'.
+__device__ __forceinline__ Top3 top3_of4(double x0, double x1, double x2, double x3) {
+  const double l1 = __builtin_fmin(x0, x1), h1 = __builtin_fmax(x0, x1);
+  const double l2 = __builtin_fmin(x2, x3), h2 = __builtin_fmax(x2, x3);
+  const double m1 = __builtin_fmax(l1, l2), m2 = __builtin_fmin(h1, h2);
+  Top3 t;
+  t.a = __builtin_fmin(__builtin_fmin(l1, l2), t.a);
+  t.b = __builtin_fmin(__builtin_fmin(m1, m2), t.b);
+  t.c = __builtin_fmin(__builtin_fmax(m1, m2), t.c);
+  return t;
+}
+
 // Hoare partition of node v's slots [base, base+size) around thr, permuting
 // (nbr, D, orig) in place (qpart_ptr_B, graph_cuda_partition.hpp:3-25).
 __device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,

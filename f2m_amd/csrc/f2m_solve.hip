@@ -503,11 +503,13 @@ __device__ __forceinline__ double dval_h(double D, double hv, double hw, int v, 
 // <= 1e6), so no lane diverges.
 template <int TPP>
 __device__ __forceinline__ void push_padded(Top3& t3, const TSlot* sl, const double* hn, double hv, int v, int L) {
+  static_assert(TPP >= 4, "the first four slots go through top3_of4");
+  double d[TPP];
 #pragma unroll
-  for (int k = 0; k < TPP; ++k) {
-    const double d = dval_h(sl[k].D, hv, hn[k], v, sl[k].nbr);
-    t3.push(k < L ? d : 1e6);
-  }
+  for (int k = 0; k < TPP; ++k) d[k] = k < L ? dval_h(sl[k].D, hv, hn[k], v, sl[k].nbr) : 1e6;
+  t3 = top3_of4(d[0], d[1], d[2], d[3]);  // t3 is fresh: the same as pushing them
+#pragma unroll
+  for (int k = 4; k < TPP; ++k) t3.push(d[k]);
 }
 
 // S sweeps over the tiles (S <= fit_min + 1); Fin -> Fout, intermediate
