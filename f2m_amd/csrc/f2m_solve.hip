@@ -482,6 +482,9 @@ __global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp
       if (k < tpp) tslot[((int64_t)t * TL + j) * tpp + k] = ts;
       else if (ovo + (k - tpp) < TOV) tovf[(int64_t)t * TOV + ovo + (k - tpp)] = ts;
     }
+    // padded slots past the list: D = +inf and the node itself, so that a
+    // sweep computes them unmasked, as +inf (above every sentinel of Top3)
+    for (int k = L; k < tpp; ++k) tslot[((int64_t)t * TL + j) * tpp + k] = TSlot{__builtin_inf(), v, j};
   }
   __syncthreads();
   if (tid == 0) {
@@ -498,15 +501,15 @@ __device__ __forceinline__ double dval_h(double D, double hv, double hw, int v, 
   const bool lo = v < w;
   return (D + (lo ? hv : hw)) + (lo ? hw : hv);
 }
-// Top3 over the padded slots, every slot pushed: a slot past the list
-// pushes the sentinel 1e6, which leaves the three values unchanged (each is
-// <= 1e6), so no lane diverges.
+// Top3 over the padded slots, every slot pushed, unmasked: a slot past the
+// list holds D = +inf (s_tile_build), so it gives +inf, which leaves the
+// three values unchanged (each is <= 1e6), and no lane diverges.
 template <int TPP>
-__device__ __forceinline__ void push_padded(Top3& t3, const TSlot* sl, const double* hn, double hv, int v, int L) {
+__device__ __forceinline__ void push_padded(Top3& t3, const TSlot* sl, const double* hn, double hv, int v) {
   static_assert(TPP >= 4, "the first four slots go through top3_of4");
   double d[TPP];
 #pragma unroll
-  for (int k = 0; k < TPP; ++k) d[k] = k < L ? dval_h(sl[k].D, hv, hn[k], v, sl[k].nbr) : 1e6;
+  for (int k = 0; k < TPP; ++k) d[k] = dval_h(sl[k].D, hv, hn[k], v, sl[k].nbr);
   t3 = top3_of4(d[0], d[1], d[2], d[3]);  // t3 is fresh: the same as pushing them
 #pragma unroll
   for (int k = 4; k < TPP; ++k) t3.push(d[k]);
@@ -563,8 +566,7 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
       fv[p] = Fin[nd[p].x];
       if (FROZEN) fz[p] = frozen[nd[p].x] != 0;
 #pragma unroll
-      for (int k = 0; k < TPP; ++k)
-        if (k < nd[p].y) fw[p][k] = Fin[sl[p][k].nbr];
+      for (int k = 0; k < TPP; ++k) fw[p][k] = Fin[sl[p][k].nbr];  // padded slots: the node itself
     }
   }
   double of = 0.0;
@@ -589,9 +591,9 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
       const double hv = 0.5 * fv[p];
       double hn[TPP];
 #pragma unroll
-      for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * (k < L ? fw[p][k] : 0.0);
+      for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * fw[p][k];
       Top3 t3;
-      push_padded<TPP>(t3, sl[p], hn, hv, v, L);
+      push_padded<TPP>(t3, sl[p], hn, hv, v);
 #pragma unroll 1
       for (int k = TPP; k < L; ++k) {
         const int i = (nd[p].z + k - TPP) & (TOV - 1);
@@ -624,7 +626,7 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
 #pragma unroll
         for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * src[t_loc<W>(sl[p][k].loc)];
         Top3 t3;
-        push_padded<TPP>(t3, sl[p], hn, 0.5 * f0, v, L);
+        push_padded<TPP>(t3, sl[p], hn, 0.5 * f0, v);
 #pragma unroll 1
         for (int k = TPP; k < L; ++k) {
           const TSlot o = ov[(nd[p].z + k - TPP) & (TOV - 1)];
