@@ -197,10 +197,12 @@ __device__ __forceinline__ Top3 top3_of4(double x0, double x1, double x2, double
 }
 
 // Hoare partition of node v's slots [base, base+size) around thr, permuting
-// (nbr, D, orig) in place (qpart_ptr_B, graph_cuda_partition.hpp:3-25).
-__device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
-                              double* __restrict__ D, int* __restrict__ orig,
-                              const double* __restrict__ F) {
+// (nbr, D, orig) in place (qpart_ptr_B, graph_cuda_partition.hpp:3-25).  This
+// is the step-by-step form; lists of up to 64 slots take qpart_mask below,
+// which makes the same swaps.
+__device__ inline int qpart_B_loop(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
+                                   double* __restrict__ D, int* __restrict__ orig,
+                                   const double* __restrict__ F) {
   const double Fv = F[v];
   int i = 0, j = size - 1;
   while (i < j) {
@@ -224,25 +226,112 @@ __device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __
   return j + 1;
 }
 
+// The predicate of a partition, for a list of up to 64 slots in its current
+// order: bit k of the result = (d of slot k <= thr); *above = the least d
+// above thr (+inf if none).  The slots are read in groups of 8 whose loads
+// do not wait on each other.
+__device__ inline unsigned long long part_mask(int v, int64_t base, int size, double thr,
+                                               const int* __restrict__ nbr, const double* __restrict__ D,
+                                               const double* __restrict__ F, double* above) {
+  const double Fv = F[v];
+  unsigned long long m = 0;
+  double ab = 1.0 / 0.0;
+  for (int k0 = 0; k0 < size; k0 += 8) {
+    int w[8];
+    double dd[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = k0 + q < size ? k0 + q : size - 1;  // past the list: repeats the last slot, unused
+      w[q] = nbr[base + k];
+      dd[q] = D[base + k];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (k0 + q < size) {
+        const double d = dval(dd[q], Fv, F[w[q]], v, w[q]);
+        if (d <= thr) m |= 1ull << (k0 + q);
+        else ab = d < ab ? d : ab;
+      }
+    }
+  }
+  *above = ab;
+  return m;
+}
+
+// The Hoare partition of qpart_B_loop driven by the predicate bits m of the
+// slots in their current order: i = the first slot from i not at or below
+// the threshold, j = the last slot to j at or below it, swapped while i < j.
+// The scans are bit scans and the bits follow the swaps, so the swaps (and
+// the returned pivot) are those of the step-by-step form.
+__device__ inline int qpart_mask(int64_t base, int size, unsigned long long m, int* __restrict__ nbr,
+                                 double* __restrict__ D, int* __restrict__ orig) {
+  const unsigned long long full = size >= 64 ? ~0ull : ((1ull << size) - 1ull);
+  m &= full;
+  int i = 0, j = size - 1;
+  while (i < j) {
+    const unsigned long long z = ~m & full & (i >= 64 ? 0ull : (~0ull << i));  // not at or below, from i
+    i = z ? __builtin_ctzll(z) : size;
+    const unsigned long long o = m & (j >= 63 ? ~0ull : ((2ull << j) - 1ull));  // at or below, up to j
+    j = o ? 63 - __builtin_clzll(o) : -1;
+    if (i < j) {
+      const int64_t si = base + i, sj = base + j;
+      const int tn = nbr[si], to = orig[si];
+      const double tD = D[si];
+      nbr[si] = nbr[sj];
+      D[si] = D[sj];
+      orig[si] = orig[sj];
+      nbr[sj] = tn;
+      D[sj] = tD;
+      orig[sj] = to;
+      m = (m | (1ull << i)) & ~(1ull << j);
+    }
+  }
+  return j + 1;
+}
+
+__device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
+                              double* __restrict__ D, int* __restrict__ orig,
+                              const double* __restrict__ F) {
+  if (size > 64) return qpart_B_loop(v, base, size, thr, nbr, D, orig, F);
+  double above;
+  const unsigned long long m = part_mask(v, base, size, thr, nbr, D, F, &above);
+  return qpart_mask(base, size, m, nbr, D, orig);
+}
+
 // qpart_ptr_EF (graph_cuda_partition.hpp:27-54): raise thr by 1 until >= 3
 // entries lie at or below it.  Re-partitions that admit no new entry swap
-// nothing, so only thresholds admitting a new entry are partitioned.
+// nothing, so only thresholds admitting a new entry are partitioned.  After a
+// partition at thr the slots from the pivot on are exactly those above thr,
+// so their least d is the least d above thr of the whole list.
 __device__ inline int qpart_EF(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
                                double* __restrict__ D, int* __restrict__ orig,
                                const double* __restrict__ F) {
-  int p = qpart_B(v, base, size, thr, nbr, D, orig, F);
-  if (size < 3) return p;
-  const double Fv = F[v];
-  while (p < 3) {
-    double m = 1.0 / 0.0;
-    for (int k = p; k < size; ++k) {
-      int w = nbr[base + k];
-      double d = dval(D[base + k], Fv, F[w], v, w);
-      m = d < m ? d : m;
+  if (size > 64) {
+    int p = qpart_B_loop(v, base, size, thr, nbr, D, orig, F);
+    if (size < 3) return p;
+    const double Fv = F[v];
+    while (p < 3) {
+      double m = 1.0 / 0.0;
+      for (int k = p; k < size; ++k) {
+        int w = nbr[base + k];
+        double d = dval(D[base + k], Fv, F[w], v, w);
+        m = d < m ? d : m;
+      }
+      long long guard = 0;
+      do { thr += 1.0; } while (!(m <= thr) && ++guard < (1ll << 40));
+      p = qpart_B_loop(v, base, size, thr, nbr, D, orig, F);
     }
+    return p;
+  }
+  double above;
+  unsigned long long mask = part_mask(v, base, size, thr, nbr, D, F, &above);
+  int p = qpart_mask(base, size, mask, nbr, D, orig);
+  if (size < 3) return p;
+  while (p < 3) {
     long long guard = 0;
-    do { thr += 1.0; } while (!(m <= thr) && ++guard < (1ll << 40));
-    p = qpart_B(v, base, size, thr, nbr, D, orig, F);
+    do { thr += 1.0; } while (!(above <= thr) && ++guard < (1ll << 40));
+    mask = part_mask(v, base, size, thr, nbr, D, F, &above);  // the slots in their new order
+    p = qpart_mask(base, size, mask, nbr, D, orig);
   }
   return p;
 }
