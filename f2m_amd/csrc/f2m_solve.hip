@@ -388,9 +388,8 @@ __global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp
   __shared__ int s_nl, s_full, s_ov;
   const int t = blockIdx.x, tid = threadIdx.x;
   const int t0 = t * ts, c0 = min(ts, n - t0);
-  auto slot_at = [&](int v, int k) -> const ESlot& {
-    return psl[k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + povo[v] + (k - pp)];
-  };
+  // slot k >= pp of node v, whose overflow offset is ov = povo[v]
+  auto slot_at = [&](int ov, int k) -> const ESlot& { return psl[(int64_t)n * pp + ov + (k - pp)]; };
   for (int i = tid; i < THSZ; i += TW) {
     hkey[i] = -1;
     hval[i] = -1;
@@ -406,16 +405,16 @@ __global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp
   int lo = 0, hi = c0, fit = 0;
   for (int d = 0; d < depth; ++d) {
     for (int j = lo + tid; j < hi; j += TW) {
-      const int v = lg[j], L = act[v];
-      int wk[PP_MAX];  // the padded slots' neighbours, loaded together
+      const int v = lg[j], L = act[v], ov = povo[v];
+      int wk[PP_MAX];  // the padded slots' neighbours, loaded together with act[v] (slots past the list unused)
 #pragma unroll
-      for (int k = 0; k < PP_MAX; ++k) wk[k] = (k < pp && k < L) ? psl[(int64_t)v * pp + k].nbr : 0;
+      for (int k = 0; k < PP_MAX; ++k) wk[k] = k < pp ? psl[(int64_t)v * pp + k].nbr : 0;
       for (int k = 0; k < L; ++k) {
         int w = 0;
 #pragma unroll
         for (int q = 0; q < PP_MAX; ++q)
           if (q == k) w = wk[q];
-        if (k >= pp) w = slot_at(v, k).nbr;
+        if (k >= pp) w = slot_at(ov, k).nbr;
         if (w >= t0 && w < t0 + c0) continue;
         unsigned h = t_hash<W>(w);
         int probe = 0;
@@ -448,20 +447,20 @@ __global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp
     if (tid == 0) hdr[t * THDR + d + 1] = hi;
   }
   for (int j = tid; j < hi; j += TW) {
-    const int v = lg[j], L = act[v];
+    const int v = lg[j], L = act[v], ov = povo[v];
     int ovo = 0;
     if (L > tpp) ovo = atomicAdd(&s_ov, L - tpp);
     tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, 0);
     ESlot ek[PP_MAX];  // the padded slots, loaded together
 #pragma unroll
     for (int k = 0; k < PP_MAX; ++k)
-      if (k < pp && k < L) ek[k] = psl[(int64_t)v * pp + k];
+      if (k < pp) ek[k] = psl[(int64_t)v * pp + k];  // with act[v]; slots past the list unused
     for (int k = 0; k < L; ++k) {
       ESlot e;
 #pragma unroll
       for (int q = 0; q < PP_MAX; ++q)
         if (q == k) e = ek[q];
-      if (k >= pp) e = slot_at(v, k);
+      if (k >= pp) e = slot_at(ov, k);
       int loc = -1;
       if (e.nbr >= t0 && e.nbr < t0 + c0) {
         loc = e.nbr - t0;
