@@ -200,12 +200,14 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   const int32_t* eO = I.efoff;
   const double* eD = I.efD;
   const int32_t* eN = I.efN;
-  const uint16_t* eN16 = nullptr;
-  if (LL) {  // lists in shared memory, neighbours as 16 bits; offsets in global memory (prefetched)
+  // neighbours as 16 bits: in global memory at efN (the shared-memory modes
+  // hold instances of at most 65536 nodes), or in shared memory after D
+  const uint16_t* eN16 = reinterpret_cast<const uint16_t*>(I.efN);
+  if (LL) {  // lists in shared memory; offsets in global memory (prefetched)
     eD = b_dyn + I.lofs;
     eN16 = reinterpret_cast<const uint16_t*>(eD + I.ltot);
   }
-  auto nbr_at = [&](int k) -> int { return LL ? (int)eN16[k] : eN[k]; };
+  auto nbr_at = [&](int k) -> int { return (int)eN16[k]; };
   // node u's list bounds and first PASS_U slots (past its list: D 0 and the
   // node itself, never used); nodes past n have empty lists
   auto bounds = [&](int u, int& bb, int& ee) {
@@ -284,10 +286,13 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 // scan of act (global memory), then each node's active prefix copied into the
 // split lists -- in shared memory at b_dyn + lofs (efD doubles, then efN as
 // 16 bits; the offsets stay in global memory) when they fit in room bytes
-// (lofs >= 0, n <= 65536), else in global memory.  Returns in (lofs, ltot) the shared offset (-1: global) and the
+// (lofs >= 0, n <= 65536), else in global memory (neighbours as 16 bits too
+// when lofs >= 0: the potentials then fit in shared memory, so n <= 20480).  Returns in (lofs, ltot) the shared offset (-1: global) and the
 // slot count.
 template <int BS>
 __device__ void wg_ef_rebuild(const Inst& I, int* shi, int lofs, int room, int& lofs_out, int& ltot) {
+  // the shared-memory passes (lofs >= 0) read 16-bit neighbours, the others 32-bit
+  uint16_t* gN16 = reinterpret_cast<uint16_t*>(I.efN);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int carry = 0;
   for (int base = 0; base < I.n; base += BS) {
@@ -331,7 +336,8 @@ __device__ void wg_ef_rebuild(const Inst& I, int* shi, int lofs, int room, int& 
       const int o = I.efoff[v], L = I.act[v];
       for (int k = 0; k < L; ++k) {
         I.efD[o + k] = I.D[b + k];
-        I.efN[o + k] = I.nbr[b + k];
+        if (lofs >= 0) gN16[o + k] = (uint16_t)I.nbr[b + k];
+        else I.efN[o + k] = I.nbr[b + k];
       }
     }
   }
