@@ -56,6 +56,29 @@ def test_exact_lattice_bit_exact_and_optimal(ctx, orc, golden, n):
     _feasible(rp, nb, res["x"])
 
 
+def test_exact_rounds_in_workgroup_or_grid(ctx, orc, monkeypatch):
+    # the rounds of small active lists run inside one workgroup (x_rounds_wg),
+    # the others as grid-wide steps: all rounds in grid steps (0), the
+    # default switch, and all rounds inside one workgroup give the same bits
+    x, y, mode, K = W.c3(n=20000)
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    ref = None
+    for wg in ("0", None, "100000000"):
+        if wg is None:
+            monkeypatch.delenv("F2M_EXACT_WG", raising=False)
+        else:
+            monkeypatch.setenv("F2M_EXACT_WG", wg)
+        st, res = _compare(ctx, orc, rp, nb, c, f"lattice20000 wg={wg}")
+        assert st["exact_certified"] == 1
+        if ref is None:
+            ref = (st, res)
+        else:
+            for k in EXACT_KEYS:
+                assert st[k] == ref[0][k], (wg, k)
+            assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+
+
 def test_exact_forced_raw_costs(ctx, orc):
     # non-integer costs: certified to 1e-10 relative; forced after a pipeline
     # whose own recovery succeeded, so the optimum must not move
