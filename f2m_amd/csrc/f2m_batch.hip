@@ -208,8 +208,9 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
     eN16 = reinterpret_cast<const uint16_t*>(eD + I.ltot);
   }
   auto nbr_at = [&](int k) -> int { return (int)eN16[k]; };
-  // node u's list bounds and first PASS_U slots (past its list: D 0 and the
-  // node itself, never used); nodes past n have empty lists
+  // node u's list bounds and first PASS_U slots (past its list: D = +inf and
+  // the node itself, which give +inf, above every sentinel of Top3, so no
+  // mask is needed); nodes past n have empty lists
   auto bounds = [&](int u, int& bb, int& ee) {
     bb = ee = 0;
     if (u < I.n) {
@@ -220,7 +221,7 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   auto slots = [&](int u, int bb, int ee, double* Dk, int* wk) {
 #pragma unroll
     for (int k = 0; k < PASS_U; ++k) {
-      Dk[k] = 0.0;
+      Dk[k] = __builtin_inf();
       wk[k] = u;
       if (k < ee - bb) {
         Dk[k] = eD[bb + k];
@@ -242,12 +243,11 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
     int wn[PASS_U];
     slots(v + BS, b1, e1, Dn, wn);
     if (v < I.n) {
-      const int L = e - b;
       const double Fv = Fl[v];
       static_assert(PASS_U == 4, "the first slots go through top3_of4");
       double d[PASS_U];
 #pragma unroll
-      for (int k = 0; k < PASS_U; ++k) d[k] = k < L ? dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]) : 1e6;  // sentinel: no change
+      for (int k = 0; k < PASS_U; ++k) d[k] = dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]);  // +inf past the list
       Top3 t = top3_of4(d[0], d[1], d[2], d[3]);
       for (int k = b + PASS_U; k < e; ++k) {
         const int w = nbr_at(k);
