@@ -243,15 +243,15 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
     int wn[PASS_U];
     slots(v + BS, b1, e1, Dn, wn);
     if (v < I.n) {
-      const double Fv = Fl[v];
+      const double Fv = Fl[v], hv = 0.5 * Fv;
       static_assert(PASS_U == 4, "the first slots go through top3_of4");
       double d[PASS_U];
 #pragma unroll
-      for (int k = 0; k < PASS_U; ++k) d[k] = dval(Dk[k], Fv, Fl[wk[k]], v, wk[k]);  // +inf past the list
+      for (int k = 0; k < PASS_U; ++k) d[k] = dval_h(Dk[k], hv, 0.5 * Fl[wk[k]], v, wk[k]);  // +inf past the list
       Top3 t = top3_of4(d[0], d[1], d[2], d[3]);
       for (int k = b + PASS_U; k < e; ++k) {
         const int w = nbr_at(k);
-        t.push(dval(eD[k], Fv, Fl[w], v, w));
+        t.push(dval_h(eD[k], hv, 0.5 * Fl[w], v, w));
       }
       const double g = t.grad();
       if (write) {

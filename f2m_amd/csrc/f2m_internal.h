@@ -159,6 +159,13 @@ __device__ __forceinline__ double dval(double D, double Fv, double Fw, int v, in
   return t + 0.5 * b;
 }
 
+// Reduced cost of slot (v -> w) from the halves hv = 0.5*F[v], hw = 0.5*F[w]:
+// the operations of dval (0.5*F is exact), so the same bits.
+__device__ __forceinline__ double dval_h(double D, double hv, double hw, int v, int w) {
+  const bool lo = v < w;
+  return (D + (lo ? hv : hw)) + (lo ? hw : hv);
+}
+
 // Three smallest values, sentinels 1e6 (graph_cuda_grad.hpp:68-79).  The
 // reference inserts with a branch chain (v < a: shift both; else v < b; else
 // v < c); the min/max network below keeps the same three values: min and

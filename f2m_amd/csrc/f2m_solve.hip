@@ -494,12 +494,6 @@ __global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp
   }
 }
 
-// Reduced cost of slot (v -> w) from the halves hv = 0.5*F[v], hw = 0.5*F[w]:
-// the operations of dval (0.5*F is exact), so the same bits.
-__device__ __forceinline__ double dval_h(double D, double hv, double hw, int v, int w) {
-  const bool lo = v < w;
-  return (D + (lo ? hv : hw)) + (lo ? hw : hv);
-}
 // Top3 over the padded slots, every slot pushed, unmasked: a slot past the
 // list holds D = +inf (s_tile_build), so it gives +inf, which leaves the
 // three values unchanged (each is <= 1e6), and no lane diverges.
