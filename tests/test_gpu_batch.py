@@ -124,6 +124,15 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
         assert sts == outs[0][0]  # replays of one captured graph: identical bits
         same_stats(sts, ref[0])
         assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    # the size buckets of this batch launched on one stream, or on two
+    for ov in ("0", "1"):
+        monkeypatch.setenv("F2M_BATCH_OVERLAP", ov)
+        say("buckets overlapped on", ov, "side streams")
+        sts = ctx.solve_batched(b)
+        res = ctx.batch_result(b)
+        assert sts == outs[0][0], ov
+        assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    monkeypatch.delenv("F2M_BATCH_OVERLAP")
     monkeypatch.setenv("F2M_NO_GRAPH", "1")
     sts = ctx.solve_batched(b)
     assert sts == outs[0][0]
