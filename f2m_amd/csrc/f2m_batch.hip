@@ -1023,13 +1023,17 @@ struct BatchPlan {
 // Schedule of an instance of n nodes: (lanes per node, workgroup threads).
 // F2M_BATCH_LPN / F2M_BATCH_BS override; else the context's autotuned table
 // (f2m_batch_autotune); else lanes 2 below 1000 nodes and 4 above, 256
-// threads below 4000 nodes, 512 below 10000, 1024 above -- and when the
-// whole batch fits the GPU at a larger workgroup (n_inst * threads <= CUs *
-// 1024), the larger workgroup, since idle CUs would otherwise wait on the
-// slowest instance.
+// threads below 1000 nodes, 512 below 2000, 1024 above (F2M_BATCH_N512,
+// F2M_BATCH_N1024) -- and when the whole batch fits the GPU at a larger
+// workgroup (n_inst * threads <= CUs * 1024), the larger workgroup, since idle
+// CUs would otherwise wait on the slowest instance.  With the buckets
+// overlapped, larger workgroups for mid-size instances finish them sooner and
+// pack better: the 4096-instance batch took 1.02 s at 4000 / 10000 nodes,
+// 0.85 s at 2000 / 5000, 0.81-0.82 s from 2000 / 3000 down to 500 / 2000.
 void batch_cfg_for(const f2m_ctx* c, int64_t n, int n_inst, int ncu, int& lpn, int& bs) {
   lpn = n < 1000 ? 2 : 4;
-  bs = n < 4000 ? 256 : (n < 10000 ? 512 : 1024);
+  const int64_t n512 = env_int("F2M_BATCH_N512", 1000), n1024 = env_int("F2M_BATCH_N1024", 2000);
+  bs = n < n512 ? 256 : (n < n1024 ? 512 : 1024);
   while (bs < 1024 && (int64_t)n_inst * bs * 2 <= (int64_t)ncu * 1024 && n >= 2 * bs) bs *= 2;
   for (const auto& t : c->b_tune)
     if (n <= t.max_n) { lpn = t.lpn; bs = t.bs; break; }
@@ -1262,6 +1266,8 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     key.insert(key.end(), q, q + sizeof t);
   }
   if (const char* e = std::getenv("F2M_BATCH_BS")) key.insert(key.end(), e, e + std::strlen(e));
+  for (const char* nm : {"F2M_BATCH_N512", "F2M_BATCH_N1024"})
+    if (const char* e = std::getenv(nm)) key.insert(key.end(), e, e + std::strlen(e));
   if (const char* e = std::getenv("F2M_BATCH_LDS")) key.insert(key.end(), e, e + std::strlen(e));
   if (const char* e = std::getenv("F2M_BATCH_OVERLAP")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
