@@ -235,6 +235,7 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   double Dk[PASS_U];
   int wk[PASS_U];
   slots(threadIdx.x, b, e, Dk, wk);
+#pragma unroll 2
   for (int base = 0; base < I.n; base += BS) {
     const int v = base + threadIdx.x;
     int b2, e2;
