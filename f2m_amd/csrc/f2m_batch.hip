@@ -1021,6 +1021,28 @@ struct BatchPlan {
   hipEvent_t ev_fork = nullptr, ev_join[MAXSIDE] = {nullptr, nullptr, nullptr};
 };
 
+// The context keeps the two most recently used plans, most recent first, so a
+// training loop that alternates warm-started and cold solves of one batch (two
+// keys) replays both graphs instead of capturing one anew on every call.  The
+// queue order lives in one device buffer (b_order): order_of names the plan
+// whose order it holds.
+struct BatchPlans {
+  BatchPlan* slot[2] = {nullptr, nullptr};
+  const BatchPlan* order_of = nullptr;
+};
+
+static void destroy_plan(BatchPlan* P) {
+  if (!P) return;
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  for (int i = 0; i < BatchPlan::MAXSIDE; ++i) {
+    if (P->ev_join[i]) (void)hipEventDestroy(P->ev_join[i]);
+    if (P->side[i]) (void)hipStreamDestroy(P->side[i]);
+  }
+  delete P;
+}
+
 // Schedule of an instance of n nodes: (lanes per node, workgroup threads).
 // F2M_BATCH_LPN / F2M_BATCH_BS override; else the context's autotuned table
 // (f2m_batch_autotune); else lanes 2 below 1000 nodes and 4 above, 256
@@ -1129,16 +1151,11 @@ hipError_t launch_bucket_any(const BArgs& ak, int bs, int lpn, int grid, size_t 
 
 namespace f2m {
 void release_batch_plan(f2m_ctx* c) {
-  BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
-  if (!P) return;
-  if (P->exec) (void)hipGraphExecDestroy(P->exec);
-  if (P->graph) (void)hipGraphDestroy(P->graph);
-  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
-  for (int i = 0; i < BatchPlan::MAXSIDE; ++i) {
-    if (P->ev_join[i]) (void)hipEventDestroy(P->ev_join[i]);
-    if (P->side[i]) (void)hipStreamDestroy(P->side[i]);
-  }
-  delete P;
+  BatchPlans* Q = static_cast<BatchPlans*>(c->b_plan);
+  if (!Q) return;
+  destroy_plan(Q->slot[0]);
+  destroy_plan(Q->slot[1]);
+  delete Q;
   c->b_plan = nullptr;
 }
 }  // namespace f2m
@@ -1249,6 +1266,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     F2M_HIP(c, w.xe.reserve(N * 4 * 2));
     F2M_HIP(c, w.xact.reserve(N * 4 * 2));
   }
+  if (c->b_clk_on) F2M_HIP(c, c->b_clk.reserve((n_inst + 64) * 8ull));
 
   int ncu = 0;
   F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -1272,12 +1290,28 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   if (const char* e = std::getenv("F2M_BATCH_LDS")) key.insert(key.end(), e, e + std::strlen(e));
   if (const char* e = std::getenv("F2M_BATCH_OVERLAP")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
-  {  // the warm-start source is a captured kernel argument
-    const unsigned char* q = reinterpret_cast<const unsigned char*>(&F_init);
-    key.insert(key.end(), q, q + sizeof F_init);
+  {  // the warm-start source and every workspace are captured kernel arguments
+    auto addp = [&](const void* p) {
+      const unsigned char* q = reinterpret_cast<const unsigned char*>(&p);
+      key.insert(key.end(), q, q + sizeof p);
+    };
+    addp(F_init);
+    for (const DevBuf* d : {&w.nbr, &w.D, &w.orig, &w.F0, &w.F1, &w.act, &w.deg, &w.clen, &w.A, &w.T, &w.root,
+                            &w.irr, &w.sortv, &w.frozen, &w.xw, &w.xc, &w.axb, &w.sortk, &w.xmir, &w.xf, &w.xp,
+                            &w.xe, &w.xact, &c->b_ef, &c->b_efoff, &c->b_stats, &c->b_order, &c->b_cur, &c->b_clk})
+      addp(d->p);
   }
-  if (!c->b_plan) c->b_plan = new BatchPlan;
-  BatchPlan* P = static_cast<BatchPlan*>(c->b_plan);
+  if (!c->b_plan) c->b_plan = new BatchPlans;
+  BatchPlans* Q = static_cast<BatchPlans*>(c->b_plan);
+  if (Q->slot[1] && Q->slot[1]->key == key && !(Q->slot[0] && Q->slot[0]->key == key))
+    std::swap(Q->slot[0], Q->slot[1]);
+  if (!Q->slot[0] || Q->slot[0]->key != key) {
+    if (Q->slot[1] == Q->order_of) Q->order_of = nullptr;
+    destroy_plan(Q->slot[1]);
+    Q->slot[1] = Q->slot[0];
+    Q->slot[0] = new BatchPlan;
+  }
+  BatchPlan* P = Q->slot[0];
   const bool rebuild = P->key != key || !P->exec;
   if (rebuild) {
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
@@ -1285,6 +1319,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     P->exec = nullptr;
     P->graph = nullptr;
     P->key = key;
+    if (Q->order_of == P) Q->order_of = nullptr;
     std::vector<int32_t> ids(n_inst);
     std::iota(ids.begin(), ids.end(), 0);
     std::stable_sort(ids.begin(), ids.end(), [&](int a, int b2) {
@@ -1318,8 +1353,11 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
       P->bucket_dyn.push_back(dyn);
       P->bucket_grid.push_back(grid);
     }
+  }
+  if (Q->order_of != P) {
     F2M_HIP(c, hipMemcpyAsync(c->b_order.p, P->order.data(), n_inst * 4ull, hipMemcpyHostToDevice, c->stream));
     F2M_HIP(c, hipStreamSynchronize(c->stream));
+    Q->order_of = P;
   }
   f2m_stats* dstats = c->b_stats.as<f2m_stats>();
   int32_t* queues = reinterpret_cast<int32_t*>(dstats + n_inst);
@@ -1366,10 +1404,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   a.xe = w.xe.as<int32_t>();
   a.xact = w.xact.as<int32_t>();
   a.clk = nullptr;
-  if (c->b_clk_on) {
-    F2M_HIP(c, c->b_clk.reserve((n_inst + 64) * 8ull));
-    a.clk = c->b_clk.as<long long>();
-  }
+  if (c->b_clk_on) a.clk = c->b_clk.as<long long>();
 
   // The buckets after the first run on up to 3 more streams (bucket k on side
   // stream (k-1) mod s; F2M_BATCH_OVERLAP=s, 0 = one stream), forked after the

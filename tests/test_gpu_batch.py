@@ -282,6 +282,40 @@ def test_batch_warm_start_matches_plain_warm(ctx, batch_case):
     assert all(w["status"] == 0 for w in warm2)
 
 
+def test_batch_plans_warm_cold_alternating(ctx, batch_case):
+    """The context keeps a warm and a cold plan of a batch: alternating
+    solves replay both graphs and give the first results every time, also
+    after a larger batch has grown the workspaces in between."""
+    xs, ys = batch_case
+    xa, ya = xs[:9], ys[:9]
+    b = ctx.knn_build_batched(xa, ya, 16, W.COST_NINT)
+    cold = ctx.solve_batched(b)
+    rc = ctx.batch_result(b)
+    Fw = rc["F"] + 0.01 * np.cos(np.arange(len(rc["F"])))
+    warm = ctx.solve_batched(b, F_init=Fw)
+    rw = ctx.batch_result(b)
+    for rep in range(2):
+        say("alternating cold / warm", rep)
+        assert ctx.solve_batched(b) == cold
+        r = ctx.batch_result(b)
+        assert np.array_equal(r["F"], rc["F"]) and np.array_equal(r["x"], rc["x"])
+        assert ctx.solve_batched(b, F_init=Fw) == warm
+        r = ctx.batch_result(b)
+        assert np.array_equal(r["F"], rw["F"]) and np.array_equal(r["x"], rw["x"])
+    xb, yb = mixed_instances(seed=11, count=40)
+    say("larger batch between")
+    bb = ctx.knn_build_batched(xb, yb, 16, W.COST_NINT)
+    assert all(st["status"] == 0 for st in ctx.solve_batched(bb))
+    b = ctx.knn_build_batched(xa, ya, 16, W.COST_NINT)
+    say("first batch again, warm then cold")
+    assert ctx.solve_batched(b, F_init=Fw) == warm
+    r = ctx.batch_result(b)
+    assert np.array_equal(r["F"], rw["F"]) and np.array_equal(r["x"], rw["x"])
+    assert ctx.solve_batched(b) == cold
+    r = ctx.batch_result(b)
+    assert np.array_equal(r["F"], rc["F"]) and np.array_equal(r["x"], rc["x"])
+
+
 def test_autograd_warm_start(orc):
     import torch
 
