@@ -920,6 +920,11 @@ struct Runner {
   int tw = TW_EF, tw_glob = TW_EF;   // tile workgroup shape of the current lists, of block_GLOB's
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
+  // halo levels the next build of short (0) / long (1) lists runs: one more
+  // than the last build of that kind kept (0: fuse - 1).  Levels no launch
+  // uses are not built; the sweeps' bits do not depend on the width.
+  int depth_next[2] = {0, 0};
+  int pending_kind = 0;
   int tpp = 4;    // padded slots per local node on the current lists
   int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
@@ -1028,6 +1033,7 @@ struct Runner {
   int32_t* tile_fit() const { return w.thdr.as<int32_t>() + (int64_t)ntiles * THDR; }
   void set_width(int32_t fit_min) {
     width = std::max(1, std::min(fuse, fit_min + 1));
+    if (fit_min >= 0) depth_next[pending_kind] = fit_min + 1;
     tiles_pending = false;
   }
 
@@ -1184,9 +1190,11 @@ struct Runner {
         const int ts = long_lists ? ts_glob : ts_ef;
         tw = long_lists ? tw_glob : TW_EF;
         ntiles = (n + ts - 1) / ts;
-        hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)(fuse - 1));
+        pending_kind = long_lists ? 1 : 0;
+        const int depth = depth_next[pending_kind] > 0 ? std::min(fuse - 1, depth_next[pending_kind]) : fuse - 1;
+        hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)depth);
         auto build = [&](auto kern, int threads) {
-          hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, fuse - 1,
+          hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, depth,
                              (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
                              w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
                              tile_fit());
