@@ -156,6 +156,129 @@ static hipError_t scan_i32_impl(f2m_ctx* c, const int32_t* in, Tout* out, int64_
   return e;
 }
 
+// Single-pass exclusive scan (decoupled look-back).  Tile b publishes its
+// aggregate, then walks back over tiles b-1, b-2, ... adding aggregates until
+// it meets a published prefix, and publishes its own prefix.  A status word
+// is (epoch << 34) | (flag << 32) | value, flag 1 = aggregate, 2 = prefix;
+// a word of an earlier call carries another epoch and reads as not ready.
+// Tiles are dispatched in index order and a tile waits only on lower ones,
+// all of which publish, so every wait ends.
+constexpr int LB_T = 256, LB_V = 8, LB_TILE = LB_T * LB_V;
+constexpr unsigned LB_EPOCH_MAX = (1u << 30) - 1;
+
+__device__ __forceinline__ int32_t wave_sum_i32(int32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+__global__ void k_lb_clear(unsigned long long* st, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) st[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(LB_T) k_scan_lb(const int32_t* __restrict__ in, int32_t sub,
+                                                  int32_t* __restrict__ out, int64_t n,
+                                                  unsigned long long* __restrict__ st, unsigned epoch,
+                                                  int32_t* __restrict__ set_ptr, int32_t set_val) {
+  __shared__ int32_t wsum[LB_T / 64];
+  __shared__ int32_t s_pre;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t base = (int64_t)b * LB_TILE + (int64_t)tid * LB_V;
+  int32_t v[LB_V];
+  int32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < LB_V; ++i) {
+    const int64_t idx = base + i;
+    const int32_t x = idx < n ? in[idx] : 0;
+    v[i] = x > sub ? x - sub : 0;
+    s += v[i];
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+  int32_t inc = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int32_t wpre = 0, agg = 0;
+#pragma unroll
+  for (int k = 0; k < LB_T / 64; ++k) {
+    if (k < wid) wpre += wsum[k];
+    agg += wsum[k];
+  }
+  if (wid == 0) {
+    // the first wave looks back 64 tiles at a time: lane l reads tile b-1-l
+    // (below tile 0: a prefix of 0); it stops at the nearest prefix when
+    // every tile above it is ready, adding their aggregates and the prefix
+    const unsigned long long tag = (unsigned long long)epoch << 34;
+    int32_t pre = 0;
+    if (b > 0) {
+      if (lane == 0)
+        __hip_atomic_store(&st[b], tag | (1ull << 32) | (uint32_t)agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      for (int q0 = b - 1;;) {
+        const int q = q0 - lane;
+        const unsigned long long w =
+            q >= 0 ? __hip_atomic_load(&st[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : (tag | (2ull << 32));
+        const unsigned flag = (unsigned)(w >> 32) & 3u;
+        const bool ready = (w >> 34) == epoch && flag != 0;
+        const unsigned long long notready = __ballot(!ready);
+        const unsigned long long pmask = __ballot(ready && flag == 2);
+        if (pmask) {
+          const int lp = __builtin_ctzll(pmask);
+          if ((notready & ((2ull << lp) - 1ull)) == 0) {
+            pre += wave_sum_i32(lane <= lp ? (int32_t)(uint32_t)w : 0);
+            break;
+          }
+        } else if (notready == 0) {
+          pre += wave_sum_i32((int32_t)(uint32_t)w);
+          q0 -= 64;
+          continue;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (lane == 0) {
+      __hip_atomic_store(&st[b], tag | (2ull << 32) | (uint32_t)(pre + agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      s_pre = pre;
+      if (b == 0 && set_ptr) *set_ptr = set_val;
+    }
+  }
+  __syncthreads();
+  int32_t excl = s_pre + wpre + inc - s;
+#pragma unroll
+  for (int i = 0; i < LB_V; ++i) {
+    const int64_t idx = base + i;
+    if (idx < n) out[idx] = excl;
+    excl += v[i];
+  }
+  if (b == (int)gridDim.x - 1 && tid == 0) out[n] = s_pre + agg;
+}
+
+hipError_t scan_excess_i32(f2m_ctx* c, const int32_t* in, int32_t sub, int32_t* out, int64_t n, int32_t* set_ptr,
+                           int32_t set_val) {
+  const int64_t tiles = std::max<int64_t>(1, (n + LB_TILE - 1) / LB_TILE);
+  if (tiles > INT32_MAX / 2) return hipErrorInvalidValue;
+  const void* old = c->k_lbst.p;
+  hipError_t e = c->k_lbst.reserve((size_t)tiles * 8);
+  if (e != hipSuccess) return e;
+  if (c->k_lbst.p != old || c->k_lb_epoch >= LB_EPOCH_MAX) {
+    // fresh memory, or the epochs wrapped: no word may carry a live epoch
+    const int64_t m = (int64_t)(c->k_lbst.cap / 8);
+    hipLaunchKernelGGL(k_lb_clear, dim3(grid_for(m, 256)), dim3(256), 0, c->stream, c->k_lbst.as<unsigned long long>(),
+                       m);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    c->k_lb_epoch = 0;
+  }
+  const unsigned epoch = ++c->k_lb_epoch;
+  hipLaunchKernelGGL(k_scan_lb, dim3((unsigned)tiles), dim3(LB_T), 0, c->stream, in, sub, out, n,
+                     c->k_lbst.as<unsigned long long>(), epoch, set_ptr, set_val);
+  return hipGetLastError();
+}
+
 hipError_t scan_i32_to_i32(f2m_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* total_host) {
   return scan_i32_impl<int32_t>(c, in, out, n, total_host);
 }
@@ -258,7 +381,7 @@ void f2m_ctx_destroy(f2m_ctx* c) {
   f2m::release_batch_plan(c);
   f2m::DevBuf* all[] = {&c->g_rowptr, &c->g_nbr, &c->g_cost, &c->k_x, &c->k_y, &c->k_cell,
                         &c->k_cstart, &c->k_order, &c->k_sx, &c->k_sy, &c->k_knn, &c->k_kcost,
-                        &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->k_geom, &c->b_node_off,
+                        &c->k_rev, &c->k_cur, &c->k_deg, &c->k_red, &c->k_scan, &c->k_geom, &c->k_lbst, &c->b_node_off,
                         &c->b_rowptr, &c->b_nbr, &c->b_cost, &c->b_order, &c->b_stats,
                         &c->b_cur, &c->b_knn, &c->b_kc, &c->b_clk, &c->b_ef, &c->b_efoff, &c->b_finit};
   for (auto* b : all) b->release();

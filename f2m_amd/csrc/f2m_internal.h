@@ -65,6 +65,9 @@ struct f2m_ctx {
   // kNN scratch
   f2m::DevBuf k_x, k_y, k_cell, k_cstart, k_order, k_sx, k_sy, k_knn, k_kcost, k_rev, k_cur,
       k_deg, k_red, k_scan, k_geom;
+  // single-pass scan: per-tile status words, tagged with the call's epoch
+  f2m::DevBuf k_lbst;
+  uint32_t k_lb_epoch = 0;
   // staging of host arrays
   f2m::DevBuf stage[6];
   // spatial ordering scratch (f2m_order.hip)
@@ -112,6 +115,12 @@ hipError_t copy_out(f2m_ctx* c, void* dst, const void* src, size_t bytes);
 // out[n] = total); total_host may be null.  tmp is scratch.
 hipError_t scan_i32_to_i32(f2m_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* total_host);
 hipError_t scan_i32_to_i64(f2m_ctx* c, const int32_t* in, int64_t* out, int64_t n, int64_t* total_host);
+// Exclusive scan of max(in[i] - sub, 0) into out (n+1 entries, out[n] =
+// total) in one launch (tiles chained by decoupled look-back); the totals fit
+// an int32.  With set_ptr, the launch also stores set_val there (stream order
+// for a flag the next kernel reads).
+hipError_t scan_excess_i32(f2m_ctx* c, const int32_t* in, int32_t sub, int32_t* out, int64_t n,
+                           int32_t* set_ptr = nullptr, int32_t set_val = 0);
 
 // Destroys the captured batched-launch plan of a context (f2m_batch.hip).
 void release_batch_plan(f2m_ctx* c);

@@ -207,10 +207,6 @@ __global__ void s_knn_certify(int n, const int32_t* __restrict__ rp, const doubl
 // against 6.2 us for 8,4, 5.8 us for 4,4 and 2,2, 6.5 us for 2,1).
 constexpr int PP_MAX = 8;  // largest padded stride (workspace sizing)
 
-__global__ void s_excess(int n, const int32_t* __restrict__ a, int p, int32_t* __restrict__ out) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < n) out[v] = a[v] > p ? a[v] - p : 0;
-}
 __global__ void s_pad_fill(int n, int pp, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                            const int32_t* __restrict__ ovo, const int32_t* __restrict__ nbr,
                            const double* __restrict__ D, ESlot* __restrict__ sl) {
@@ -969,7 +965,6 @@ struct Runner {
     F2M_HIP(c, w.efoff.reserve((n + 1) * 4));
     F2M_HIP(c, w.psl.reserve(((int64_t)n * PP_MAX + m) * 16));
     F2M_HIP(c, w.povo.reserve((n + 1) * 4));
-    F2M_HIP(c, w.ptmp.reserve(n * 4));
     F2M_HIP(c, w.red.reserve(MAXS * 8 + 3 * RED_BLOCKS * 8 + 64 * 8));
     maxs = w.red.as<unsigned long long>();
     part = reinterpret_cast<double*>(maxs + MAXS);
@@ -1181,18 +1176,20 @@ struct Runner {
     if (!padded || c->prof_every > 0) F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
-      hipLaunchKernelGGL(s_excess, gn, bs, 0, c->stream, n, I(w.act), pp, I(w.ptmp));
-      F2M_HIP(c, scan_i32_to_i32(c, I(w.ptmp), I(w.povo), n, nullptr));
+      // overflow offsets: exclusive scan of max(act - pp, 0); the same launch
+      // sets the tiles' fit counter to the halo depth the build will run
+      const int kind = long_lists ? 1 : 0;
+      const int depth = depth_next[kind] > 0 ? std::min(fuse - 1, depth_next[kind]) : fuse - 1;
+      const bool tiled = fuse > 1;
+      if (tiled) ntiles = (n + (long_lists ? ts_glob : ts_ef) - 1) / (long_lists ? ts_glob : ts_ef);
+      F2M_HIP(c, scan_excess_i32(c, I(w.act), pp, I(w.povo), n, tiled ? tile_fit() : nullptr, (int32_t)depth));
       hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, pp, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
                          w.psl.as<ESlot>());
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
         const int ts = long_lists ? ts_glob : ts_ef;
         tw = long_lists ? tw_glob : TW_EF;
-        ntiles = (n + ts - 1) / ts;
-        pending_kind = long_lists ? 1 : 0;
-        const int depth = depth_next[pending_kind] > 0 ? std::min(fuse - 1, depth_next[pending_kind]) : fuse - 1;
-        hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), (int32_t)depth);
+        pending_kind = kind;
         auto build = [&](auto kern, int threads) {
           hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, depth,
                              (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
