@@ -52,6 +52,25 @@ __global__ void s_zero_u8(uint8_t* p, int64_t n) {
 __global__ void s_set_i32(int32_t* p, int32_t v) {
   if (threadIdx.x == 0) *p = v;
 }
+// One wave: the max of the 32 strided max slots of a region (|g| bits, >= 0)
+// and, with fit, the tiles' fit counter, stored to pinned host memory
+// (host[0], host[1]); the slots are cleared for the region's next use.
+__global__ void s_read_max(unsigned long long* __restrict__ region, const int32_t* __restrict__ fit,
+                           unsigned long long* __restrict__ host) {
+  const int k = threadIdx.x;
+  unsigned long long b = k < 32 ? region[16 * k] : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off, 64);
+    b = o > b ? o : b;
+  }
+  if (k < 32) region[16 * k] = 0ull;
+  if (k == 0) {  // system scope: written through to host memory
+    __hip_atomic_store(&host[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&host[1], fit ? (unsigned long long)(uint32_t)*fit : 0ull, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
 __global__ void s_zero_f64(double* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -930,6 +949,7 @@ struct Runner {
   unsigned long long* acc = nullptr;
   size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
+  bool region_clear[2] = {false, false};  // max regions 0 / 1 hold zeros (cleared by s_read_max)
   double* part = nullptr;              // reduction partials
   double* sums = nullptr;              // final sums [3]
   int32_t* ibuf = nullptr;             // misc ints
@@ -976,6 +996,7 @@ struct Runner {
 
   f2m_status setup() {
     hipLaunchKernelGGL(s_zero_u64, dim3(4), dim3(256), 0, c->stream, maxs, (int64_t)MAXS, INF_BITS);
+    region_clear[0] = region_clear[1] = false;
     hipLaunchKernelGGL(s_zero_u64, dim3(1), dim3(64), 0, c->stream, acc, (int64_t)2, 0ull);
     int nb = grid_for(m, BS);
     if (nb > 1024) nb = 1024;
@@ -1209,22 +1230,26 @@ struct Runner {
     return F2M_OK;
   }
   // max over the 32 strided slots of region r (0: gradient, 1: early exit)
+  // (one launch that also reads the pending tiles' fit and clears the region)
   f2m_status read_max(int r, double& out) {
     unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
-    F2M_HIP(c, hipMemcpyAsync(hp, maxs + 512 * r, 512 * 8, hipMemcpyDeviceToHost, c->stream));
-    int32_t* hfit = reinterpret_cast<int32_t*>(hp + 1024);
     const bool tp = tiles_pending;
-    if (tp) F2M_HIP(c, hipMemcpyAsync(hfit, tile_fit(), 4, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(s_read_max, dim3(1), dim3(64), 0, c->stream, maxs + 512 * r, tp ? tile_fit() : nullptr, hp);
+    F2M_LAUNCH_CHECK(c);
     F2M_HIP(c, hipStreamSynchronize(c->stream));
-    if (tp) set_width(*hfit);
-    unsigned long long b = 0;
-    for (int k = 0; k < 32; ++k) b = hp[16 * k] > b ? hp[16 * k] : b;
-    std::memcpy(&out, &b, 8);
+    region_clear[r] = true;
+    if (tp) set_width((int32_t)(uint32_t)hp[1]);
+    std::memcpy(&out, &hp[0], 8);
     return F2M_OK;
   }
+  // zeros region r before kernels take their max into it (skipped when the
+  // last read left it cleared)
   f2m_status zero_region(int r) {
-    hipLaunchKernelGGL(s_zero_u64, dim3(2), dim3(256), 0, c->stream, maxs + 512 * r, (int64_t)512, 0ull);
-    F2M_LAUNCH_CHECK(c);
+    if (!region_clear[r]) {
+      hipLaunchKernelGGL(s_zero_u64, dim3(2), dim3(256), 0, c->stream, maxs + 512 * r, (int64_t)512, 0ull);
+      F2M_LAUNCH_CHECK(c);
+    }
+    region_clear[r] = false;
     return F2M_OK;
   }
   // gradient over the active lists + max |g| (host round trip)
