@@ -314,6 +314,26 @@ __device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __
   return qpart_mask(base, size, m, nbr, D, orig);
 }
 
+// The threshold after qpart_ptr_EF's `do thr += 1 while (!(above <= thr)
+// && ++guard < 2^40)` (graph_cuda_partition.hpp:27-54).  From an integral
+// thr (|thr| < 2^52) every step is exact and thr + t is an integer, so
+// above <= thr + t holds exactly from t = ceil(above) - thr on: the loop ends
+// at thr + 1, at ceil(above), or at the guard's thr + 2^40.  Any other thr
+// (or a NaN above) runs the loop itself.  part_v2's thresholds start at 1.0.
+// Not inlined: a rare path, kept out of the callers' register allocation.
+__device__ __attribute__((noinline)) double raise_thr(double thr, double above) {
+  if (thr == floor(thr) && fabs(thr) < 4503599627370496.0 && above == above) {
+    const double t1 = thr + 1.0;
+    if (above <= t1) return t1;
+    const double cmax = thr + 1099511627776.0;
+    const double c = ceil(above);
+    return c < cmax ? c : cmax;
+  }
+  long long guard = 0;
+  do { thr += 1.0; } while (!(above <= thr) && ++guard < (1ll << 40));
+  return thr;
+}
+
 // qpart_ptr_EF (graph_cuda_partition.hpp:27-54): raise thr by 1 until >= 3
 // entries lie at or below it.  Re-partitions that admit no new entry swap
 // nothing, so only thresholds admitting a new entry are partitioned.  After a
@@ -333,8 +353,7 @@ __device__ inline int qpart_EF(int v, int64_t base, int size, double thr, int* _
         double d = dval(D[base + k], Fv, F[w], v, w);
         m = d < m ? d : m;
       }
-      long long guard = 0;
-      do { thr += 1.0; } while (!(m <= thr) && ++guard < (1ll << 40));
+      thr = raise_thr(thr, m);
       p = qpart_B_loop(v, base, size, thr, nbr, D, orig, F);
     }
     return p;
@@ -344,8 +363,7 @@ __device__ inline int qpart_EF(int v, int64_t base, int size, double thr, int* _
   int p = qpart_mask(base, size, mask, nbr, D, orig);
   if (size < 3) return p;
   while (p < 3) {
-    long long guard = 0;
-    do { thr += 1.0; } while (!(above <= thr) && ++guard < (1ll << 40));
+    thr = raise_thr(thr, above);
     mask = part_mask(v, base, size, thr, nbr, D, F, &above);  // the slots in their new order
     p = qpart_mask(base, size, mask, nbr, D, orig);
   }
