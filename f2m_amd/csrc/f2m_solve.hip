@@ -226,13 +226,17 @@ __global__ void s_knn_certify(int n, const int32_t* __restrict__ rp, const doubl
 // against 6.2 us for 8,4, 5.8 us for 4,4 and 2,2, 6.5 us for 2,1).
 constexpr int PP_MAX = 8;  // largest padded stride (workspace sizing)
 
+// Four lanes per node, lane q copying slots q, q + 4, ...: the padded slots
+// of consecutive nodes are consecutive, so the stores of a wave coalesce.
+constexpr int PF_LANES = 4;
 __global__ void s_pad_fill(int n, int pp, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                            const int32_t* __restrict__ ovo, const int32_t* __restrict__ nbr,
                            const double* __restrict__ D, ESlot* __restrict__ sl) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / PF_LANES), q = (int)(t % PF_LANES);
   if (v >= n) return;
   const int b = rp[v], L = act[v], oo = ovo[v];
-  for (int k = 0; k < L; ++k) {
+  for (int k = q; k < L; k += PF_LANES) {
     const int64_t idx = k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + oo + (k - pp);
     sl[idx] = ESlot{D[b + k], nbr[b + k], 0};
   }
@@ -1204,8 +1208,8 @@ struct Runner {
       const bool tiled = fuse > 1;
       if (tiled) ntiles = (n + (long_lists ? ts_glob : ts_ef) - 1) / (long_lists ? ts_glob : ts_ef);
       F2M_HIP(c, scan_excess_i32(c, I(w.act), pp, I(w.povo), n, tiled ? tile_fit() : nullptr, (int32_t)depth));
-      hipLaunchKernelGGL(s_pad_fill, gn, bs, 0, c->stream, n, pp, rp, I(w.act), I(w.povo), I(w.nbr), w.D.as<double>(),
-                         w.psl.as<ESlot>());
+      hipLaunchKernelGGL(s_pad_fill, dim3(grid_for((int64_t)n * PF_LANES, BS)), bs, 0, c->stream, n, pp, rp, I(w.act),
+                         I(w.povo), I(w.nbr), w.D.as<double>(), w.psl.as<ESlot>());
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
         const int ts = long_lists ? ts_glob : ts_ef;
