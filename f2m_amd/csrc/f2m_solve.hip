@@ -394,13 +394,16 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // tpp of local node j at tslot[(t*TL + j)*tpp + k], the others at
 // tovf[t*TOV + offset + k - tpp].  *fit_min = min over tiles of fit: a launch
 // may run up to fit_min + 1 sweeps.
-template <int W>
-__global__ void __launch_bounds__(W) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
+// The build runs T threads (TL by default: every level pass and the record
+// pass are one pass over the local nodes) for the tiles of the W-thread sweep.
+template <int W, int T = TCfg<W>::TL>
+__global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
                                                    const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
                                                    int32_t* __restrict__ fit_min) {
-  constexpr int TW = W, TL = TCfg<W>::TL, THSZ = TCfg<W>::THSZ, TOV = TCfg<W>::TOV;
+  constexpr int TW = T, TL = TCfg<W>::TL, THSZ = TCfg<W>::THSZ, TOV = TCfg<W>::TOV;
+  static_assert(T % 64 == 0 && T <= 1024, "whole waves, one workgroup");
   __shared__ int hkey[THSZ];
   __shared__ short hval[THSZ];
   __shared__ int lg[TL];
@@ -1221,8 +1224,8 @@ struct Runner {
                              w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
                              tile_fit());
         };
-        if (tw == TW_GLOB) build(s_tile_build<TW_GLOB>, TW_GLOB);
-        else build(s_tile_build<TW_EF>, TW_EF);
+        if (tw == TW_GLOB) build(s_tile_build<TW_GLOB, 1024>, 1024);
+        else build(s_tile_build<TW_EF>, TCfg<TW_EF>::TL);
         tiles_pending = true;
         width = 1;
       }
