@@ -16,7 +16,7 @@ Prints one JSON line.  `value` = ms per solve (lower is better).
 The timed solves run without instrumentation.
 
 roofline: the dominant kernel is the multi-sweep launch over spatial tiles
-(s_sweep_tile, F2M_FUSE sweeps per launch, default 6).  Its algorithmic
+(s_sweep_tile, F2M_FUSE sweeps per launch, default 7).  Its algorithmic
 bytes per launch are, per sweep it runs, 12 B per active slot (D double +
 nbr int32) + 20 B per node (list offset, F in, F out), counted on the device
 for the timed launches; the average kernel duration comes from the HIP
@@ -147,7 +147,8 @@ def main():
     bytes_per_launch = prof_tot["sampled_alg_bytes"] / samples
     sweeps_per_launch = prof_tot["sampled_sweeps"] / samples
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    fuse = int(os.environ.get("F2M_FUSE", "6"))
+    # the sampled launches are those of the configured width (F2M_FUSE)
+    fuse = int(round(sweeps_per_launch))
     kernel = f"s_sweep_tile<{fuse},false,4>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")

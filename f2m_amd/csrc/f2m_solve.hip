@@ -333,7 +333,9 @@ constexpr int TS = 256;      // most nodes per tile (the default is chosen per g
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
 constexpr int THDR = 16;     // header ints per tile
 constexpr int TMAXS = 8;     // most sweeps per launch
-constexpr int DEFAULT_FUSE = 6;  // sweeps per launch unless F2M_FUSE says otherwise
+// sweeps per launch unless F2M_FUSE says otherwise.  C2 at widths 5 / 6 / 7 / 8: 28.8 / 28.4 /
+// 27.95 / 28.0 ms (after the cheaper list rebuilds; 6 was best before them).
+constexpr int DEFAULT_FUSE = 7;
 // Tile workgroup shapes: W threads, local-node capacity TL = 2W (tile +
 // halo), a halo hash of THSZ >= 2*TL entries, TOV overflow slots.  W = 384 by
 // default.  768 (F2M_TILE_W_GLOB) lets the block_GLOB lists, whose halos grow
