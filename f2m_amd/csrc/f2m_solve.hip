@@ -671,6 +671,85 @@ __global__ void s_part_v4(int n, const int32_t* __restrict__ rp, const int32_t* 
   if (v < n) act[v] = part_v4_node(v, rp[v], deg[v], kk, nbr, D, orig, F);
 }
 
+// The same partition with four lanes per node: lane q computes the d of
+// slots q, q + 4, ... (a row's slots are loaded four at a time, consecutive),
+// the threshold is the max of the first kk d's in slot order (kk <= 8; they
+// sit in the lanes' first two chunks), the predicate bits are or-ed across
+// the four lanes, and the group's first lane runs qpart_mask on them: the
+// bits, swaps and pivot of part_v4_node (p >= kk >= 3, so qpart_EF's raise
+// loop never runs).  Lists above 64 slots, kk < 3 or kk > 8 take
+// part_v4_node on the first lane.
+constexpr int PV4_LANES = 4;
+__device__ __attribute__((noinline)) int part_v4_long(int v, int64_t b, int dg, int kk, int32_t* __restrict__ nbr,
+                                                      double* __restrict__ D, int32_t* __restrict__ orig,
+                                                      const double* __restrict__ F) {
+  return part_v4_node(v, b, dg, kk, nbr, D, orig, F);
+}
+__global__ void __launch_bounds__(BS) s_part_v4_q(int n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ deg, int32_t* __restrict__ act, int kk,
+                                                  int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                  int32_t* __restrict__ orig, const double* __restrict__ F) {
+  const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int v = (int)(t / PV4_LANES), q = (int)(t % PV4_LANES);
+  if (v >= n) return;  // the four lanes of a node leave together
+  const int dg = deg[v];
+  if (dg < kk) {
+    if (q == 0) act[v] = dg;
+    return;
+  }
+  const int64_t b = rp[v];
+  if (dg > 64 || kk < 3 || kk > 8) {
+    if (q == 0) act[v] = part_v4_long(v, b, dg, kk, nbr, D, orig, F);
+    return;
+  }
+  const double Fv = F[v];
+  // slots q and q + 4: the threshold's candidates
+  double d0 = 0.0, d1 = 0.0;
+  {
+    int w[2];
+    double dd[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int k = q + 4 * u < dg ? q + 4 * u : dg - 1;  // past the list: repeats the last slot, unused
+      w[u] = nbr[b + k];
+      dd[u] = D[b + k];
+    }
+    d0 = dval(dd[0], Fv, F[w[0]], v, w[0]);
+    d1 = dval(dd[1], Fv, F[w[1]], v, w[1]);
+  }
+  double thr = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (i < kk) {
+      const double di = __shfl(i < 4 ? d0 : d1, i & 3, PV4_LANES);
+      thr = (i == 0 || di > thr) ? di : thr;
+    }
+  unsigned long long m = 0;
+  if (q < dg && d0 <= thr) m |= 1ull << q;
+  if (q + 4 < dg && d1 <= thr) m |= 1ull << (q + 4);
+  for (int k0 = 8; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
+    int w[4];
+    double dd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + q + 4 * u < dg ? k0 + q + 4 * u : dg - 1;
+      w[u] = nbr[b + k];
+      dd[u] = D[b + k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + q + 4 * u;
+      if (k < dg && dval(dd[u], Fv, F[w[u]], v, w[u]) <= thr) m |= 1ull << k;
+    }
+  }
+  unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+  lo |= __shfl_xor(lo, 1, PV4_LANES);
+  hi |= __shfl_xor(hi, 1, PV4_LANES);
+  lo |= __shfl_xor(lo, 2, PV4_LANES);
+  hi |= __shfl_xor(hi, 2, PV4_LANES);
+  if (q == 0) act[v] = qpart_mask(b, dg, ((unsigned long long)hi << 32) | lo, nbr, D, orig);
+}
+
 // cuda_partition_v2 (graph_cuda_partition.hpp:57-69)
 __global__ void s_part_v2(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
                           int32_t* __restrict__ act, double thr, int32_t* __restrict__ nbr,
@@ -958,7 +1037,8 @@ struct Runner {
   unsigned long long* acc = nullptr;
   size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
-  bool region_clear[2] = {false, false};  // max regions 0 / 1 hold zeros (cleared by s_read_max)
+  bool region_clear[2] = {false, false};
+  int part_lanes = PV4_LANES;  // lanes per node of block_GLOB's partition (F2M_PART_LANES: 4, or 1)  // max regions 0 / 1 hold zeros (cleared by s_read_max)
   double* part = nullptr;              // reduction partials
   double* sums = nullptr;              // final sums [3]
   int32_t* ibuf = nullptr;             // misc ints
@@ -1024,6 +1104,7 @@ struct Runner {
     F2M_LAUNCH_CHECK(c);
     eps = o.err_grad * o.err_comb_mult;
     lpn = env_int("F2M_LPN", 4);
+    part_lanes = env_int("F2M_PART_LANES", PV4_LANES) == 1 ? 1 : PV4_LANES;
     const char* v = std::getenv("F2M_SWEEP");
     padded = !(v && std::strcmp(v, "compact") == 0);
     if (const char* pv = std::getenv("F2M_PAD")) {
@@ -1374,8 +1455,12 @@ struct Runner {
     if ((s = ef_rebuild(long_lists))) return s;
     for (; go < o.glob_rounds; ++go) {
       if ((s = sweep_n(o.glob_sweeps, false))) return s;
-      hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
-                         o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
+      if (part_lanes == PV4_LANES)
+        hipLaunchKernelGGL(s_part_v4_q, dim3(grid_for((int64_t)n * PV4_LANES, BS)), dim3(BS), 0, c->stream, n, rp,
+                           I(w.deg), I(w.act), o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
+      else
+        hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
+                           o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
       F2M_LAUNCH_CHECK(c);
       if ((s = ef_rebuild(long_lists))) return s;
       double M;

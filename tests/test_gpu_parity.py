@@ -158,7 +158,7 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     p = ctx.profile()
     assert p["sweeps"] == st["sweeps"] and p["sweep_launches"] < st["sweeps"] / 3, p
     g = ctx.knn_build(x, y, K, mode)
-    for w in (1, 2, 3, 4, 5, 7, 8):
+    for w in (1, 2, 3, 4, 5, 6, 8):
         monkeypatch.setenv("F2M_FUSE", str(w))
         say("solve C2 Morton, F2M_FUSE", w)
         s2 = ctx.solve(g)
@@ -167,6 +167,13 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
         assert np.array_equal(ctx.result()["F"], res["F"]), w
         assert (p["sweep_launches"] == st["sweeps"]) if w == 1 else (p["sweep_launches"] < st["sweeps"]), (w, p)
     monkeypatch.delenv("F2M_FUSE")
+    # block_GLOB's partition with one thread per node instead of four lanes
+    monkeypatch.setenv("F2M_PART_LANES", "1")
+    say("solve C2 Morton, F2M_PART_LANES 1")
+    s2 = ctx.solve(g)
+    assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
+    assert np.array_equal(ctx.result()["F"], res["F"]) and np.array_equal(ctx.result()["x"], res["x"])
+    monkeypatch.delenv("F2M_PART_LANES")
     # nodes per tile of the block_GLOB and block_EF lists; block_GLOB's workgroup shape
     for tg, te, wg in (("256", "256", "768"), ("64", "128", "384")):
         monkeypatch.setenv("F2M_TILE_GLOB", tg)
