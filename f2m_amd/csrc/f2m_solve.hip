@@ -406,11 +406,15 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
                                                    int32_t* __restrict__ fit_min) {
   constexpr int TW = T, TL = TCfg<W>::TL, THSZ = TCfg<W>::THSZ, TOV = TCfg<W>::TOV;
   static_assert(T % 64 == 0 && T <= 1024, "whole waves, one workgroup");
+  // four lanes per local node in the level and record passes: lane q takes
+  // slots q, q + 4, ... (a node's padded slots are read together)
+  constexpr int LN = 4, NG = T / LN;
   __shared__ int hkey[THSZ];
   __shared__ short hval[THSZ];
   __shared__ int lg[TL];
   __shared__ int s_nl, s_full, s_ov;
   const int t = blockIdx.x, tid = threadIdx.x;
+  const int q = tid & (LN - 1), g = tid / LN;
   const int t0 = t * ts, c0 = min(ts, n - t0);
   // slot k >= pp of node v, whose overflow offset is ov = povo[v]
   auto slot_at = [&](int ov, int k) -> const ESlot& { return psl[(int64_t)n * pp + ov + (k - pp)]; };
@@ -428,17 +432,10 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
   __syncthreads();
   int lo = 0, hi = c0, fit = 0;
   for (int d = 0; d < depth; ++d) {
-    for (int j = lo + tid; j < hi; j += TW) {
+    for (int j = lo + g; j < hi; j += NG) {
       const int v = lg[j], L = act[v], ov = povo[v];
-      int wk[PP_MAX];  // the padded slots' neighbours, loaded together with act[v] (slots past the list unused)
-#pragma unroll
-      for (int k = 0; k < PP_MAX; ++k) wk[k] = k < pp ? psl[(int64_t)v * pp + k].nbr : 0;
-      for (int k = 0; k < L; ++k) {
-        int w = 0;
-#pragma unroll
-        for (int q = 0; q < PP_MAX; ++q)
-          if (q == k) w = wk[q];
-        if (k >= pp) w = slot_at(ov, k).nbr;
+      for (int k = q; k < L; k += LN) {
+        const int w = k < pp ? psl[(int64_t)v * pp + k].nbr : slot_at(ov, k).nbr;
         if (w >= t0 && w < t0 + c0) continue;
         unsigned h = t_hash<W>(w);
         int probe = 0;
@@ -470,21 +467,14 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
     fit = d + 1;
     if (tid == 0) hdr[t * THDR + d + 1] = hi;
   }
-  for (int j = tid; j < hi; j += TW) {
+  for (int j = g; j < hi; j += NG) {  // the four lanes of a node run together
     const int v = lg[j], L = act[v], ov = povo[v];
     int ovo = 0;
-    if (L > tpp) ovo = atomicAdd(&s_ov, L - tpp);
-    tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, 0);
-    ESlot ek[PP_MAX];  // the padded slots, loaded together
-#pragma unroll
-    for (int k = 0; k < PP_MAX; ++k)
-      if (k < pp) ek[k] = psl[(int64_t)v * pp + k];  // with act[v]; slots past the list unused
-    for (int k = 0; k < L; ++k) {
-      ESlot e;
-#pragma unroll
-      for (int q = 0; q < PP_MAX; ++q)
-        if (q == k) e = ek[q];
-      if (k >= pp) e = slot_at(ov, k);
+    if (q == 0 && L > tpp) ovo = atomicAdd(&s_ov, L - tpp);
+    ovo = __shfl(ovo, 0, LN);
+    if (q == 0) tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, 0);
+    for (int k = q; k < L; k += LN) {
+      const ESlot e = k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k);
       int loc = -1;
       if (e.nbr >= t0 && e.nbr < t0 + c0) {
         loc = e.nbr - t0;
@@ -507,7 +497,8 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
     }
     // padded slots past the list: D = +inf and the node itself, so that a
     // sweep computes them unmasked, as +inf (above every sentinel of Top3)
-    for (int k = L; k < tpp; ++k) tslot[((int64_t)t * TL + j) * tpp + k] = TSlot{__builtin_inf(), v, j};
+    for (int k = q; k < tpp; k += LN)
+      if (k >= L) tslot[((int64_t)t * TL + j) * tpp + k] = TSlot{__builtin_inf(), v, j};
   }
   __syncthreads();
   if (tid == 0) {
