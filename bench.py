@@ -146,9 +146,11 @@ def main():
     launch_ms = prof_tot["sweep_ms_sampled"] / samples
     bytes_per_launch = prof_tot["sampled_alg_bytes"] / samples
     sweeps_per_launch = prof_tot["sampled_sweeps"] / samples
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    # no full-width launch sampled (the halos of a tile configuration did not
+    # fit the configured width): no rate to report
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None
     # the sampled launches are those of the configured width (F2M_FUSE)
-    fuse = int(round(sweeps_per_launch))
+    fuse = int(round(sweeps_per_launch)) if launch_ms > 0 else int(os.environ.get("F2M_FUSE", "7"))
     kernel = f"s_sweep_tile<{fuse},false,4>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
@@ -157,8 +159,8 @@ def main():
             tj = json.load(f)
         if tj.get("kernel") == kernel:
             traffic = tj.get("bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None, "traffic": traffic,
                 "kernel": kernel, "avg_launch_us": round(launch_ms * 1e3, 3),
                 "sweeps_per_launch": round(sweeps_per_launch, 2),
                 "alg_bytes_per_launch": round(bytes_per_launch, 1),
