@@ -68,6 +68,8 @@ struct f2m_ctx {
   // single-pass scan: per-tile status words, tagged with the call's epoch
   f2m::DevBuf k_lbst;
   uint32_t k_lb_epoch = 0;
+  // sequence number of the plain solve's max read-backs (pinned word 2050)
+  unsigned long long read_seq = 0;
   // staging of host arrays
   f2m::DevBuf stage[6];
   // spatial ordering scratch (f2m_order.hip)

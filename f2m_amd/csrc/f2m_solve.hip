@@ -54,9 +54,11 @@ __global__ void s_set_i32(int32_t* p, int32_t v) {
 }
 // One wave: the max of the 32 strided max slots of a region (|g| bits, >= 0)
 // and, with fit, the tiles' fit counter, stored to pinned host memory
-// (host[0], host[1]); the slots are cleared for the region's next use.
+// (host[0], host[1]), then seq to host[2] with a system-scope release, so a
+// host that reads seq there reads the values too; the slots are cleared for
+// the region's next use.
 __global__ void s_read_max(unsigned long long* __restrict__ region, const int32_t* __restrict__ fit,
-                           unsigned long long* __restrict__ host) {
+                           unsigned long long* __restrict__ host, unsigned long long seq) {
   const int k = threadIdx.x;
   unsigned long long b = k < 32 ? region[16 * k] : 0ull;
 #pragma unroll
@@ -69,6 +71,7 @@ __global__ void s_read_max(unsigned long long* __restrict__ region, const int32_
     __hip_atomic_store(&host[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[1], fit ? (unsigned long long)(uint32_t)*fit : 0ull, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&host[2], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 __global__ void s_zero_f64(double* p, int64_t n) {
@@ -1028,8 +1031,8 @@ struct Runner {
   unsigned long long* acc = nullptr;
   size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
-  bool region_clear[2] = {false, false};
-  int part_lanes = PV4_LANES;  // lanes per node of block_GLOB's partition (F2M_PART_LANES: 4, or 1)  // max regions 0 / 1 hold zeros (cleared by s_read_max)
+  bool region_clear[2] = {false, false};  // max regions 0 / 1 hold zeros (cleared by s_read_max)
+  int part_lanes = PV4_LANES;  // lanes per node of block_GLOB's partition (F2M_PART_LANES: 4, or 1)
   double* part = nullptr;              // reduction partials
   double* sums = nullptr;              // final sums [3]
   int32_t* ibuf = nullptr;             // misc ints
@@ -1312,15 +1315,30 @@ struct Runner {
   }
   // max over the 32 strided slots of region r (0: gradient, 1: early exit)
   // (one launch that also reads the pending tiles' fit and clears the region)
+  // The host waits by reading the kernel's sequence number in pinned memory
+  // (a few us sooner than a stream synchronisation); after 50 ms it falls
+  // back to the synchronisation, which also reports any error.
   f2m_status read_max(int r, double& out) {
-    unsigned long long* hp = static_cast<unsigned long long*>(c->pinned);
+    unsigned long long* rb = static_cast<unsigned long long*>(c->pinned) + 2048;
     const bool tp = tiles_pending;
-    hipLaunchKernelGGL(s_read_max, dim3(1), dim3(64), 0, c->stream, maxs + 512 * r, tp ? tile_fit() : nullptr, hp);
+    const unsigned long long want = ++c->read_seq;
+    hipLaunchKernelGGL(s_read_max, dim3(1), dim3(64), 0, c->stream, maxs + 512 * r, tp ? tile_fit() : nullptr, rb,
+                       want);
     F2M_LAUNCH_CHECK(c);
-    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    bool seen = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 1;; ++it) {
+      if (__atomic_load_n(&rb[2], __ATOMIC_ACQUIRE) == want) {
+        seen = true;
+        break;
+      }
+      if ((it & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+    }
+    if (!seen) F2M_HIP(c, hipStreamSynchronize(c->stream));
     region_clear[r] = true;
-    if (tp) set_width((int32_t)(uint32_t)hp[1]);
-    std::memcpy(&out, &hp[0], 8);
+    if (tp) set_width((int32_t)(uint32_t)__atomic_load_n(&rb[1], __ATOMIC_ACQUIRE));
+    const unsigned long long bits = __atomic_load_n(&rb[0], __ATOMIC_ACQUIRE);
+    std::memcpy(&out, &bits, 8);
     return F2M_OK;
   }
   // zeros region r before kernels take their max into it (skipped when the
