@@ -1488,11 +1488,11 @@ struct Runner {
     if (s) return s;
     return grad_max(M);
   }
-  // block_EF (graph_cuda.cu:308-338)
-  f2m_status block_ef(int N, bool frozen) {
-    double MA;
+  // block_EF (graph_cuda.cu:308-338).  Its opening max |g| is MA: every
+  // caller runs it right after check_opt, whose last step computed that same
+  // max on the same lists and potentials.
+  f2m_status block_ef(int N, bool frozen, double MA) {
     f2m_status s;
-    if ((s = grad_max(MA))) return s;
     if (!o.early_exit) {
       int cnt = (MA < o.err_grad) ? std::min(N, o.check_every) : N;
       return sweep_n(cnt, frozen);
@@ -1520,7 +1520,7 @@ struct Runner {
     if ((s = check_opt(M))) return s;
     int factor = o.factor0, go = 1;
     for (; go <= o.max_outer; ++go) {
-      if ((s = block_ef(factor, false))) return s;
+      if ((s = block_ef(factor, false, M))) return s;
       if ((s = check_opt(M))) return s;
       if (factor < o.factor_max) factor += o.factor_step;
       if (M < o.err_grad) break;
@@ -1533,7 +1533,7 @@ struct Runner {
     f2m_status s;
     if ((s = check_opt(M))) return s;
     for (int go = 1; go <= o.fix_rounds; ++go) {
-      if ((s = block_ef(o.fix_factor, false))) return s;
+      if ((s = block_ef(o.fix_factor, false, M))) return s;
       if ((s = check_opt(M))) return s;
       if (M < o.err_grad) break;
     }
