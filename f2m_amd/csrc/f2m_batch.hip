@@ -473,8 +473,9 @@ struct Sched {
     rebuild();
     return grad_act();
   }
-  __device__ void block_ef(int N, bool frozen) {
-    const double MA = grad_act();
+  // block_EF; MA = its opening max |g|, which check_opt (run right before
+  // by every caller) computed on the same lists and potentials
+  __device__ void block_ef(int N, bool frozen, double MA) {
     if (!o.early_exit) {
       const int cnt = (MA < o.err_grad) ? min(N, o.check_every) : N;
       sweep_n(cnt, frozen);
@@ -492,7 +493,7 @@ struct Sched {
     M = check_opt();
     int factor = o.factor0, go = 1;
     for (; go <= o.max_outer; ++go) {
-      block_ef(factor, false);
+      block_ef(factor, false, M);
       M = check_opt();
       if (factor < o.factor_max) factor += o.factor_step;
       if (M < o.err_grad) break;
@@ -502,7 +503,7 @@ struct Sched {
   __device__ double run_fix() {
     double M = check_opt();
     for (int go = 1; go <= o.fix_rounds; ++go) {
-      block_ef(o.fix_factor, false);
+      block_ef(o.fix_factor, false, M);
       M = check_opt();
       if (M < o.err_grad) break;
     }
