@@ -744,6 +744,79 @@ __global__ void __launch_bounds__(BS) s_part_v4_q(int n, const int32_t* __restri
   if (q == 0) act[v] = qpart_mask(b, dg, ((unsigned long long)hi << 32) | lo, nbr, D, orig);
 }
 
+// check_opt's partition (part_v2_node: lists of 10 or more slots at thr 1.0)
+// with four lanes per node as s_part_v4_q.  Each pass: the four lanes read
+// the row in its current order (four slots at a time), or their predicate
+// bits and take the least d above thr (a min: the same in any order); the
+// first lane swaps (qpart_mask).  While fewer than 3 slots lie at or below
+// thr, thr is raised (raise_thr, the same on every lane) and the permuted row
+// is partitioned again, as qpart_EF does.  Lists above 64 slots take
+// part_v2_node on the first lane.
+__device__ __attribute__((noinline)) int part_v2_long(int v, int64_t b, int dg, double thr,
+                                                      int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                      int32_t* __restrict__ orig, const double* __restrict__ F) {
+  return part_v2_node(v, b, dg, thr, nbr, D, orig, F);
+}
+__global__ void __launch_bounds__(BS) s_part_v2_q(int n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ deg, int32_t* __restrict__ act,
+                                                  double thr, int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                  int32_t* __restrict__ orig, const double* __restrict__ F) {
+  const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int v = (int)(t / PV4_LANES), q = (int)(t % PV4_LANES);
+  if (v >= n) return;  // the four lanes of a node leave together
+  const int dg = deg[v];
+  if (dg < 10) {
+    if (q == 0) act[v] = dg;
+    return;
+  }
+  const int64_t b = rp[v];
+  if (dg > 64) {
+    if (q == 0) act[v] = part_v2_long(v, b, dg, thr, nbr, D, orig, F);
+    return;
+  }
+  const double Fv = F[v];
+  for (int pass = 0;; ++pass) {
+    unsigned long long m = 0;
+    double ab = 1.0 / 0.0;
+    for (int k0 = 0; k0 < dg; k0 += 16) {  // 16 slots (four per lane) per step
+      int w[4];
+      double dd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + q + 4 * u < dg ? k0 + q + 4 * u : dg - 1;  // past the list: the last slot, unused
+        w[u] = nbr[b + k];
+        dd[u] = D[b + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + q + 4 * u;
+        if (k < dg) {
+          const double d = dval(dd[u], Fv, F[w[u]], v, w[u]);
+          if (d <= thr) m |= 1ull << k;
+          else ab = d < ab ? d : ab;
+        }
+      }
+    }
+    unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+#pragma unroll
+    for (int o = 1; o < PV4_LANES; o <<= 1) {
+      lo |= __shfl_xor(lo, o, PV4_LANES);
+      hi |= __shfl_xor(hi, o, PV4_LANES);
+      const double a2 = __shfl_xor(ab, o, PV4_LANES);
+      ab = a2 < ab ? a2 : ab;
+    }
+    int p = 0;
+    if (q == 0) p = qpart_mask(b, dg, ((unsigned long long)hi << 32) | lo, nbr, D, orig);
+    p = __shfl(p, 0, PV4_LANES);
+    if (p >= 3) {
+      if (q == 0) act[v] = p;
+      return;
+    }
+    __threadfence_block();  // the first lane's swaps, before the lanes read the row again
+    thr = raise_thr(thr, ab);
+  }
+}
+
 // cuda_partition_v2 (graph_cuda_partition.hpp:57-69)
 __global__ void s_part_v2(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ deg,
                           int32_t* __restrict__ act, double thr, int32_t* __restrict__ nbr,
@@ -1032,7 +1105,7 @@ struct Runner {
   size_t prof_used = 0;
   unsigned long long* maxs = nullptr;  // device max slots
   bool region_clear[2] = {false, false};  // max regions 0 / 1 hold zeros (cleared by s_read_max)
-  int part_lanes = PV4_LANES;  // lanes per node of block_GLOB's partition (F2M_PART_LANES: 4, or 1)
+  int part_lanes = PV4_LANES;  // lanes per node of the list partitions (F2M_PART_LANES: 4, or 1)
   double* part = nullptr;              // reduction partials
   double* sums = nullptr;              // final sums [3]
   int32_t* ibuf = nullptr;             // misc ints
@@ -1481,8 +1554,12 @@ struct Runner {
   }
   // check_OPT + update_EF (graph_cuda.cu:254-271, 229-252)
   f2m_status check_opt(double& M) {
-    hipLaunchKernelGGL(s_part_v2, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act), 1.0,
-                       I(w.nbr), w.D.as<double>(), I(w.orig), F());
+    if (part_lanes == PV4_LANES)
+      hipLaunchKernelGGL(s_part_v2_q, dim3(grid_for((int64_t)n * PV4_LANES, BS)), dim3(BS), 0, c->stream, n, rp,
+                         I(w.deg), I(w.act), 1.0, I(w.nbr), w.D.as<double>(), I(w.orig), F());
+    else
+      hipLaunchKernelGGL(s_part_v2, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act), 1.0,
+                         I(w.nbr), w.D.as<double>(), I(w.orig), F());
     F2M_LAUNCH_CHECK(c);
     f2m_status s = ef_rebuild();
     if (s) return s;
