@@ -134,34 +134,38 @@ __device__ __forceinline__ void x_discharge_r(const XState& X, int w, int b, int
 // missing or asymmetric mirror (then mir[s] = s).  Rows sorted by (cost, id)
 // are searched by bisection; a row out of that order falls back to a scan.
 template <class RP>
+__device__ __forceinline__ bool x_mirror_slot(int v, int s, RP rp, const int32_t* __restrict__ nbr,
+                                              const double* __restrict__ c, int32_t* __restrict__ mir,
+                                              bool& nonint) {
+  const int w = nbr[s];
+  const double cs = c[s];
+  const int qb = (int)rp(w), qe = (int)rp(w + 1);
+  int lo = qb, hi = qe;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (c[mid] < cs || (c[mid] == cs && nbr[mid] < v)) lo = mid + 1;
+    else hi = mid;
+  }
+  int q = lo;
+  if (!(q < qe && nbr[q] == v)) {
+    q = qb;
+    while (q < qe && nbr[q] != v) ++q;
+  }
+  if (cs != floor(cs)) nonint = true;
+  if (q == qe || c[q] != cs) {
+    mir[s] = s;
+    return false;
+  }
+  mir[s] = q;
+  return true;
+}
+template <class RP>
 __device__ __forceinline__ bool x_mirror_node(int v, RP rp, const int32_t* __restrict__ nbr,
                                               const double* __restrict__ c, int32_t* __restrict__ mir,
                                               bool& nonint) {
   bool ok = true;
   const int b = (int)rp(v), e = (int)rp(v + 1);
-  for (int s = b; s < e; ++s) {
-    const int w = nbr[s];
-    const double cs = c[s];
-    const int qb = (int)rp(w), qe = (int)rp(w + 1);
-    int lo = qb, hi = qe;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (c[mid] < cs || (c[mid] == cs && nbr[mid] < v)) lo = mid + 1;
-      else hi = mid;
-    }
-    int q = lo;
-    if (!(q < qe && nbr[q] == v)) {
-      q = qb;
-      while (q < qe && nbr[q] != v) ++q;
-    }
-    if (q == qe || c[q] != cs) {
-      ok = false;
-      mir[s] = s;
-    } else {
-      mir[s] = q;
-    }
-    if (cs != floor(cs)) nonint = true;
-  }
+  for (int s = b; s < e; ++s) ok = x_mirror_slot(v, s, rp, nbr, c, mir, nonint) && ok;
   return ok;
 }
 
