@@ -98,6 +98,37 @@ __global__ void x_flow(XDev X, double eps) {
   if (v < X.n) x_flow_node<SAT, SAT>(X.S, v, X.rp[v], X.rp[v + 1], eps);
 }
 
+// The later phases' x_flow<true> with XF_LANES lanes per node: lane l flips
+// the arcs of slots l, l + 8, ... (each arc's test and flip are its own; eR
+// moves by atomics as there) and the lanes' changes of eL[v], integers, are
+// summed with shuffles before the first lane applies them.
+constexpr int XF_LANES = 8;
+__global__ void x_flow_sat_lanes(XDev X, double eps) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
+  if (v >= X.n) return;  // the lanes of a node leave together
+  const XState& S = X.S;
+  const int b = X.rp[v], e = X.rp[v + 1];
+  const double pv = S.pL[v];
+  int dl = 0;
+  for (int s = b + l; s < e; s += XF_LANES) {
+    const double r = (S.c[s] + pv) - S.pR[S.nbr[s]];
+    const uint8_t fs = S.f[s];
+    if (!fs && r < -eps) {
+      S.f[s] = 1;
+      --dl;
+      atomicAdd(&S.eR[S.nbr[s]], 1);
+    } else if (fs && r > eps) {
+      S.f[s] = 0;
+      ++dl;
+      atomicAdd(&S.eR[S.nbr[s]], -1);
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < XF_LANES; o <<= 1) dl += __shfl_xor(dl, o, XF_LANES);
+  if (l == 0 && dl) S.eL[v] += dl;
+}
+
 __global__ void x_zero_ring(int32_t* cnt) {
   if (threadIdx.x < 2 * RING) cnt[threadIdx.x] = 0;
 }
@@ -369,7 +400,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     return F2M_OK;
   };
   for (;;) {
-    if (phases > 0) hipLaunchKernelGGL((x_flow<true>), gn, bs, 0, st, X, eps);
+    if (phases > 0)
+      hipLaunchKernelGGL(x_flow_sat_lanes, dim3(grid_for((int64_t)n * XF_LANES, bs.x)), bs, 0, st, X, eps);
     hipLaunchKernelGGL(x_zero_ring, dim3(1), dim3(64), 0, st, X.cnt);
     if (phases == 0) hipLaunchKernelGGL((x_excess<true>), gn, bs, 0, st, X, r);
     else hipLaunchKernelGGL((x_excess<false>), gn, bs, 0, st, X, r);
