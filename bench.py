@@ -17,8 +17,8 @@ The timed solves run without instrumentation.
 
 roofline: the dominant kernel is the multi-sweep launch over spatial tiles
 (s_sweep_tile, F2M_FUSE sweeps per launch, default 7).  Its algorithmic
-bytes per launch are, per sweep it runs, 12 B per active slot (D double +
-nbr int32) + 20 B per node (list offset, F in, F out), counted on the device
+bytes per launch are, per sweep it runs, 8 B per active slot (nbr int32 +
+a 4-byte cost) + 20 B per node (list offset, F in, F out), counted on the device
 for the timed launches; the average kernel duration comes from the HIP
 events the library records at the start and end of every 16th such launch
 (hipExtLaunchKernelGGL, on its stream) in two further, instrumented solves.
@@ -169,23 +169,33 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        host = host_cpu()
+        # the job's CPU share: the cores this process may run on, at most 16
+        # (the share of one GPU on a shared host); then one core
+        share = min(host["affinity_cpus"], 16)
         rp, nb, c = ctx.graph_csr(g)
-        log("cpu baseline: oracle solve with", threads, "threads")
-        t = time.perf_counter()
-        r = O.solve(rp, nb, c, threads=threads)
-        cpu_ms = (time.perf_counter() - t) * 1e3
-        ok = abs(r["stats"]["objective"] - st["objective"]) <= 1e-9 * abs(st["objective"])
-        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+        runs = {}
+        for threads in sorted({share, 1}, reverse=True):
+            log("cpu baseline: oracle solve with", threads, "threads")
+            t = time.perf_counter()
+            r = O.solve(rp, nb, c, threads=threads)
+            runs[threads] = ((time.perf_counter() - t) * 1e3, r)
+        best = min(runs, key=lambda k: runs[k][0])
+        cpu_ms, r = runs[best]
+        ok = all(abs(rr["stats"]["objective"] - st["objective"]) <= 1e-9 * abs(st["objective"])
+                 for _, rr in runs.values())
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": best, "kind": "port",
                "sample": f"one full solve of the same n={n} K={K} graph, nodes in Morton order "
                          f"({r['stats']['sweeps']} sweeps), objective agrees: {ok}",
-               "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1)}
+               "speedup_gpu_vs_cpu": round(cpu_ms / ms, 1),
+               "ms_by_threads": {str(k): round(v[0], 1) for k, v in runs.items()},
+               "speedup_vs_1_thread": round(runs[1][0] / ms, 1), **host}
         if input_order is not None:
             # the same solve with nodes in input order (the reference numbers
             # nodes arbitrarily): the CPU loses its cache locality too
-            rp0, nb0, c0 = O.graph_build(x0, y0, K, mode, threads=threads)
+            rp0, nb0, c0 = O.graph_build(x0, y0, K, mode, threads=share)
             t = time.perf_counter()
-            r0 = O.solve(rp0, nb0, c0, threads=threads)
+            r0 = O.solve(rp0, nb0, c0, threads=share)
             cpu0 = (time.perf_counter() - t) * 1e3
             cpu["input_order_ms"] = round(cpu0, 1)
             cpu["input_order_speedup"] = round(cpu0 / input_order["ms"], 1)
@@ -209,6 +219,21 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def host_cpu():
+    """nproc, the cores this process may use, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model}
 
 
 def time_input_order(ctx, x, y, K, mode, args):

@@ -12,9 +12,10 @@ Backward runs the hand-written kernel f2m_bound_backward:
 (envelope theorem; valid where x* is unique and the kNN graph is locally
 constant), scaled by the upstream gradient of the instance.
 
-The solver context's batch buffers are reused by the next forward on the
-same context: call backward before the next forward, or use one context per
-pending graph.
+Each forward keeps its own copy of the graph (node offsets, row pointers
+and neighbour ids, on the device) for its backward: the solver context's
+batch buffers are overwritten by the next forward on the same context, so
+several forwards may be pending before their backwards, in any order.
 
 warm (optional, a WarmStart): keeps each forward's potentials and starts the
 next forward of the same instances from them (f2m_solve_batched_warm), which
@@ -25,6 +26,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import _lib as L
 from .solver import COST_RAW_FP32, Context
 
 
@@ -53,19 +55,28 @@ class F2MBound(torch.autograd.Function):
         res = solver.batch_result(b, F=Fd, x=xs)
         if warm is not None:
             warm.F = Fd
-        ctx.save_for_backward(px.detach().contiguous(), py.detach().contiguous(), xs)
+        # this forward's graph, copied out of the context's reusable buffers
+        dev = px.device
+        g_off = torch.empty(b.n_inst + 1, dtype=torch.int64, device=dev)
+        g_rp = torch.empty(b.n_total + 1, dtype=torch.int64, device=dev)
+        g_nbr = torch.empty(b.nslots, dtype=torch.int32, device=dev)
+        solver.batch_csr(b, out=(g_off, g_rp, g_nbr, None))
+        ctx.save_for_backward(px.detach().contiguous(), py.detach().contiguous(), xs, g_off, g_rp, g_nbr)
         ctx.solver = solver
-        ctx.batch = b
+        ctx.shape = (b.n_inst, b.k, b.cost_mode, b.n_total, b.nslots)
         return torch.from_numpy(res["objective"]).to(px.device)
 
     @staticmethod
     def backward(ctx, gout):
-        px, py, xs = ctx.saved_tensors
+        px, py, xs, g_off, g_rp, g_nbr = ctx.saved_tensors
+        n_inst, k, mode, n_total, nslots = ctx.shape
+        graph = L.Batch(n_inst, k, mode, 0, n_total, nslots, g_off.data_ptr(), g_rp.data_ptr(), g_nbr.data_ptr(),
+                        None)
         scale = gout.detach().to(torch.float32).contiguous()
         gx = torch.empty_like(px)
         gy = torch.empty_like(py)
         torch.cuda.current_stream().synchronize()
-        ctx.solver.bound_backward(ctx.batch, xs, px, py, scale, gx, gy)
+        ctx.solver.bound_backward(graph, xs, px, py, scale, gx, gy)
         ctx.solver.synchronize()
         return gx, gy, None, None, None, None, None
 

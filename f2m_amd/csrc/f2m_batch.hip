@@ -1491,6 +1491,13 @@ f2m_status f2m_batch_result_copy(f2m_ctx* c, double* F, double* x, double* objec
   return F2M_OK;
 }
 
+f2m_status f2m_batch_comb_root_copy(f2m_ctx* c, int32_t* comb_root) {
+  if (!c || c->b_n_inst <= 0 || !comb_root) return F2M_ERR_ARG;
+  F2M_HIP(c, hipSetDevice(c->device));
+  F2M_HIP(c, copy_out(c, comb_root, c->bws.root.p, c->b_n_total * 4ull));
+  return F2M_OK;
+}
+
 // Autotuner: picks lanes per node for each instance-size class by timing
 // every candidate on the first `sample` instances of the batch (each
 // workgroup records its instance's wall_clock64 ticks) and keeping, per

@@ -161,8 +161,13 @@ static hipError_t scan_i32_impl(f2m_ctx* c, const int32_t* in, Tout* out, int64_
 // it meets a published prefix, and publishes its own prefix.  A status word
 // is (epoch << 34) | (flag << 32) | value, flag 1 = aggregate, 2 = prefix;
 // a word of an earlier call carries another epoch and reads as not ready.
-// Tiles are dispatched in index order and a tile waits only on lower ones,
-// all of which publish, so every wait ends.
+// A workgroup takes its tile index from a ticket counter (word 0 of the
+// status array; the statuses follow it), not from blockIdx.x: tickets are
+// handed out in the order workgroups start running, so every tile a tile
+// waits on belongs to a workgroup that already runs and will publish, and
+// every wait ends whatever order the dispatcher uses.  The workgroup that
+// draws the last ticket resets the counter to 0 for the next launch (all
+// other tickets were drawn before it).
 constexpr int LB_T = 256, LB_V = 8, LB_TILE = LB_T * LB_V;
 constexpr unsigned LB_EPOCH_MAX = (1u << 30) - 1;
 
@@ -183,7 +188,16 @@ __global__ void __launch_bounds__(LB_T) k_scan_lb(const int32_t* __restrict__ in
                                                   int32_t* __restrict__ set_ptr, int32_t set_val) {
   __shared__ int32_t wsum[LB_T / 64];
   __shared__ int32_t s_pre;
-  const int b = blockIdx.x, tid = threadIdx.x;
+  __shared__ int32_t s_tile;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    const unsigned long long t = atomicAdd(st, 1ull);
+    if (t == (unsigned long long)gridDim.x - 1ull) atomicExch(st, 0ull);
+    s_tile = (int32_t)t;
+  }
+  __syncthreads();
+  const int b = s_tile;
+  st += 1;  // the tile statuses
   const int64_t base = (int64_t)b * LB_TILE + (int64_t)tid * LB_V;
   int32_t v[LB_V];
   int32_t s = 0;
@@ -263,7 +277,7 @@ hipError_t scan_excess_i32(f2m_ctx* c, const int32_t* in, int32_t sub, int32_t* 
   const int64_t tiles = std::max<int64_t>(1, (n + LB_TILE - 1) / LB_TILE);
   if (tiles > INT32_MAX / 2) return hipErrorInvalidValue;
   const void* old = c->k_lbst.p;
-  hipError_t e = c->k_lbst.reserve((size_t)tiles * 8);
+  hipError_t e = c->k_lbst.reserve((size_t)(tiles + 1) * 8);  // ticket counter + statuses
   if (e != hipSuccess) return e;
   if (c->k_lbst.p != old || c->k_lb_epoch >= LB_EPOCH_MAX) {
     // fresh memory, or the epochs wrapped: no word may carry a live epoch

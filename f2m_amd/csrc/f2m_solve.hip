@@ -35,6 +35,11 @@ namespace {
 
 constexpr int BS = 256;
 constexpr int RED_BLOCKS = 512;
+// Algorithmic bytes of one sweep (instrumentation): per active slot the
+// neighbour id and the cost (4 + 4 B), per node the list offset, F in and
+// F out (4 + 8 + 8 B): B_sweep = 8 * (active slots) + 20 * n.
+constexpr double kAlgSlotBytes = 8.0;
+constexpr double kAlgNodeBytes = 20.0;
 constexpr unsigned long long INF_BITS = 0x7FF0000000000000ull;
 
 __global__ void s_zero_u64(unsigned long long* p, int64_t n, unsigned long long v) {
@@ -1326,7 +1331,7 @@ struct Runner {
       sslots = hp[1];
     }
     const int64_t active_slots = (int64_t)slots;
-    const double alg_bytes = 12.0 * (double)slots + 20.0 * (double)n * (double)sweeps;
+    const double alg_bytes = kAlgSlotBytes * (double)slots + kAlgNodeBytes * (double)n * (double)sweeps;
     double ms = 0;
     for (size_t i = 0; i < prof_used; ++i) {
       float t = 0;
@@ -1336,7 +1341,7 @@ struct Runner {
     c->prof.sweep_launches = launches;
     c->prof.sweeps = sweeps;
     c->prof.sampled_sweeps = sampled_sweeps;
-    c->prof.sampled_alg_bytes = 12.0 * (double)sslots + 20.0 * (double)n * (double)sampled_sweeps;
+    c->prof.sampled_alg_bytes = kAlgSlotBytes * (double)sslots + kAlgNodeBytes * (double)n * (double)sampled_sweeps;
     c->prof.sweep_samples = (int64_t)prof_used;
     c->prof.sweep_ms_sampled = ms;
     c->prof.sweep_alg_bytes = alg_bytes;

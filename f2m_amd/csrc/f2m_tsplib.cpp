@@ -27,11 +27,17 @@ struct f2m_points {
 };
 
 namespace {
+// Hash of a rounded point.  -0.0 == +0.0 under the map's equality (and
+// std::hash<double> gives both the same value, as the reference's
+// unordered_map<double, ...> relies on, FXY.hpp:259-267), so a zero is
+// hashed as +0.0: a point rounding to -0.0 then meets an earlier +0.0.
 struct PairHash {
   size_t operator()(const std::pair<double, double>& p) const {
+    const double x = p.first == 0.0 ? 0.0 : p.first;
+    const double y = p.second == 0.0 ? 0.0 : p.second;
     uint64_t a, b;
-    std::memcpy(&a, &p.first, 8);
-    std::memcpy(&b, &p.second, 8);
+    std::memcpy(&a, &x, 8);
+    std::memcpy(&b, &y, 8);
     return std::hash<uint64_t>()(a * 0x9E3779B97F4A7C15ull ^ (b + 0x7F4A7C15ull));
   }
 };

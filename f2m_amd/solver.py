@@ -251,8 +251,13 @@ class Context:
         self.batch_node_off = node_off
         return b
 
-    def batch_csr(self, b: L.Batch | None = None):
+    def batch_csr(self, b: L.Batch | None = None, out=None):
+        """node_off, rowptr, nbr, cost of a batch; `out` = four destination
+        arrays (numpy or CUDA tensors; None skips that array)."""
         b = b or self.batch
+        if out is not None:
+            self._check(lib.f2m_batch_copy(self.h, ctypes.byref(b), *(_ptr(a) for a in out)))
+            return out
         node_off = np.zeros(b.n_inst + 1, np.int64)
         rowptr = np.zeros(b.n_total + 1, np.int64)
         nbr = np.zeros(b.nslots, np.int32)
@@ -280,14 +285,20 @@ class Context:
         self._check(lib.f2m_batch_autotune(self.h, ctypes.byref(b), ctypes.byref(opts) if opts else None, sample,
                                            os.fsencode(cache_path) if cache_path else None))
 
-    def batch_result(self, b: L.Batch | None = None, F=None, x=None, objective=None):
-        """Copies results; pass torch tensors to keep them on the device."""
+    def batch_result(self, b: L.Batch | None = None, F=None, x=None, objective=None, comb_root=False):
+        """Copies results; pass torch tensors to keep them on the device.
+        comb_root=True adds the per-node tight-component roots (local ids)."""
         b = b or self.batch
         F = np.zeros(b.n_total) if F is None else F
         x = np.zeros(b.nslots) if x is None else x
         objective = np.zeros(b.n_inst) if objective is None else objective
         self._check(lib.f2m_batch_result_copy(self.h, _ptr(F), _ptr(x), _ptr(objective)))
-        return {"F": F, "x": x, "objective": objective}
+        out = {"F": F, "x": x, "objective": objective}
+        if comb_root:
+            root = np.zeros(b.n_total, np.int32)
+            self._check(lib.f2m_batch_comb_root_copy(self.h, _ptr(root)))
+            out["comb_root"] = root
+        return out
 
     def bound_backward(self, b, x_slot, px, py, grad_scale, gx, gy):
         """Device tensors; b=None uses the context's single graph."""

@@ -70,14 +70,17 @@ def c4_sizes(count: int = 4096, seed: int = SEED0 + 4):
     return np.floor(np.exp(rng.uniform(math.log(100), math.log(20000), size=count))).astype(np.int64)
 
 
-def c4(count: int = 4096, seed: int = SEED0 + 4):
-    """List of (x, y) per instance; cost mode NINT, K=16."""
+def c4(count: int = 4096, seed: int = SEED0 + 4, only=None):
+    """List of (x, y) per instance; cost mode NINT, K=16.  With `only` (instance
+    indices), just those instances, in that order: every instance's points are
+    still drawn in sequence, so each equals its copy in the full workload."""
     sizes = c4_sizes(count, seed)
     rng = np.random.default_rng(seed + 1000)
-    out = []
+    pts = []
     for n in sizes:
-        p = (rng.random((int(n), 2), dtype=np.float32) * np.float32(1e6)).astype(np.float32)
-        out.append(dedup(p[:, 0], p[:, 1]))
+        pts.append((rng.random((int(n), 2), dtype=np.float32) * np.float32(1e6)).astype(np.float32))
+    idx = range(count) if only is None else [int(i) for i in only]
+    out = [dedup(pts[i][:, 0], pts[i][:, 1]) for i in idx]
     return out, COST_NINT, 16
 
 

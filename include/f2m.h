@@ -167,8 +167,9 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
  * stream); f2m_get_profile reports the sampled kernel time, the sweeps and
  * algorithmic bytes of those launches, and the algorithmic bytes of all
  * sweeps:
- * 12 B per active slot (D double + nbr int32) + 20 B per node (list
- * offset, F in, F out). */
+ * 8 B per active slot (nbr int32 + a 4-byte cost) + 20 B per node (list
+ * offset, F in, F out).  The solver stores D as a double (12 B per slot
+ * moved); the count is the algorithm's, not the layout's. */
 typedef struct f2m_profile {
   int64_t sweep_launches;   /* sweep kernel launches of the last solve         */
   int64_t sweep_samples;    /* launches timed with events                      */
@@ -292,6 +293,12 @@ f2m_status f2m_batch_copy(f2m_ctx* ctx, const f2m_batch* b, int64_t* node_off, i
                           double* cost);
 /* x [dev|host] nslots per canonical slot, F [dev|host] n_total. */
 f2m_status f2m_batch_result_copy(f2m_ctx* ctx, double* F, double* x, double* objective);
+/* comb_root [dev|host] n_total: per node of the last f2m_solve_batched, the
+ * smallest instance-local node id of its tight component, or -1 (as
+ * f2m_result_copy's comb_root; with the exact stage: components of the
+ * x = 1/2 edges).  Replaces reading graph_c's comb_id / comb_V arrays
+ * (graph_cuda_comb.hpp:201-273) per instance. */
+f2m_status f2m_batch_comb_root_copy(f2m_ctx* ctx, int32_t* comb_root);
 
 /* ---- differentiable bound (new) -------------------------------------------
  * dV/dp for V = sum_e x_e |p_u - p_w| (envelope theorem: x* is fixed).

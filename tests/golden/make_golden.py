@@ -145,7 +145,7 @@ def main():
                 fo.write(fi.read())
 
 
-if __name__ == "__main__" and "--heavy" not in sys.argv and "--lattice" not in sys.argv:
+if __name__ == "__main__" and not {"--heavy", "--lattice", "--ara"} & set(sys.argv):
     main()
 
 
@@ -212,3 +212,69 @@ def lattice_optima(full: bool):
 
 if __name__ == "__main__" and "--lattice" in sys.argv:
     lattice_optima("--c3" in sys.argv)
+
+
+def known_answers_ara():
+    """`python tests/golden/make_golden.py --ara`: the published ara238025 and
+    sra104815 answers against this build's oracle, on the graphs the solver
+    builds from the same files with nodes in input order (K=20 and K=50):
+      * the oracle with the reference's recovery (least squares, repairs):
+        its objective, MAX edge rank and x range;
+      * the oracle with exact recovery: its certified objective;
+      * D_all, a weak-duality lower bound of the LP over ALL pairs of points
+        (tests/helpers.py: complete_graph_dual_bound) -- a published value
+        below it is not the objective of any feasible x on any edge set;
+      * HiGHS (interior point + crossover) on the K=20 edge set.
+    Also writes tests/golden/ara238025.tsp.gz (the input, as data)."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from oracle import tsplib
+    from tests.helpers import complete_graph_dual_bound
+    path = os.path.join(HERE, "golden.json")
+    g = json.load(open(path))
+    src_dir = "/root/reference/examples/vsli"
+    with open(os.path.join(src_dir, "ara238025.tsp"), "rb") as fi, \
+            gzip.open(os.path.join(HERE, "ara238025.tsp.gz"), "wb", compresslevel=9) as fo:
+        fo.write(fi.read())
+    g["known"] = [k for k in g["known"] if k["instance"] not in ("ara238025",)]
+    for inst, Ks in (("ara238025", (20, 50)), ("sra104815", (50,))):
+        lab, x, y, _ = tsplib.read(os.path.join(src_dir, inst + ".tsp"))
+        for K in Ks:
+            rep = open(os.path.join(src_dir, f"{inst}.tsp.top{K}.F2M_report")).read().splitlines()
+            ent = [k for k in g["known"] if k["instance"] == inst and k["K"] == K]
+            e = ent[0] if ent else {"instance": inst, "K": K, "mode": 0,
+                                    "source": f"examples/vsli/{inst}.tsp.top{K}.F2M_report:2-4"}
+            e["reference_opt"] = float(rep[1].split(": ")[1])
+            e["reference_max_rank"] = int(rep[3].split(": ")[1])
+            rp, nb, c = O.graph_build(x, y, K, 0)
+            r = O.solve(rp, nb, c)
+            r0 = O.solve(rp, nb, c, O.default_opts(exact=0))
+            Dall, nviol = complete_graph_dual_bound(x, y, rp, nb, c, r["F"], r["stats"]["d_min"], K, 0)
+            e.update({"node_order": "input", "nslots": int(len(nb)), "graph_sha256": graph_hash(rp, nb, c),
+                      "oracle_ref_recovery_opt": r0["stats"]["objective"],
+                      "oracle_ref_recovery_max_rank": r0["stats"]["max_rank"],
+                      "oracle_ref_recovery_repairs": r0["stats"]["n_repairs"],
+                      "oracle_ref_recovery_x_range": [float(r0["x"].min()), float(r0["x"].max())],
+                      "oracle_exact_opt": r["stats"]["objective"], "oracle_exact_gap": r["stats"]["exact_gap"],
+                      "oracle_exact_max_rank": r["stats"]["max_rank"],
+                      "dual_bound_all_pairs": Dall, "dual_bound_endpoints_checked": nviol})
+            if inst == "ara238025" and K == 20:
+                n = len(rp) - 1
+                s_ = np.repeat(np.arange(n), np.diff(rp))
+                m = s_ < nb
+                u, w, cc = s_[m], nb[m], c[m]
+                E = len(u)
+                A = coo_matrix((np.ones(2 * E), (np.concatenate([u, w]), np.concatenate([np.arange(E)] * 2))),
+                               shape=(n, E)).tocsr()
+                lp = linprog(cc, A_eq=A, b_eq=2 * np.ones(n), bounds=(0, 1), method="highs-ipm")
+                assert lp.status == 0, lp.message
+                e["lp_opt_same_edges"] = float(lp.fun)
+                e["lp_method"] = "scipy.optimize.linprog(method='highs-ipm')"
+            if not ent:
+                g["known"].append(e)
+            print(inst, K, {k: e[k] for k in e if k not in ("graph_sha256",)}, flush=True)
+            json.dump(g, open(path, "w"), indent=1)
+
+
+if __name__ == "__main__" and "--ara" in sys.argv:
+    known_answers_ara()

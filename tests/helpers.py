@@ -80,3 +80,40 @@ def morton_order(x, y):
 
     key = spread(quant(x)) | (spread(quant(y)) << np.uint64(1))
     return np.argsort(key, kind="stable").astype(np.int32)
+
+
+def complete_graph_dual_bound(x, y, rowptr, nbr, cost, F, d_min, K, mode):
+    """Lower bound on the F2M LP optimum over ALL pairs of points (weak
+    duality with y = -d_min F / 2, as certificate()):
+        D_all = 2 sum_v y_v - sum_{u<w} max(0, y_u + y_w - c_uw).
+    A pair outside the symmetrised kNN graph costs at least max(r_u, r_w)
+    (r_v = v's K-th nearest cost), so its term is nonzero only when an
+    endpoint has 2 y_v > r_v; the rows of those endpoints are computed in
+    full against every point.  Returns (D_all, number of such endpoints).
+    Any x feasible on any edge set has sum c x >= D_all."""
+    n = len(rowptr) - 1
+    _, D = certificate(rowptr, nbr, cost, np.zeros(len(nbr)), F, d_min)
+    yv = -0.5 * d_min * np.asarray(F, np.float64)
+    r = cost[np.asarray(rowptr[:-1], np.int64) + K - 1]
+    viol = np.nonzero(2 * yv > r)[0]
+    vset = set(viol.tolist())
+    xs = np.asarray(x, np.float64)
+    ys = np.asarray(y, np.float64)
+    extra = 0.0
+    for v in viol:
+        dx = xs[v] - xs
+        dy = ys[v] - ys
+        c = np.sqrt(dx * dx + dy * dy)
+        if mode == 1:
+            c = np.floor(c + 0.5)
+        elif mode == 2:
+            c = c.astype(np.float32).astype(np.float64)
+        c[v] = np.inf
+        pen = yv[v] + yv - c
+        inside = set(nbr[rowptr[v]:rowptr[v + 1]].tolist())
+        for w in np.nonzero(pen > 0)[0]:
+            w = int(w)
+            if w in inside or (w in vset and w < v):  # graph edge, or counted from w
+                continue
+            extra += float(pen[w])
+    return D - extra, len(viol)

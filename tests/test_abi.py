@@ -72,6 +72,14 @@ def test_tsplib_reader_matches_restatement():
         pts.write_duplicates(os.path.join(tmp, "t.tsp_duplicates"))
         dup = open(os.path.join(tmp, "t.tsp_duplicates")).read()
         assert dup == "node used: 1 duplicates: 3 \nnode used: 2 duplicates: 5 \n"
+        # a coordinate rounding to -0.0 equals an earlier +0.0 (and the other way
+        # round): the point is a duplicate, as in the reference's hash map
+        text = "NODE_COORD_SECTION\n1 0.0 5\n2 -0.0004 5\n3 7 -0.0002\n4 7 0\n5 1 1\nEOF\n"
+        p = _write(tmp, "z.tsp", text)
+        pts = read_tsplib(p)
+        lab, x, y, d = tsplib.read(p)
+        assert pts.labels.tolist() == lab.tolist() == [1, 3, 5]
+        assert pts.n_duplicates == 2 and d == {1: [2], 3: [4]}
 
 
 def test_tsplib_reader_errors():
