@@ -85,6 +85,7 @@ EXPORTS = [
     "f2m_graph_copy", "f2m_solve", "f2m_result_copy", "f2m_sweep", "f2m_knn_build_batched", "f2m_solve_batched", "f2m_batch_autotune", "f2m_batch_copy",
     "f2m_batch_result_copy", "f2m_bound_backward", "f2m_spatial_order", "f2m_spatial_order_batched",
     "f2m_solve_warm", "f2m_knn_certify", "f2m_solve_batched_warm", "f2m_batch_comb_root_copy",
+    "f2m_sweep_noalloc",
 ]
 
 _sig = {
@@ -122,6 +123,7 @@ _sig = {
     "f2m_knn_certify": (c_i32, [c_p, ctypes.POINTER(Graph), c_p]),
     "f2m_solve_batched_warm": (c_i32, [c_p, ctypes.POINTER(Batch), ctypes.POINTER(Opts), c_p, c_p]),
     "f2m_batch_comb_root_copy": (c_i32, [c_p, c_p]),
+    "f2m_sweep_noalloc": (c_i32, [c_p, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i32, c_p]),
 }
 
 for _name in EXPORTS:

@@ -11,6 +11,10 @@ f2m_status fail(f2m_ctx* c, f2m_status s, const std::string& msg) {
   if (c) c->err = msg;
   return s;
 }
+f2m_status fail(f2m_ctx* c, f2m_status s, const char* msg) {
+  if (c) c->err.assign(msg);
+  return s;
+}
 
 f2m_status hip_fail(f2m_ctx* c, hipError_t e, const char* what, const char* file, int line) {
   char buf[512];
@@ -360,8 +364,10 @@ f2m_status f2m_ctx_create(int32_t device, void* stream, f2m_ctx** out) {
   if (e != hipSuccess || ndev <= 0) return F2M_ERR_HIP;
   if (device < 0 || device >= ndev) return F2M_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return F2M_ERR_HIP;
+  if (f2m::noalloc_init(device) != hipSuccess) return F2M_ERR_HIP;
   f2m_ctx* c = new f2m_ctx;
   c->device = device;
+  c->err.reserve(1024);
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {

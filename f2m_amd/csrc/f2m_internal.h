@@ -104,6 +104,12 @@ struct f2m_ctx {
 namespace f2m {
 
 f2m_status fail(f2m_ctx* c, f2m_status s, const std::string& msg);
+// Same, assigning into the context's message buffer (reserved at create), so
+// a message shorter than that buffer allocates nothing.
+f2m_status fail(f2m_ctx* c, f2m_status s, const char* msg);
+// Looks up this device's address of the allocation-free entry point's scratch
+// array (f2m_noalloc.hip); call with `device` current.  Allocates nothing.
+hipError_t noalloc_init(int device);
 f2m_status hip_fail(f2m_ctx* c, hipError_t e, const char* what, const char* file, int line);
 
 // Device pointer for p: p itself if device-accessible, else a staged copy in

@@ -263,6 +263,23 @@ f2m_status f2m_sweep(f2m_ctx* ctx, int32_t n, const int32_t* rowptr, const int32
                      const double* D, const int32_t* len, const double* F_in, double* F_out,
                      double* g_out, int32_t nsweeps, double* max_abs_g);
 
+/* ---- allocation-free sweeps (new; no counterpart in the reference) -----------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc.  The same sweeps
+ * and results as f2m_sweep, bit for bit, with every array a [dev] array of the
+ * caller (host arrays are rejected, F2M_ERR_ARG) and F_in != F_out; g_out may
+ * be NULL; max_abs_g [dev|host] may be NULL (a device pointer keeps the call
+ * free of host synchronisation).  No hipMalloc, hipFree or other allocation:
+ * the intermediate potentials and the max reduction use a fixed device array
+ * of the library (8 MiB + 4 KiB per device, held until the process ends).
+ * n above F2M_NOALLOC_MAX_N fails with F2M_ERR_LIMIT.  All calls on one device
+ * share that array: calls that overlap on a device (two streams, two threads)
+ * race, so serialise them. */
+#define F2M_NOALLOC_MAX_N 1048576
+f2m_status f2m_sweep_noalloc(f2m_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* nbr,
+                             const double* D, const int32_t* len, const double* F_in, double* F_out,
+                             double* g_out, int32_t nsweeps, double* max_abs_g);
+
 /* ---- batched form (new) ------------------------------------------------------
  * node_off [host] n_inst+1 offsets into x, y [dev|host]; one kNN graph per
  * instance, all with the same k and cost mode. */
