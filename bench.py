@@ -54,6 +54,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "F2M solve ms @n=100k,k=20 (plain vs batched); fwd+bwd ms; % HBM peak"
+# schedule table of the 4096-instance batch, written by `bench.py --tune-batch`
+TUNED_C4 = os.path.join(ROOT, "f2m_amd", "tuned", "batch_n100-20000_k16.txt")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -76,6 +78,10 @@ def main():
     ap.add_argument("--batch-order", choices=["morton", "input"], default="morton")
     ap.add_argument("--only", choices=["batched", "fwd_bwd"], default=None,
                     help="run only this part and print its JSON (for profiling runs)")
+    ap.add_argument("--batch-table", default=TUNED_C4,
+                    help="schedule table of the batched leg (f2m_batch_autotune cache file; 'none': built-in)")
+    ap.add_argument("--tune-batch", default=None, metavar="PATH",
+                    help="choose the batched leg's schedule table (tune_batch_table) and write it to PATH")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -98,7 +104,8 @@ def main():
     if args.only:
         if args.only == "batched":
             c = Context(0)
-            print(json.dumps({"batched": bench_batched(c, args.batch_order)}), flush=True)
+            print(json.dumps({"batched": bench_batched(c, args.batch_order, args.batch_table, args.tune_batch)}),
+                  flush=True)
             c.close()
         else:
             print(json.dumps({"fwd_bwd": bench_fwd_bwd()}), flush=True)
@@ -202,7 +209,7 @@ def main():
             cpu["input_order_objective_agrees"] = bool(
                 abs(r0["stats"]["objective"] - input_order["objective"]) <= 1e-9 * abs(input_order["objective"]))
 
-    batched = None if args.no_batched else bench_batched(ctx, args.batch_order)
+    batched = None if args.no_batched else bench_batched(ctx, args.batch_order, args.batch_table, args.tune_batch)
     fwd_bwd = None if args.no_fwd_bwd else bench_fwd_bwd()
 
     out = {
@@ -251,7 +258,7 @@ def time_input_order(ctx, x, y, K, mode, args):
     return {"ms": round(ms, 3), "objective": st["objective"], "sweeps": int(st["sweeps"])}
 
 
-def bench_batched(ctx, order):
+def bench_batched(ctx, order, table=None, tune_to=None):
     import numpy as np
 
     from f2m_amd import workloads as W
@@ -266,6 +273,14 @@ def bench_batched(ctx, order):
     t = time.perf_counter()
     b = ctx.knn_build_batched(xs, ys, K, mode)
     knn_s = time.perf_counter() - t
+    tuned = None
+    if tune_to:
+        tune_batch_table(ctx, b, tune_to)
+        table = tune_to
+    if table and table != "none" and os.path.exists(table):
+        ctx.batch_autotune(b, cache_path=table)  # loads the table
+        tuned = {"file": os.path.relpath(table, ROOT),
+                 "classes": [ln.split() for ln in open(table).read().splitlines() if ln.strip()]}
     t = time.perf_counter()
     st = ctx.solve_batched(b)  # first call captures the hipGraph
     first_s = time.perf_counter() - t
@@ -292,7 +307,59 @@ def bench_batched(ctx, order):
             "plain_loop_per_instance_ms": round(plain_ms, 3),
             "speedup_per_instance": round(plain_ms / per_inst_ms, 2), "identical_optima_sampled": bool(same),
             "all_optimal_found": all(x["status"] == 0 for x in st), "knn_build_s": round(knn_s, 3),
-            "sweeps_max": int(max(x["sweeps"] for x in st))}
+            "sweeps_max": int(max(x["sweeps"] for x in st)),
+            "schedule_table": tuned or "built-in (lanes 2/4, 256/512/1024 threads from 1000/2000 nodes)"}
+
+
+def tune_batch_table(ctx, b, path):
+    """Schedule table of the batched leg, written to `path`: per instance-size
+    class (n <= 500, 1000, 2000, 4000, 8000, 16000, larger) the lanes per node,
+    the workgroup threads (over the classes: the bucket thresholds) and, for
+    the classes above 8000 nodes, the shared-memory placement.  Scored by the
+    makespan of the whole captured launch of the batch (the better of two
+    timed replays after the capture call).  One pass of coordinate descent
+    from the built-in schedule, classes from the largest down, a change kept
+    when it is more than 0.5% faster."""
+    B = [500, 1000, 2000, 4000, 8000, 16000, 2**63 - 1]
+    # the built-in schedule (lanes 2 below 1000 nodes, else 4; 256 threads
+    # below 1000, 512 below 2000, else 1024; default placement)
+    tab = [[500, 2, 256, -1], [1000, 2, 256, -1], [2000, 4, 512, -1], [4000, 4, 1024, -1], [8000, 4, 1024, -1],
+           [16000, 4, 1024, -1], [B[-1], 4, 1024, -1]]
+
+    def write(t):
+        with open(path, "w") as f:
+            f.write("".join(" ".join(str(v) for v in r) + "\n" for r in t))
+
+    def makespan(t):
+        write(t)
+        ctx.batch_autotune(b, cache_path=path)  # loads the table
+        ctx.solve_batched(b)  # captures the graph
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            ctx.solve_batched(b)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    best = makespan(tab)
+    log("batched: built-in table", round(best, 4), "s")
+    cands = {B[6]: [(2, 1024, -1), (8, 1024, -1), (4, 1024, 2)], B[5]: [(2, 1024, -1), (8, 1024, -1), (4, 1024, 2)],
+             B[4]: [(2, 1024, -1), (8, 1024, -1)], B[3]: [(2, 1024, -1), (8, 1024, -1)],
+             B[2]: [(4, 1024, -1), (8, 512, -1), (2, 512, -1)], B[1]: [(4, 256, -1), (2, 512, -1)],
+             B[0]: [(1, 256, -1), (4, 256, -1)]}
+    for k in range(6, -1, -1):
+        for l, bs, lds in cands[B[k]]:
+            t = [list(r) for r in tab]
+            t[k][1:] = [l, bs, lds if lds != -1 else t[k][3]]
+            if t == tab:
+                continue
+            ms = makespan(t)
+            log("batched: class", B[k], "lanes", l, "threads", bs, "placement", t[k][3], round(ms, 4), "s")
+            if ms < best * 0.995:
+                tab, best = t, ms
+    write(tab)
+    log("batched: chosen", tab, round(best, 4), "s ->", path)
 
 
 def bench_fwd_bwd():

@@ -1011,7 +1011,8 @@ namespace {
 
 struct BatchPlan {
   std::vector<int32_t> order;  // all instances, bucketed, decreasing size
-  std::vector<int> bucket_start, bucket_lpn, bucket_bs, bucket_grid;
+  std::vector<int> bucket_start, bucket_lpn, bucket_bs, bucket_lds, bucket_grid;
+  static constexpr int MAXBUCKETS = 16;  // queue counters (b_zero_i32 clears 16)
   std::vector<size_t> bucket_dyn;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -1054,17 +1055,25 @@ static void destroy_plan(BatchPlan* P) {
 // overlapped, larger workgroups for mid-size instances finish them sooner and
 // pack better: the 4096-instance batch took 1.02 s at 4000 / 10000 nodes,
 // 0.85 s at 2000 / 5000, 0.81-0.82 s from 2000 / 3000 down to 500 / 2000.
-void batch_cfg_for(const f2m_ctx* c, int64_t n, int n_inst, int ncu, int& lpn, int& bs) {
+void batch_cfg_for(const f2m_ctx* c, int64_t n, int n_inst, int ncu, int& lpn, int& bs, int& lds) {
   lpn = n < 1000 ? 2 : 4;
   const int64_t n512 = env_int("F2M_BATCH_N512", 1000), n1024 = env_int("F2M_BATCH_N1024", 2000);
   bs = n < n512 ? 256 : (n < n1024 ? 512 : 1024);
   while (bs < 1024 && (int64_t)n_inst * bs * 2 <= (int64_t)ncu * 1024 && n >= 2 * bs) bs *= 2;
+  lds = 3;
   for (const auto& t : c->b_tune)
-    if (n <= t.max_n) { lpn = t.lpn; bs = t.bs; break; }
+    if (n <= t.max_n) {
+      lpn = t.lpn;
+      bs = t.bs;
+      if (t.lds >= 0) lds = t.lds;
+      break;
+    }
   if (const char* e = std::getenv("F2M_BATCH_LPN")) lpn = std::atoi(e);
   if (const char* e = std::getenv("F2M_BATCH_BS")) bs = std::atoi(e);
+  if (const char* e = std::getenv("F2M_BATCH_LDS")) lds = std::atoi(e);
   if (lpn != 1 && lpn != 2 && lpn != 8) lpn = 4;
   if (bs != 512 && bs != 1024) bs = 256;
+  if (lds < 0 || lds > 3) lds = 3;
 }
 
 template <int BS, int LPN>
@@ -1091,9 +1100,11 @@ const void* bucket_kernel_any(int bs, int lpn) {
 // the exact stage's 2*bs doubles; grid = co-resident workgroups, at most one
 // per instance.  By default the workgroup gets all the shared memory its
 // occupancy allows, so that the active lists can follow the potentials.
-// F2M_BATCH_LDS=0 keeps the potentials in global memory, =1 allows one shared
-// buffer only, =2 the potentials only (paths for tests and timing).
-hipError_t bucket_shape(int device, int bs, int lpn, int64_t nmax, int n_order, int ncu, size_t& dyn, int& grid) {
+// lds (F2M_BATCH_LDS, or the autotuned table): 0 keeps the potentials in
+// global memory, 1 allows one shared buffer only, 2 the potentials only, 3
+// (default) the potentials and then the lists.
+hipError_t bucket_shape(int device, int bs, int lpn, int lds, int64_t nmax, int n_order, int ncu, size_t& dyn,
+                        int& grid) {
   const void* k = bucket_kernel_any(bs, lpn);
   hipFuncAttributes fa;
   hipError_t e = hipFuncGetAttributes(&fa, k);
@@ -1109,14 +1120,18 @@ hipError_t bucket_shape(int device, int bs, int lpn, int64_t nmax, int n_order, 
   size_t room = (size_t)lds_cu / per_cu;
   room = room > fa.sharedSizeBytes + 64 ? (room - fa.sharedSizeBytes - 64) & ~(size_t)255 : 0;
   dyn = base;
-  const char* env = std::getenv("F2M_BATCH_LDS");
-  const int lds = env ? std::atoi(env) : 3;
   if (lds > 0) {
     const size_t two = (size_t)nmax * 16, one = (size_t)nmax * 8;
     if (lds >= 3) dyn = std::max(base, room);  // potentials, then the lists where they fit
     else dyn = std::max(base, (two <= room && lds >= 2) ? two : (one <= room ? one : room));
   }
-  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  // The limit is a property of the kernel, shared by every bucket that
+  // launches it (buckets of one kernel may differ in placement, so in dyn):
+  // set it to what the kernel can have at all, never to one bucket's dyn,
+  // which would leave a larger bucket of the same kernel above the limit.
+  const size_t lim = (size_t)lds_cu > fa.sharedSizeBytes ? (size_t)lds_cu - fa.sharedSizeBytes : 0;
+  if (dyn > lim) return hipErrorInvalidValue;
+  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
   if (e != hipSuccess) (void)hipGetLastError();  // the default limit may already allow it
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, dyn);
   if (e != hipSuccess) return e;
@@ -1331,17 +1346,22 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     P->bucket_start.clear();
     P->bucket_lpn.clear();
     P->bucket_bs.clear();
+    P->bucket_lds.clear();
     int prev = -1;
     for (int i = 0; i < n_inst; ++i) {
-      int l, bs;
-      batch_cfg_for(c, off[ids[i] + 1] - off[ids[i]], n_inst, ncu, l, bs);
-      if (l * 4096 + bs != prev) {
+      int l, bs, lds;
+      batch_cfg_for(c, off[ids[i] + 1] - off[ids[i]], n_inst, ncu, l, bs, lds);
+      const int kind = (l * 4096 + bs) * 4 + lds;
+      if (kind != prev) {
         P->bucket_start.push_back(i);
         P->bucket_lpn.push_back(l);
         P->bucket_bs.push_back(bs);
-        prev = l * 4096 + bs;
+        P->bucket_lds.push_back(lds);
+        prev = kind;
       }
     }
+    if (P->bucket_lpn.size() > (size_t)BatchPlan::MAXBUCKETS)
+      return fail(c, F2M_ERR_LIMIT, "the schedule table splits the batch into more than 16 buckets");
     P->bucket_start.push_back(n_inst);
     P->bucket_dyn.clear();
     P->bucket_grid.clear();
@@ -1349,8 +1369,9 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
       const int first = P->order[P->bucket_start[k]];  // the bucket's largest instance
       size_t dyn = 0;
       int grid = 0;
-      F2M_HIP(c, bucket_shape(c->device, P->bucket_bs[k], P->bucket_lpn[k], off[first + 1] - off[first],
-                              P->bucket_start[k + 1] - P->bucket_start[k], ncu, dyn, grid));
+      F2M_HIP(c, bucket_shape(c->device, P->bucket_bs[k], P->bucket_lpn[k], P->bucket_lds[k],
+                              off[first + 1] - off[first], P->bucket_start[k + 1] - P->bucket_start[k], ncu, dyn,
+                              grid));
       P->bucket_dyn.push_back(dyn);
       P->bucket_grid.push_back(grid);
     }
@@ -1428,7 +1449,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
       F2M_HIP(c, hipEventRecord(P->ev_fork, c->stream));
       for (int i = 0; i < nside; ++i) F2M_HIP(c, hipStreamWaitEvent(P->side[i], P->ev_fork, 0));
     }
-    for (int k = 0; k < nb && k < 16; ++k) {
+    for (int k = 0; k < nb; ++k) {
       BArgs ak = a;
       ak.order = c->b_order.as<int32_t>() + P->bucket_start[k];
       ak.n_order = P->bucket_start[k + 1] - P->bucket_start[k];
@@ -1504,7 +1525,9 @@ f2m_status f2m_batch_comb_root_copy(f2m_ctx* c, int32_t* comb_root) {
 // class, the candidate with the smallest summed time.  All candidates give
 // bit-identical results; only speed differs.  The table is kept on the
 // context and, when cache_path is given, written there and reused on later
-// calls (one line per class: "<max instance size> <lanes per node>").
+// calls (one line per class: "<max instance size> <lanes per node>
+// <threads> <placement>", placement -1 = the default; a table written by
+// hand or by bench.py --tune-batch may set it, as F2M_BATCH_LDS).
 f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o, int32_t sample,
                               const char* cache_path) {
   if (!c || !b || b->n_inst < 1) return F2M_ERR_ARG;
@@ -1518,8 +1541,9 @@ f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o,
     if (f) {
       std::vector<TE> t;
       long long mx;
-      int l, bs;
-      while (std::fscanf(f, "%lld %d %d", &mx, &l, &bs) == 3) t.push_back(TE{(int64_t)mx, (int32_t)l, (int32_t)bs});
+      int l, bs, lds;
+      while (std::fscanf(f, "%lld %d %d %d", &mx, &l, &bs, &lds) == 4)
+        t.push_back(TE{(int64_t)mx, (int32_t)l, (int32_t)bs, (int32_t)lds, 0});
       std::fclose(f);
       if (!t.empty()) {
         c->b_tune = t;
@@ -1534,14 +1558,14 @@ f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o,
   F2M_HIP(c, hipMemcpy(off.data(), b->node_off, (ns + 1) * 8ull, hipMemcpyDefault));
   double best_t[NC];
   TE best[NC];
-  for (int k = 0; k < NC; ++k) { best_t[k] = -1; best[k] = TE{bounds[k], 0, 0}; }
+  for (int k = 0; k < NC; ++k) { best_t[k] = -1; best[k] = TE{bounds[k], 0, 0, -1, 0}; }
   const auto saved = c->b_tune;
   c->b_clk_on = true;
   std::vector<long long> clk(ns);
   f2m_status st = F2M_OK;
   for (int bs : cand_bs)
     for (int l : cand_lpn) {
-      c->b_tune = {TE{INT64_MAX, l, bs}};
+      c->b_tune = {TE{INT64_MAX, l, bs, -1, 0}};
       st = f2m_solve_batched(c, &sub, o, nullptr);
       if (st) break;
       if (hipMemcpy(clk.data(), c->b_clk.p, ns * 8ull, hipMemcpyDeviceToHost) != hipSuccess) { st = F2M_ERR_HIP; break; }
@@ -1557,23 +1581,23 @@ f2m_status f2m_batch_autotune(f2m_ctx* c, const f2m_batch* b, const f2m_opts* o,
         ++cnt[k];
       }
       for (int k = 0; k < NC; ++k)
-        if (cnt[k] && (best_t[k] < 0 || tc[k] < best_t[k])) { best_t[k] = tc[k]; best[k] = TE{bounds[k], l, bs}; }
+        if (cnt[k] && (best_t[k] < 0 || tc[k] < best_t[k])) { best_t[k] = tc[k]; best[k] = TE{bounds[k], l, bs, -1, 0}; }
     }
   c->b_clk_on = false;
   if (st) { c->b_tune = saved; return st; }
   // classes without samples take the nearest sampled class below, else above
   std::vector<TE> t;
-  TE last{0, 0, 0};
+  TE last{0, 0, 0, -1, 0};
   for (int k = 0; k < NC; ++k) if (best[k].lpn) { last = best[k]; break; }
   for (int k = 0; k < NC; ++k) {
     if (best[k].lpn) last = best[k];
-    t.push_back(TE{bounds[k], last.lpn ? last.lpn : 4, last.bs ? last.bs : 256});
+    t.push_back(TE{bounds[k], last.lpn ? last.lpn : 4, last.bs ? last.bs : 256, -1, 0});
   }
   c->b_tune = t;
   if (cache_path) {
     FILE* f = std::fopen(cache_path, "w");
     if (f) {
-      for (auto& e : t) std::fprintf(f, "%lld %d %d\n", (long long)e.max_n, e.lpn, e.bs);
+      for (auto& e : t) std::fprintf(f, "%lld %d %d %d\n", (long long)e.max_n, e.lpn, e.bs, e.lds);
       std::fclose(f);
     }
   }

@@ -83,10 +83,13 @@ struct f2m_ctx {
     int64_t max_n;  // instances with n <= max_n ...
     int32_t lpn;    // ... use this many lanes per node
     int32_t bs;     // ... and workgroups of this many threads
+    int32_t lds;    // ... and this shared-memory placement (-1: default, else as F2M_BATCH_LDS)
+    int32_t pad;
   };
   std::vector<TuneEntry> b_tune;
   f2m::DevBuf b_clk;                               // per-instance ticks while autotuning
   bool b_clk_on = false;
+
   int64_t b_n_total = 0, b_nslots = 0;
   int32_t b_n_inst = 0;
   // pinned host scalars for small read-backs

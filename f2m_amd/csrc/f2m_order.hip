@@ -9,9 +9,17 @@
 // key), ties keeping input order.  Numbering the nodes of a graph in this
 // order puts graph neighbours close in memory, which is what the sweeps'
 // neighbour-potential gathers need.
-#include <hipcub/hipcub.hpp>
-
+//
+// The sort is a least-significant-digit radix sort written here: 8-bit
+// digits, one pass per digit of the key's used bits (4 for one instance, up
+// to 6 for 4096), each pass a per-tile digit histogram, one exclusive scan of
+// the digit-major counts (the single-pass look-back scan of f2m_common.hip)
+// and a stable scatter.  Stability comes from ranking each tile's elements in
+// their input order: items one after another, waves in order within an item,
+// lanes in order within a wave (lanes of equal digit found with eight
+// ballots).
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "f2m_internal.h"
@@ -85,6 +93,80 @@ __global__ void __launch_bounds__(256) o_keys(const int64_t* __restrict__ off, c
   }
 }
 
+constexpr int RS_T = 256, RS_I = 8, RS_TILE = RS_T * RS_I, RS_BITS = 8, RS_BINS = 1 << RS_BITS;
+static_assert(RS_T == RS_BINS, "one thread per digit in the per-tile tables");
+
+// cnt[d * tiles + tile] = elements of the tile whose digit (at shift) is d
+__global__ void __launch_bounds__(RS_T) o_hist(const unsigned long long* __restrict__ key, int64_t n, int shift,
+                                               int32_t* __restrict__ cnt, int tiles) {
+  __shared__ int32_t h[RS_BINS];
+  const int tid = threadIdx.x;
+  h[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+  for (int it = 0; it < RS_I; ++it) {
+    const int64_t i = base + (int64_t)it * RS_T + tid;
+    if (i < n) atomicAdd(&h[(unsigned)(key[i] >> shift) & (RS_BINS - 1)], 1);
+  }
+  __syncthreads();
+  cnt[(int64_t)tid * tiles + blockIdx.x] = h[tid];
+}
+
+// Stable scatter of one pass: element i of the tile (item it = i / RS_T,
+// thread i % RS_T) goes to off[d * tiles + tile] + (elements of digit d
+// before it in the tile).  off = exclusive scan of o_hist's counts.
+__global__ void __launch_bounds__(RS_T) o_scatter(const unsigned long long* __restrict__ kin,
+                                                  const int64_t* __restrict__ vin,
+                                                  unsigned long long* __restrict__ kout,
+                                                  int64_t* __restrict__ vout, int64_t n, int shift,
+                                                  const int32_t* __restrict__ off, int tiles) {
+  constexpr int NW = RS_T / 64;
+  __shared__ int32_t run[RS_BINS];     // next position of each digit in this tile
+  __shared__ int32_t wc[NW][RS_BINS];  // per-wave digit counts of the current item
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  run[tid] = off[(int64_t)tid * tiles + blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) wc[k][tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int it = 0; it < RS_I; ++it) {
+    const int64_t i = base + (int64_t)it * RS_T + tid;
+    const bool ok = i < n;
+    const unsigned long long k = ok ? kin[i] : 0ull;
+    const int d = (int)((unsigned)(k >> shift) & (RS_BINS - 1));
+    // lanes of this wave holding a valid element with digit d
+    unsigned long long m = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < RS_BITS; ++b) {
+      const bool bit = (d >> b) & 1;
+      const unsigned long long bb = __ballot(ok && bit);
+      m &= bit ? bb : ~bb;
+    }
+    const int r = __popcll(m & lt);
+    if (ok && r == 0) wc[w][d] = __popcll(m);  // the lowest such lane
+    __syncthreads();
+    if (ok) {
+      int pos = run[d] + r;
+      for (int q = 0; q < w; ++q) pos += wc[q][d];
+      if (pos >= 0 && (int64_t)pos < n) {
+        kout[pos] = k;
+        vout[pos] = vin[i];
+      }
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      add += wc[q][tid];
+      wc[q][tid] = 0;
+    }
+    run[tid] += add;
+    __syncthreads();
+  }
+}
+
 __global__ void o_to_i32(int64_t n, const int64_t* __restrict__ a, int32_t* __restrict__ b) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (int32_t)a[i];
@@ -105,14 +187,28 @@ f2m_status order_into(f2m_ctx* c, int n_inst, const int64_t* doff, int64_t n_tot
   int hb = 0;
   while (hb < 31 && (1ll << hb) < n_inst) ++hb;
   const int end_bit = 32 + hb;
-  size_t tmp = 0;
-  F2M_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, B[1].as<unsigned long long>(),
-                                               B[3].as<unsigned long long>(), B[2].as<int64_t>(), perm,
-                                               (int)n_total, 0, end_bit, c->stream));
-  F2M_HIP(c, B[4].reserve(tmp + 256));
-  F2M_HIP(c, hipcub::DeviceRadixSort::SortPairs(B[4].p, tmp, B[1].as<unsigned long long>(),
-                                               B[3].as<unsigned long long>(), B[2].as<int64_t>(), perm,
-                                               (int)n_total, 0, end_bit, c->stream));
+  const int64_t tiles = (n_total + RS_TILE - 1) / RS_TILE;
+  const int64_t ncnt = tiles * RS_BINS;
+  if (ncnt >= INT32_MAX / 2) return fail(c, F2M_ERR_LIMIT, "spatial order: too many points");
+  F2M_HIP(c, B[4].reserve((size_t)(2 * ncnt + 1) * 4));
+  int32_t* cnt = B[4].as<int32_t>();
+  int32_t* off = cnt + ncnt;
+  // ping-pong: keys B[1] <-> B[3], values B[2] <-> perm
+  unsigned long long* kin = B[1].as<unsigned long long>();
+  unsigned long long* kout = B[3].as<unsigned long long>();
+  int64_t* vin = B[2].as<int64_t>();
+  int64_t* vout = perm;
+  for (int shift = 0; shift < end_bit; shift += RS_BITS) {
+    hipLaunchKernelGGL(o_hist, dim3((unsigned)tiles), dim3(RS_T), 0, c->stream, kin, n_total, shift, cnt, (int)tiles);
+    F2M_LAUNCH_CHECK(c);
+    F2M_HIP(c, scan_excess_i32(c, cnt, 0, off, ncnt));
+    hipLaunchKernelGGL(o_scatter, dim3((unsigned)tiles), dim3(RS_T), 0, c->stream, kin, vin, kout, vout, n_total,
+                       shift, off, (int)tiles);
+    F2M_LAUNCH_CHECK(c);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (vin != perm) F2M_HIP(c, hipMemcpyAsync(perm, vin, n_total * 8ull, hipMemcpyDeviceToDevice, c->stream));
   return F2M_OK;
 }
 

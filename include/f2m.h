@@ -299,9 +299,10 @@ f2m_status f2m_solve_batched_warm(f2m_ctx* ctx, const f2m_batch* b, const f2m_op
  * clocks, weighted by workgroup size), keeps the fastest per size class
  * (<=500, 1000, 2000, 4000, 8000, 16000, larger) for later f2m_solve_batched
  * calls on this context.  With cache_path, an existing file ("max_n lanes
- * threads" per line) is loaded instead of tuning, else the chosen table is
- * written there.  The schedules differ in speed only: results are
- * bit-identical. */
+ * threads placement" per line; placement -1 = default, else 0-3 as
+ * F2M_BATCH_LDS) is loaded instead of tuning, else the chosen table is
+ * written there (placement -1).  The schedules differ in speed only: results
+ * are bit-identical. */
 f2m_status f2m_batch_autotune(f2m_ctx* ctx, const f2m_batch* b, const f2m_opts* o, int32_t sample,
                               const char* cache_path);
 /* Copy a batch's arrays to caller arrays [dev|host]: node_off (n_inst+1),

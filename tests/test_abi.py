@@ -97,3 +97,13 @@ def test_tsplib_reader_errors():
             raise AssertionError("no error")
         except L.F2MError as e:
             assert "-4" in str(e)
+
+
+def test_no_library_sorts_or_scans_in_the_product():
+    """Every sort and scan of the product is written here: no hipCUB,
+    rocPRIM or Thrust in the HIP sources."""
+    csrc = os.path.join(ROOT, "f2m_amd", "csrc")
+    for f in sorted(os.listdir(csrc)):
+        text = open(os.path.join(csrc, f)).read()
+        for lib in ("hipcub", "rocprim", "thrust"):
+            assert not re.search(r"#\s*include\s*[<\"]" + lib, text), (f, lib)

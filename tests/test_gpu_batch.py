@@ -146,7 +146,8 @@ def test_batch_autotune_keeps_results(ctx, batch_case, tmp_path):
     say("autotune")
     ctx.batch_autotune(b, sample=len(xs), cache_path=cache)
     lines = open(cache).read().split()
-    assert len(lines) == 21 and set(lines[1::3]) <= {"1", "2", "4", "8"} and set(lines[2::3]) <= {"256", "512", "1024"}
+    assert len(lines) == 28 and set(lines[1::4]) <= {"1", "2", "4", "8"} and set(lines[2::4]) <= {"256", "512", "1024"}
+    assert set(lines[3::4]) == {"-1"}
     same_stats(ctx.solve_batched(b), ref)
     ctx.batch_autotune(b, sample=4, cache_path=cache)  # loads the cache
 

@@ -225,6 +225,14 @@ def test_spatial_order_matches_restatement(ctx):
     perms = ctx.spatial_order_batched([a for a, _ in insts], [b for _, b in insts])
     for (a, b), p in zip(insts, perms):
         assert np.array_equal(p, morton_order(a, b))
+    # 600 instances (six radix passes over instance and Morton bits), many
+    # repeated keys on a coarse grid
+    rng = np.random.default_rng(5)
+    insts = [W.uniform_int(int(n), 300, 500 + i) for i, n in enumerate(rng.integers(1, 700, size=600))]
+    say("spatial order, 600 instances")
+    perms = ctx.spatial_order_batched([a for a, _ in insts], [b for _, b in insts])
+    for (a, b), p in zip(insts, perms):
+        assert np.array_equal(p, morton_order(a, b))
 
 
 def test_morton_numbering_same_optimum(ctx, orc, golden):
