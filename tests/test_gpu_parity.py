@@ -183,6 +183,15 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
         s2 = ctx.solve(g)
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], (tg, te)
         assert np.array_equal(ctx.result()["F"], res["F"]), (tg, te)
+    for v in ("F2M_TILE_GLOB", "F2M_TILE", "F2M_TILE_W_GLOB"):
+        monkeypatch.delenv(v)
+    # tile -> workgroup mapping: tile = workgroup, or grouped per XCD (3 does not divide the tiles)
+    for xcd in ("1", "3"):
+        monkeypatch.setenv("F2M_TILE_XCD", xcd)
+        say("solve C2 Morton, tiles grouped over", xcd, "XCDs")
+        s2 = ctx.solve(g)
+        assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], xcd
+        assert np.array_equal(ctx.result()["F"], res["F"]), xcd
 
 
 def test_full_solve_random_raw(ctx, orc):
