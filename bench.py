@@ -16,7 +16,7 @@ Prints one JSON line.  `value` = ms per solve (lower is better).
 The timed solves run without instrumentation.
 
 roofline: the dominant kernel is the multi-sweep launch over spatial tiles
-(s_sweep_tile, F2M_FUSE sweeps per launch, default 7).  Its algorithmic
+(s_sweep_tile, F2M_FUSE sweeps per launch, default 8).  Its algorithmic
 bytes per launch are, per sweep it runs, 8 B per active slot (nbr int32 +
 a 4-byte cost) + 20 B per node (list offset, F in, F out), counted on the device
 for the timed launches; the average kernel duration comes from the HIP
@@ -157,7 +157,7 @@ def main():
     # fit the configured width): no rate to report
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None
     # the sampled launches are those of the configured width (F2M_FUSE)
-    fuse = int(round(sweeps_per_launch)) if launch_ms > 0 else int(os.environ.get("F2M_FUSE", "7"))
+    fuse = int(round(sweeps_per_launch)) if launch_ms > 0 else int(os.environ.get("F2M_FUSE", "8"))
     kernel = f"s_sweep_tile<{fuse},false,4>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")

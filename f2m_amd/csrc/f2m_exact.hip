@@ -252,33 +252,87 @@ __global__ void __launch_bounds__(XWG) x_rounds_wg(XDev X, int64_t r0, double ep
 }
 
 // per-node terms of the duality gap certificate
-__global__ void x_cert_terms(XDev X, double* __restrict__ pv, double* __restrict__ dv) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < X.n) x_cert_node(X.S, v, X.rp[v], X.rp[v + 1], pv[v], dv[v]);
-}
-// fixed-order sums: DET_T contiguous chunks, each summed in order (one
-// thread per chunk), then the chunk sums in order (one thread)
-constexpr int DET_B = 64;
-__global__ void __launch_bounds__(DET_B) x_det_part(int n, const double* __restrict__ pv,
-                                                    const double* __restrict__ dv, double* __restrict__ part) {
-  const int t = blockIdx.x * DET_B + threadIdx.x;
-  const int64_t chunk = ((int64_t)n + DET_T - 1) / DET_T;
-  const int64_t b = t * chunk;
-  const int64_t e = b + chunk < n ? b + chunk : n;
-  double a = 0.0, d = 0.0;
-  for (int64_t i = b; i < e; ++i) {
-    a += pv[i];
-    d += dv[i];
+// Certificate sums in a fixed order: DET_T contiguous chunks of nodes, each
+// chunk's node terms (x_cert_node: slots in order) summed in node order from
+// 0, then the chunk sums in chunk order -- the oracle's order, so the same
+// bits; both the primal P and the dual bound D.  One workgroup per chunk:
+// XC_LANES lanes per node compute the terms of slots q, q + XC_LANES, ...
+// together (their gathers in flight at once); the node's first lane adds
+// them in slot order (a round of XC_LANES slots at a time, read with
+// shuffles; a slot with w <= v adds +0.0, which leaves these sums of
+// non-negative terms, never -0.0, unchanged), thread 0 the chunk's node
+// values in node order.
+constexpr int XC_LANES = 8, XC_T = 256, XC_G = XC_T / XC_LANES;
+__global__ void __launch_bounds__(XC_T) x_cert_chunks(XDev X, double* __restrict__ part) {
+  __shared__ double na[XC_G], nd[XC_G];
+  const int tid = threadIdx.x, q = tid % XC_LANES, g = tid / XC_LANES;
+  const int64_t chunk = ((int64_t)X.n + DET_T - 1) / DET_T;
+  const int64_t b = (int64_t)blockIdx.x * chunk;
+  const int64_t e = b + chunk < X.n ? b + chunk : X.n;
+  const XState& S = X.S;
+  double A = 0.0, Dc = 0.0;  // the chunk's sums (thread 0)
+  for (int64_t v0 = b; v0 < e; v0 += XC_G) {
+    const int64_t v = v0 + g;
+    const bool ok = v < e;
+    const int s0 = ok ? X.rp[v] : 0, s1 = ok ? X.rp[v + 1] : 0;
+    const double yv = ok ? 0.5 * (S.pR[v] - S.pL[v]) : 0.0;
+    double a = 0.0, bsum = 0.0;
+    const int rounds = (s1 - s0 + XC_LANES - 1) / XC_LANES;
+    // every lane of the wave runs the wave's most rounds (the shuffles need all lanes)
+    int rmax = rounds;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off, 64));
+    for (int r = 0; r < rmax; ++r) {
+      const int s = s0 + r * XC_LANES + q;
+      double ta = 0.0, tb = 0.0;
+      if (s < s1) {
+        const int w = S.nbr[s];
+        if (w > v) {
+          const double x = 0.5 * (double)(S.f[s] + S.f[S.mir[s]]);
+          ta = S.c[s] * x;
+          const double yw = 0.5 * (S.pR[w] - S.pL[w]);
+          tb = fmax(0.0, (yv + yw) - S.c[s]);
+        }
+      }
+      const int base = (threadIdx.x & 63) - q;  // the node's first lane within the wave
+#pragma unroll
+      for (int k = 0; k < XC_LANES; ++k) {
+        const double xa = __shfl(ta, base + k, 64);
+        const double xb = __shfl(tb, base + k, 64);
+        a += xa;
+        bsum += xb;
+      }
+    }
+    if (q == 0) {
+      na[g] = a;
+      nd[g] = 2.0 * yv - bsum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int cnt = (int)(e - v0 < XC_G ? e - v0 : XC_G);
+      for (int k = 0; k < cnt; ++k) {
+        A += na[k];
+        Dc += nd[k];
+      }
+    }
+    __syncthreads();
   }
-  part[t] = a;
-  part[DET_T + t] = d;
+  if (tid == 0) {
+    part[blockIdx.x] = A;
+    part[DET_T + blockIdx.x] = Dc;
+  }
 }
-__global__ void x_det_final(const double* __restrict__ part, double* __restrict__ out) {
+// P = sum of the chunk sums in chunk order, D likewise: the sums are read
+// into shared memory with coalesced loads, then added in order by thread 0
+__global__ void __launch_bounds__(256) x_det_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sp[2 * DET_T];
+  for (int k = threadIdx.x; k < 2 * DET_T; k += 256) sp[k] = part[k];
+  __syncthreads();
   if (threadIdx.x == 0) {
     double P = 0.0, D = 0.0;
     for (int k = 0; k < DET_T; ++k) {
-      P += part[k];
-      D += part[DET_T + k];
+      P += sp[k];
+      D += sp[DET_T + k];
     }
     out[0] = P;
     out[1] = D;
@@ -388,9 +442,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   const int small = env_int("F2M_EXACT_WG", 512);
   Ring* hr = static_cast<Ring*>(c->pinned);
   auto cert = [&]() -> f2m_status {
-    hipLaunchKernelGGL(x_cert_terms, gn, bs, 0, st, X, pv, dv);
-    hipLaunchKernelGGL(x_det_part, dim3(DET_T / DET_B), dim3(DET_B), 0, st, n, pv, dv, part);
-    hipLaunchKernelGGL(x_det_final, dim3(1), dim3(64), 0, st, part, sums);
+    hipLaunchKernelGGL(x_cert_chunks, dim3(DET_T), dim3(XC_T), 0, st, X, part);
+    hipLaunchKernelGGL(x_det_final, dim3(1), dim3(256), 0, st, part, sums);
     F2M_LAUNCH_CHECK(c);
     double h[2];
     F2M_HIP(c, hipMemcpyAsync(h, sums, 16, hipMemcpyDeviceToHost, st));

@@ -342,8 +342,9 @@ constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list 
 constexpr int THDR = 16;     // header ints per tile
 constexpr int TMAXS = 8;     // most sweeps per launch
 // sweeps per launch unless F2M_FUSE says otherwise.  C2 at widths 5 / 6 / 7 / 8: 28.8 / 28.4 /
-// 27.95 / 28.0 ms (after the cheaper list rebuilds; 6 was best before them).
-constexpr int DEFAULT_FUSE = 7;
+// 27.95 / 28.0 ms after the cheaper list rebuilds; with the XCD-grouped tiles
+// widths 6 / 7 / 8: 25.15 / 24.86 / 24.62 ms.
+constexpr int DEFAULT_FUSE = 8;
 // Tile workgroup shapes: W threads, local-node capacity TL = 2W (tile +
 // halo), a halo hash of THSZ >= 2*TL entries, TOV overflow slots.  W = 384 by
 // default.  768 (F2M_TILE_W_GLOB) lets the block_GLOB lists, whose halos grow

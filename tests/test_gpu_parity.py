@@ -158,7 +158,7 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     p = ctx.profile()
     assert p["sweeps"] == st["sweeps"] and p["sweep_launches"] < st["sweeps"] / 3, p
     g = ctx.knn_build(x, y, K, mode)
-    for w in (1, 2, 3, 4, 5, 6, 8):
+    for w in (1, 2, 3, 4, 5, 6, 7):
         monkeypatch.setenv("F2M_FUSE", str(w))
         say("solve C2 Morton, F2M_FUSE", w)
         s2 = ctx.solve(g)
