@@ -16,7 +16,7 @@ Prints one JSON line.  `value` = ms per solve (lower is better).
 The timed solves run without instrumentation.
 
 roofline: the dominant kernel is the multi-sweep launch over spatial tiles
-(s_sweep_tile, F2M_FUSE sweeps per launch, default 8).  Its algorithmic
+(s_sweep_tile, F2M_FUSE sweeps per launch, default 10, one 768-thread tile per CU).  Its algorithmic
 bytes per launch are, per sweep it runs, 8 B per active slot (nbr int32 +
 a 4-byte cost) + 20 B per node (list offset, F in, F out), counted on the device
 for the timed launches; the average kernel duration comes from the HIP
@@ -157,8 +157,9 @@ def main():
     # fit the configured width): no rate to report
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None
     # the sampled launches are those of the configured width (F2M_FUSE)
-    fuse = int(round(sweeps_per_launch)) if launch_ms > 0 else int(os.environ.get("F2M_FUSE", "8"))
-    kernel = f"s_sweep_tile<{fuse},false,4>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
+    fuse = int(round(sweeps_per_launch)) if launch_ms > 0 else int(os.environ.get("F2M_FUSE", "10"))
+    tw = 384 if os.environ.get("F2M_TILE_W") == "384" else 768  # the block_EF tiles' workgroup shape
+    kernel = f"s_sweep_tile<{fuse},false,4,{tw}>" if fuse > 1 else "s_sweep_pad<4,2,true,1,false,true>"
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):

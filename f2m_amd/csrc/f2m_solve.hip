@@ -334,17 +334,21 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
 // tile, and writes only the tile's final potentials.  The halo is recomputed
 // by every tile that needs it: the arithmetic per node and sweep is the one
 // of s_sweep_pad, so the potentials are bit-identical to S single sweeps.
+constexpr int TS_W = 512;    // most nodes per tile of the 768-thread shape (1536 local nodes)
 constexpr int TS = 256;      // most nodes per tile (the default is chosen per graph, <= TS).  On C2,
                              // 128-node tiles for the block_GLOB lists keep deeper halos within TL
                              // (5 sweeps per launch instead of 3) but the doubled halo work made
                              // the solve slower (42.7 against 38.7 ms)
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
 constexpr int THDR = 16;     // header ints per tile
-constexpr int TMAXS = 8;     // most sweeps per launch
+constexpr int TMAXS = 12;    // most sweeps per launch
+constexpr int THDR_OV = 14, THDR_FIT = 15;  // header ints: overflow slots, fit (levels use 0..TMAXS-1)
+static_assert(TMAXS <= THDR_OV, "level counts stay below the overflow and fit ints");
 // sweeps per launch unless F2M_FUSE says otherwise.  C2 at widths 5 / 6 / 7 / 8: 28.8 / 28.4 /
-// 27.95 / 28.0 ms after the cheaper list rebuilds; with the XCD-grouped tiles
-// widths 6 / 7 / 8: 25.15 / 24.86 / 24.62 ms.
-constexpr int DEFAULT_FUSE = 8;
+// 27.95 / 28.0 ms after the cheaper list rebuilds; with 384-thread tiles grouped per XCD widths
+// 6 / 7 / 8: 25.15 / 24.86 / 24.62 ms; with one 768-thread tile per CU widths 7 / 8 / 9 / 10 / 12:
+// 22.8 / 22.55 / 22.42 / 22.33 / 22.58 ms.
+constexpr int DEFAULT_FUSE = 10;
 // Tile workgroup shapes: W threads, local-node capacity TL = 2W (tile +
 // halo), a halo hash of THSZ >= 2*TL entries, TOV overflow slots.  W = 384 by
 // default.  768 (F2M_TILE_W_GLOB) lets the block_GLOB lists, whose halos grow
@@ -403,8 +407,8 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // One workgroup per tile: breadth first from the tile over active slots, one
 // level per pass, up to `depth` levels; fit = the deepest level kept (levels
 // are kept while the local set fits TL and the hash has room).
-// hdr[t*THDR + d] = local nodes of level <= d (d <= fit); hdr[t*THDR + 8] =
-// overflow slots; hdr[t*THDR + 9] = fit (-1 when the overflow slots do not
+// hdr[t*THDR + d] = local nodes of level <= d (d <= fit); hdr[t*THDR + THDR_OV] =
+// overflow slots; hdr[t*THDR + THDR_FIT] = fit (-1 when the overflow slots do not
 // fit).  tnode[t*TL + j] = {id, active slots, overflow offset, 0}; slots k <
 // tpp of local node j at tslot[(t*TL + j)*tpp + k], the others at
 // tovf[t*TOV + offset + k - tpp].  *fit_min = min over tiles of fit: a launch
@@ -516,8 +520,8 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
   __syncthreads();
   if (tid == 0) {
     const int f = s_ov > TOV ? -1 : fit;
-    hdr[t * THDR + 8] = min(s_ov, TOV);
-    hdr[t * THDR + 9] = f;
+    hdr[t * THDR + THDR_OV] = min(s_ov, TOV);
+    hdr[t * THDR + THDR_FIT] = f;
     atomicMin(fit_min, f);
   }
 }
@@ -573,7 +577,7 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
 #pragma unroll
   for (int d = 0; d < S; ++d) cnt[d] = h[d];
   const int nloc = cnt[S - 1];
-  const int nov = h[8];
+  const int nov = h[THDR_OV];
   int4 nd[NP];
   TSlot sl[NP][TPP];
 #pragma unroll
@@ -1108,7 +1112,7 @@ struct Runner {
   int fuse = 1;
   int ntiles = 0;                    // tiles of the current lists
   int ts_ef = TS, ts_glob = TS;  // nodes per tile of the block_EF / block_GLOB lists
-  int tw = TW_EF, tw_glob = TW_EF;   // tile workgroup shape of the current lists, of block_GLOB's
+  int tw = TW_EF, tw_ef = TW_EF, tw_glob = TW_EF;  // tile workgroup shape: current, block_EF's, block_GLOB's
   int tile_xcd = kXcds;              // XCDs for the tile -> workgroup mapping (F2M_TILE_XCD; 1: tile = workgroup)
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
@@ -1212,16 +1216,26 @@ struct Runner {
       int ncu = 0;
       F2M_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
       ncu = std::max(1, ncu);
-      const int64_t per_cu = (n + (int64_t)TS * ncu - 1) / ((int64_t)TS * ncu);
-      const int ts0 = (int)std::max<int64_t>(64, std::min<int64_t>(TS, (n + per_cu * ncu - 1) / (per_cu * ncu)));
-      ts_ef = std::max(32, std::min(TS, env_int("F2M_TILE", ts0)));
-      ts_glob = std::max(32, std::min(TS, env_int("F2M_TILE_GLOB", ts0)));
+      // one 768-thread tile per CU by default: its halo is a smaller share of its local set than
+      // two 384-thread tiles' (n = 100k Gaussian mixture: 24.45 -> 22.69 ms per solve)
+      // (a 1024-thread shape with 2048 local nodes measured slower: 24.9-25.2 against 22.4 ms)
+      auto shape = [](int v) { return v == TW_EF ? TW_EF : TW_GLOB; };
+      tw_ef = shape(env_int("F2M_TILE_W", TW_GLOB));
+      tw_glob = shape(env_int("F2M_TILE_W_GLOB", TW_GLOB));
+      // tile nodes of a workgroup shape: the same whole number of tiles on every CU, as large as
+      // the shape's cap allows (256 nodes for 384 threads, 512 for 768)
+      auto balanced = [&](int tw_) {
+        const int cap = tw_ == TW_EF ? TS : TS_W;
+        const int64_t per_cu = (n + (int64_t)cap * ncu - 1) / ((int64_t)cap * ncu);
+        return (int)std::max<int64_t>(64, std::min<int64_t>(cap, (n + per_cu * ncu - 1) / (per_cu * ncu)));
+      };
+      ts_ef = std::max(32, std::min(tw_ef == TW_EF ? TS : TS_W, env_int("F2M_TILE", balanced(tw_ef))));
+      ts_glob = std::max(32, std::min(tw_glob == TW_EF ? TS : TS_W, env_int("F2M_TILE_GLOB", balanced(tw_glob))));
       const int64_t mt = (n + std::min(ts_ef, ts_glob) - 1) / std::min(ts_ef, ts_glob);  // most tiles
       F2M_HIP(c, w.thdr.reserve((mt * THDR + 16) * 4));
       F2M_HIP(c, w.tnode.reserve(mt * TL_MAX * 16));
       F2M_HIP(c, w.tslot.reserve(mt * TL_MAX * TPP_MAX * 16));
       F2M_HIP(c, w.tovf.reserve(mt * TOV_MAX * 16));
-      tw_glob = env_int("F2M_TILE_W_GLOB", TW_EF) == TW_GLOB ? TW_GLOB : TW_EF;
       tile_xcd = std::max(1, std::min(64, env_int("F2M_TILE_XCD", kXcds)));
     }
     return F2M_OK;
@@ -1389,7 +1403,7 @@ struct Runner {
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
         const int ts = long_lists ? ts_glob : ts_ef;
-        tw = long_lists ? tw_glob : TW_EF;
+        tw = long_lists ? tw_glob : tw_ef;
         pending_kind = kind;
         auto build = [&](auto kern, int threads) {
           hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, depth,
@@ -1484,6 +1498,9 @@ struct Runner {
     if (tpp == 6 && tw == TW_GLOB) {
       if (frozen) go(s_sweep_tile<S, true, 6, TW_GLOB>);
       else go(s_sweep_tile<S, false, 6, TW_GLOB>);
+    } else if (tw == TW_GLOB) {
+      if (frozen) go(s_sweep_tile<S, true, 4, TW_GLOB>);
+      else go(s_sweep_tile<S, false, 4, TW_GLOB>);
     } else if (tpp == 6) {
       if (frozen) go(s_sweep_tile<S, true, 6, TW_EF>);
       else go(s_sweep_tile<S, false, 6, TW_EF>);
@@ -1515,7 +1532,11 @@ struct Runner {
       case 5: launch_tile_t<5>(frozen, mslots, e0, e1); break;
       case 6: launch_tile_t<6>(frozen, mslots, e0, e1); break;
       case 7: launch_tile_t<7>(frozen, mslots, e0, e1); break;
-      default: launch_tile_t<8>(frozen, mslots, e0, e1); break;
+      case 8: launch_tile_t<8>(frozen, mslots, e0, e1); break;
+      case 9: launch_tile_t<9>(frozen, mslots, e0, e1); break;
+      case 10: launch_tile_t<10>(frozen, mslots, e0, e1); break;
+      case 11: launch_tile_t<11>(frozen, mslots, e0, e1); break;
+      default: launch_tile_t<12>(frozen, mslots, e0, e1); break;
     }
     F2M_LAUNCH_CHECK(c);
     if (timed) {

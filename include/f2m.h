@@ -161,7 +161,7 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
 /* ---- instrumentation (new; the reference prints gettimeofday deltas,
  * graph_cuda_define.hpp:29-34) -------------------------------------------------
  * With sample_every > 0, every sample_every-th sweep launch of f2m_solve
- * of the dominant kind (F2M_FUSE sweeps on spatial tiles, default 8; one
+ * of the dominant kind (F2M_FUSE sweeps on spatial tiles, default 10; one
  * sweep with F2M_FUSE=1) is timed by a pair of HIP events that the runtime
  * records at the kernel's own start and end (hipExtLaunchKernelGGL, context
  * stream); f2m_get_profile reports the sampled kernel time, the sweeps and

@@ -158,7 +158,7 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     p = ctx.profile()
     assert p["sweeps"] == st["sweeps"] and p["sweep_launches"] < st["sweeps"] / 3, p
     g = ctx.knn_build(x, y, K, mode)
-    for w in (1, 2, 3, 4, 5, 6, 7):
+    for w in (1, 2, 3, 4, 5, 6, 8, 12):
         monkeypatch.setenv("F2M_FUSE", str(w))
         say("solve C2 Morton, F2M_FUSE", w)
         s2 = ctx.solve(g)
@@ -174,16 +174,19 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
     assert np.array_equal(ctx.result()["F"], res["F"]) and np.array_equal(ctx.result()["x"], res["x"])
     monkeypatch.delenv("F2M_PART_LANES")
-    # nodes per tile of the block_GLOB and block_EF lists; block_GLOB's workgroup shape
-    for tg, te, wg in (("256", "256", "768"), ("64", "128", "384")):
+    # nodes per tile of the block_GLOB and block_EF lists; their workgroup shapes (384 threads
+    # for 768 local nodes, 768 for 1536)
+    for tg, te, wg, we in (("256", "256", "768", "384"), ("64", "128", "384", "768"), ("196", "196", "384", "384"),
+                           ("300", "450", "768", "768")):
         monkeypatch.setenv("F2M_TILE_GLOB", tg)
         monkeypatch.setenv("F2M_TILE", te)
         monkeypatch.setenv("F2M_TILE_W_GLOB", wg)
-        say("solve C2 Morton, tiles", tg, te, wg)
+        monkeypatch.setenv("F2M_TILE_W", we)
+        say("solve C2 Morton, tiles", tg, te, wg, we)
         s2 = ctx.solve(g)
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], (tg, te)
         assert np.array_equal(ctx.result()["F"], res["F"]), (tg, te)
-    for v in ("F2M_TILE_GLOB", "F2M_TILE", "F2M_TILE_W_GLOB"):
+    for v in ("F2M_TILE_GLOB", "F2M_TILE", "F2M_TILE_W_GLOB", "F2M_TILE_W"):
         monkeypatch.delenv(v)
     # tile -> workgroup mapping: tile = workgroup, or grouped per XCD (3 does not divide the tiles)
     for xcd in ("1", "3"):
