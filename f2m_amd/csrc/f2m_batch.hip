@@ -1431,7 +1431,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   // The buckets after the first run on up to 3 more streams (bucket k on side
   // stream (k-1) mod s; F2M_BATCH_OVERLAP=s, 0 = one stream), forked after the
   // queue reset: the smaller instances take CUs as the larger ones' workgroups
-  // finish instead of waiting for the slowest of them (C4: 1.31 -> 1.02 s)
+  // finish instead of waiting for the slowest of them (4096 instances of n = 100..20000: 1.31 -> 1.02 s)
   const int nside = std::min(std::max(0, env_int("F2M_BATCH_OVERLAP", BatchPlan::MAXSIDE)),
                              std::min<int>(BatchPlan::MAXSIDE, (int)P->bucket_lpn.size() - 1));
   const bool overlap = nside > 0;

@@ -335,24 +335,24 @@ __global__ void __launch_bounds__(BS) s_sweep_pad(int n, const int32_t* __restri
 // by every tile that needs it: the arithmetic per node and sweep is the one
 // of s_sweep_pad, so the potentials are bit-identical to S single sweeps.
 constexpr int TS_W = 512;    // most nodes per tile of the 768-thread shape (1536 local nodes)
-constexpr int TS = 256;      // most nodes per tile (the default is chosen per graph, <= TS).  On C2,
-                             // 128-node tiles for the block_GLOB lists keep deeper halos within TL
-                             // (5 sweeps per launch instead of 3) but the doubled halo work made
-                             // the solve slower (42.7 against 38.7 ms)
+constexpr int TS = 256;      // most nodes per tile (the default is chosen per graph, <= TS).  On the
+                             // n = 100k Gaussian-mixture solve 128-node tiles for the block_GLOB lists
+                             // keep deeper halos within TL (5 sweeps per launch instead of 3) but the
+                             // doubled halo work made the solve slower (42.7 against 38.7 ms)
 constexpr int TPP_MAX = 6;   // padded slots per local node: 6 while every list holds >= 5, else 4
 constexpr int THDR = 16;     // header ints per tile
 constexpr int TMAXS = 12;    // most sweeps per launch
 constexpr int THDR_OV = 14, THDR_FIT = 15;  // header ints: overflow slots, fit (levels use 0..TMAXS-1)
 static_assert(TMAXS <= THDR_OV, "level counts stay below the overflow and fit ints");
-// sweeps per launch unless F2M_FUSE says otherwise.  C2 at widths 5 / 6 / 7 / 8: 28.8 / 28.4 /
-// 27.95 / 28.0 ms after the cheaper list rebuilds; with 384-thread tiles grouped per XCD widths
-// 6 / 7 / 8: 25.15 / 24.86 / 24.62 ms; with one 768-thread tile per CU widths 7 / 8 / 9 / 10 / 12:
-// 22.8 / 22.55 / 22.42 / 22.33 / 22.58 ms.
+// sweeps per launch unless F2M_FUSE says otherwise.  The n = 100k Gaussian-mixture solve at
+// widths 5 / 6 / 7 / 8: 28.8 / 28.4 / 27.95 / 28.0 ms after the cheaper list rebuilds; with
+// 384-thread tiles grouped per XCD widths 6 / 7 / 8: 25.15 / 24.86 / 24.62 ms; with one
+// 768-thread tile per CU widths 7 / 8 / 9 / 10 / 12: 22.8 / 22.55 / 22.42 / 22.33 / 22.58 ms.
 constexpr int DEFAULT_FUSE = 10;
 // Tile workgroup shapes: W threads, local-node capacity TL = 2W (tile +
 // halo), a halo hash of THSZ >= 2*TL entries, TOV overflow slots.  W = 384 by
 // default.  768 (F2M_TILE_W_GLOB) lets the block_GLOB lists, whose halos grow
-// faster, fit five levels instead of two on C2 (6 sweeps per launch instead
+// faster, fit five levels instead of two at n = 100k (6 sweeps per launch instead
 // of 3), but the larger recomputed halos made the solve slower (39.1 against
 // 37.2 ms): a launch's time follows its node updates more than its count.
 template <int W>
@@ -617,7 +617,9 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
     ovF[tid] = of;
   }
   __syncthreads();
-  // sweep 1: every local node of level <= S-1
+  // sweep 1: every local node of level <= S-1.  Shared memory holds the
+  // halves 0.5*F of the local potentials (the operands of dval_h); a thread
+  // keeps its nodes' own potentials in registers.
   double gabs = 0.0;
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -641,8 +643,27 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
         Fout[v] = f;
         gabs = fmax(gabs, fabs(g));
       } else {
-        fb[0][j] = f;
+        fv[p] = f;
+        fb[0][j] = 0.5 * f;
       }
+    }
+  }
+  // byte offsets into a half buffer of each slot's two dval_h operands, the
+  // lower id's half first: the operand order is fixed per slot, so sweeps
+  // 2..S read the two halves in that order and add them without selects
+  // (padded slots: the node itself twice, D = +inf).  The n = 100k
+  // Gaussian-mixture solve: 21.73 -> 20.99 ms against selecting the halves
+  // by value each sweep.
+  uint32_t oa[NP][TPP], ob[NP][TPP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const uint32_t own = (uint32_t)t_loc<W>(tid + p * TW) * 8u;
+#pragma unroll
+    for (int k = 0; k < TPP; ++k) {
+      const uint32_t nb = t_loc<W>(sl[p][k].loc) * 8u;
+      const bool lo = nd[p].x < sl[p][k].nbr;
+      oa[p][k] = lo ? own : nb;
+      ob[p][k] = lo ? nb : own;
     }
   }
   // sweeps 2..S from shared memory: sweep q for the local nodes of level <= S-q
@@ -657,16 +678,20 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
       const int j = tid + p * TW;
       if (j < cq) {
         const int v = nd[p].x, L = nd[p].y;
-        const double f0 = src[j];
-        double hn[TPP];  // all reads first (slots past L read a clamped, unused entry)
+        const double f0 = fv[p], hv = 0.5 * f0;
+        const char* sb = reinterpret_cast<const char*>(src);
+        double d[TPP];  // dval_h of every padded slot
 #pragma unroll
-        for (int k = 0; k < TPP; ++k) hn[k] = 0.5 * src[t_loc<W>(sl[p][k].loc)];
-        Top3 t3;
-        push_padded<TPP>(t3, sl[p], hn, 0.5 * f0, v);
+        for (int k = 0; k < TPP; ++k)
+          d[k] = (sl[p][k].D + *reinterpret_cast<const double*>(sb + oa[p][k])) +
+                 *reinterpret_cast<const double*>(sb + ob[p][k]);
+        Top3 t3 = top3_of4(d[0], d[1], d[2], d[3]);  // the same as pushing them
+#pragma unroll
+        for (int k = 4; k < TPP; ++k) t3.push(d[k]);
 #pragma unroll 1
         for (int k = TPP; k < L; ++k) {
           const TSlot o = ov[(nd[p].z + k - TPP) & (TOV - 1)];
-          t3.push(dval(o.D, f0, src[t_loc<W>(o.loc)], v, o.nbr));
+          t3.push(dval_h(o.D, hv, src[t_loc<W>(o.loc)], v, o.nbr));
         }
         const double g = t3.grad();
         const double f = (FROZEN && fz[p]) ? f0 : f0 + g;
@@ -674,7 +699,8 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
           Fout[v] = f;
           gabs = fmax(gabs, fabs(g));
         } else {
-          dst[j] = f;
+          fv[p] = f;
+          dst[j] = 0.5 * f;
         }
       }
     }
