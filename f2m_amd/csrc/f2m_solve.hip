@@ -359,7 +359,7 @@ template <int W>
 struct TCfg {
   static constexpr int TW = W;
   static constexpr int TL = 2 * W;
-  static constexpr int THB = W <= 384 ? 11 : 12;  // log2 of the halo hash size
+  static constexpr int THB = W <= 384 ? 11 : 13;  // log2 of the halo hash size (1/8 full or less at 768)
   static constexpr int THSZ = 1 << THB;
   static constexpr int TOV = W <= 384 ? 1024 : 2048;
   static_assert((TOV & (TOV - 1)) == 0 && TOV >= TW, "TOV is a power of two, one speculative record per thread");
@@ -447,31 +447,66 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
     hdr[t * THDR] = c0;
   }
   __syncthreads();
+  const int lane = tid & 63;
+  const unsigned long long below = (1ull << lane) - 1;  // lanes under this one
   int lo = 0, hi = c0, fit = 0;
   for (int d = 0; d < depth; ++d) {
-    for (int j = lo + g; j < hi; j += NG) {
-      const int v = lg[j], L = act[v], ov = povo[v];
-      for (int k = q; k < L; k += LN) {
-        const int w = k < pp ? psl[(int64_t)v * pp + k].nbr : slot_at(ov, k).nbr;
-        if (w >= t0 && w < t0 + c0) continue;
-        unsigned h = t_hash<W>(w);
-        int probe = 0;
-        for (; probe < THSZ; ++probe) {
-          const int prev = atomicCAS(&hkey[h], -1, w);
-          if (prev == -1) {
-            const int idx = atomicAdd(&s_nl, 1);
+    // passes over the frontier [lo, hi), NG nodes each, the same count in every
+    // wave (ballots below need whole waves)
+    for (int j0 = lo; j0 < hi; j0 += NG) {
+      const int j = j0 + g;
+      const bool live = j < hi;
+      int v = 0, L = 0, ov = 0, wq = -1;
+      if (live) {
+        v = lg[j];
+        L = act[v];
+        ov = povo[v];
+        // slot q of the padded row read with the list length, not after it
+        // (in bounds for any length): one dependent global read per level
+        if (q < pp) wq = psl[(int64_t)v * pp + q].nbr;
+      }
+      // claim a hash entry for neighbour w (has: the slot exists) outside the
+      // tile and not seen yet; local indices for the wave's fresh nodes come
+      // from one shared-memory atomic.  Called by every lane of the wave.
+      auto claim = [&](bool has, int w) {
+        bool fresh = false;
+        unsigned hpos = 0;
+        if (has && !(w >= t0 && w < t0 + c0)) {
+          unsigned h = t_hash<W>(w);
+          int probe = 0;
+          for (; probe < THSZ; ++probe) {
+            const int prev = atomicCAS(&hkey[h], -1, w);
+            if (prev == -1) {
+              fresh = true;
+              hpos = h;
+              break;
+            }
+            if (prev == w) break;
+            h = (h + 1) & (THSZ - 1);
+          }
+          if (probe == THSZ) s_full = 1;
+        }
+        const unsigned long long m = __ballot(fresh);
+        if (m) {
+          const int first = __ffsll((long long)m) - 1;
+          int base = 0;
+          if (lane == first) base = atomicAdd(&s_nl, __popcll(m));
+          base = __shfl(base, first);
+          if (fresh) {
+            const int idx = base + __popcll(m & below);
             if (idx < TL) {
               lg[idx] = w;
-              hval[h] = (short)idx;
+              hval[hpos] = (short)idx;
             } else {
               s_full = 1;
             }
-            break;
           }
-          if (prev == w) break;
-          h = (h + 1) & (THSZ - 1);
         }
-        if (probe == THSZ) s_full = 1;
+      };
+      for (int k = q;; k += LN) {
+        const bool has = live && k < L;
+        if (!__any(has)) break;
+        claim(has, has ? (k == q && q < pp ? wq : k < pp ? psl[(int64_t)v * pp + k].nbr : slot_at(ov, k).nbr) : -1);
       }
     }
     __syncthreads();
@@ -484,14 +519,43 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
     fit = d + 1;
     if (tid == 0) hdr[t * THDR + d + 1] = hi;
   }
-  for (int j = g; j < hi; j += NG) {  // the four lanes of a node run together
-    const int v = lg[j], L = act[v], ov = povo[v];
-    int ovo = 0;
-    if (q == 0 && L > tpp) ovo = atomicAdd(&s_ov, L - tpp);
-    ovo = __shfl(ovo, 0, LN);
+  // records: every pass's list lengths and first slots are read before any
+  // is used (one dependent global level for the whole pass set)
+  constexpr int RP = (TL + NG - 1) / NG;
+  int rv[RP], rL[RP], rov[RP];
+  ESlot rq[RP];
+#pragma unroll
+  for (int r = 0; r < RP; ++r) {
+    const int j = g + r * NG;
+    rv[r] = rL[r] = rov[r] = 0;
+    if (j < hi) {
+      const int v = lg[j];
+      rv[r] = v;
+      rL[r] = act[v];
+      rov[r] = povo[v];
+      if (q < pp) rq[r] = psl[(int64_t)v * pp + q];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RP; ++r) {  // the four lanes of a node run together
+    const int j = g + r * NG;
+    // overflow offsets: a wave's exclusive prefix sum of its nodes' overflow
+    // counts plus one shared-memory atomic per wave
+    const int x = (j < hi && q == 0 && rL[r] > tpp) ? rL[r] - tpp : 0;
+    int incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    int base = 0;
+    if (lane == 63 && incl > 0) base = atomicAdd(&s_ov, incl);
+    base = __shfl(base, 63);
+    if (j >= hi) continue;
+    const int v = rv[r], L = rL[r], ov = rov[r];
+    const int ovo = __shfl(base + incl - x, 0, LN);
     if (q == 0) tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, 0);
-    for (int k = q; k < L; k += LN) {
-      const ESlot e = k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k);
+    auto rec = [&](int k, const ESlot& e) {
       int loc = -1;
       if (e.nbr >= t0 && e.nbr < t0 + c0) {
         loc = e.nbr - t0;
@@ -511,7 +575,12 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
       const TSlot ts{e.D, e.nbr, loc};
       if (k < tpp) tslot[((int64_t)t * TL + j) * tpp + k] = ts;
       else if (ovo + (k - tpp) < TOV) tovf[(int64_t)t * TOV + ovo + (k - tpp)] = ts;
+    };
+    if (q < L) {
+      if (q < pp) rec(q, rq[r]);
+      else rec(q, slot_at(ov, q));
     }
+    for (int k = q + LN; k < L; k += LN) rec(k, k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k));
     // padded slots past the list: D = +inf and the node itself, so that a
     // sweep computes them unmasked, as +inf (above every sentinel of Top3)
     for (int k = q; k < tpp; k += LN)
