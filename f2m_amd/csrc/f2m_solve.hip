@@ -359,7 +359,7 @@ template <int W>
 struct TCfg {
   static constexpr int TW = W;
   static constexpr int TL = 2 * W;
-  static constexpr int THB = W <= 384 ? 11 : 13;  // log2 of the halo hash size (1/8 full or less at 768)
+  static constexpr int THB = W <= 384 ? 11 : 13;  // log2 of the halo hash size (at most 3/16 full at 768)
   static constexpr int THSZ = 1 << THB;
   static constexpr int TOV = W <= 384 ? 1024 : 2048;
   static_assert((TOV & (TOV - 1)) == 0 && TOV >= TW, "TOV is a power of two, one speculative record per thread");
@@ -416,7 +416,8 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // The build runs T threads (TL by default: every level pass and the record
 // pass are one pass over the local nodes) for the tiles of the W-thread sweep.
 template <int W, int T = TCfg<W>::TL>
-__global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp, int depth, const int32_t* __restrict__ act,
+__global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restrict__ tst, int pp, int tpp, int depth,
+                                                   const int32_t* __restrict__ act,
                                                    const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
@@ -432,7 +433,7 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
   __shared__ int s_nl, s_full, s_ov;
   const int t = blockIdx.x, tid = threadIdx.x;
   const int q = tid & (LN - 1), g = tid / LN;
-  const int t0 = t * ts, c0 = min(ts, n - t0);
+  const int t0 = tst[t], c0 = tst[t + 1] - t0;  // the tile: ids [t0, t0 + c0)
   // slot k >= pp of node v, whose overflow offset is ov = povo[v]
   auto slot_at = [&](int ov, int k) -> const ESlot& { return psl[(int64_t)n * pp + ov + (k - pp)]; };
   for (int i = tid; i < THSZ; i += TW) {
@@ -593,6 +594,107 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, int ts, int pp, int tpp
     hdr[t * THDR + THDR_FIT] = f;
     atomicMin(fit_min, f);
   }
+}
+
+// Tile boundaries for the next build of a list kind, from this build's
+// headers.  A launch lasts as long as its slowest tile, and a tile's time
+// follows its work: the node updates of the launch's S sweeps (sum over the
+// levels d < S of the local nodes of level <= d) plus ovw per overflow slot
+// (an overflow slot adds a serial step to a sweep of its wave; on the n = 100k
+// Gaussian mixture a fit of the tiles' times gives ~280 node updates per
+// overflow slot for the block_EF lists, ~1.5 for the block_GLOB lists).  The
+// new boundaries cut the cumulative work, taken as uniform over each tile's
+// ids, into equal parts, halfway from the old boundaries.  Every tile keeps
+// between 1 and cap ids, else the boundaries go back to the uniform split
+// t*ts.  One workgroup of BAL_T threads; T <= BAL_MAX tiles (not called
+// above).  Deterministic: the headers are set sizes.
+constexpr int BAL_T = 1024, BAL_MAX = 4096;
+__global__ void __launch_bounds__(BAL_T) s_tile_balance(int T, int n, int ts, int cap, int fuse, double ovw,
+                                                       const int32_t* __restrict__ hdr,
+                                                       const int32_t* __restrict__ fit_min, int32_t* __restrict__ tst) {
+  __shared__ double P[BAL_MAX + 1];
+  __shared__ int old[BAL_MAX + 1];
+  __shared__ double part[BAL_T / 64];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  const int S = min(fuse, *fit_min + 1);
+  for (int t = tid; t <= T; t += BAL_T) old[t] = tst[t];
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  // work per tile, then an exclusive prefix sum: thread i sums its chunk
+  const int ch = (T + BAL_T - 1) / BAL_T, a = tid * ch, b = min(T, a + ch);
+  double sum = 0.0;
+  for (int t = a; t < b; ++t) {
+    const int32_t* h = hdr + (int64_t)t * THDR;
+    double wk = ovw * h[THDR_OV];
+    for (int d = 0; d < S; ++d) wk += h[d];
+    P[t] = wk;
+    sum += wk;
+  }
+  // exclusive scan of the chunk sums: within each wave by shuffles, then the
+  // wave totals (in order: the bits do not depend on timing)
+  const int lane = tid & 63, wv = tid >> 6;
+  double inc = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) part[wv] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    double run = 0.0;
+    for (int i = 0; i < BAL_T / 64; ++i) {
+      const double x = part[i];
+      part[i] = run;
+      run += x;
+    }
+  }
+  __syncthreads();
+  double acc = part[wv] + (inc - sum);
+  for (int t = a; t < b; ++t) {
+    const double x = P[t];
+    P[t] = acc;
+    acc += x;
+  }
+  if (b == T && a < b) P[T] = acc;
+  __syncthreads();
+  const double tot = P[T];
+  // boundary k: the id where the cumulative work reaches k/T of the total
+  int nb[(BAL_MAX + BAL_T - 1) / BAL_T + 1];
+  int m = 0;
+  for (int k = tid; k <= T; k += BAL_T, ++m) {
+    int r = k == 0 ? 0 : n;
+    if (k > 0 && k < T) {
+      const double target = tot * k / T;
+      int lo = 0, hi = T - 1;  // last u with P[u] <= target
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P[mid] <= target) lo = mid;
+        else hi = mid - 1;
+      }
+      const double wu = P[lo + 1] - P[lo];
+      const double f = wu > 0.0 ? fmin(1.0, (target - P[lo]) / wu) : 0.0;
+      r = old[lo] + (int)(f * (old[lo + 1] - old[lo]));
+    }
+    nb[m] = (int)(((int64_t)r + old[k]) / 2);
+  }
+  __syncthreads();  // every read of old is done
+  m = 0;
+  for (int k = tid; k <= T; k += BAL_T, ++m) old[k] = nb[m];
+  __syncthreads();
+  for (int k = tid; k < T; k += BAL_T) {
+    const int sz = old[k + 1] - old[k];
+    if (sz < 1 || sz > cap) bad = 1;
+  }
+  __syncthreads();
+  const bool ok = bad == 0 && old[0] == 0 && old[T] == n;
+  for (int k = tid; k <= T; k += BAL_T) tst[k] = ok ? old[k] : (int)min((int64_t)k * ts, (int64_t)n);
+}
+// uniform tile boundaries t*ts (the last tile ends at n)
+__global__ void s_tile_uniform(int T, int n, int ts, int32_t* __restrict__ tst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= T) tst[k] = (int)min((int64_t)k * ts, (int64_t)n);
 }
 
 // Top3 over the padded slots, every slot pushed, unmasked: a slot past the
@@ -1209,6 +1311,14 @@ struct Runner {
   int ts_ef = TS, ts_glob = TS;  // nodes per tile of the block_EF / block_GLOB lists
   int tw = TW_EF, tw_ef = TW_EF, tw_glob = TW_EF;  // tile workgroup shape: current, block_EF's, block_GLOB's
   int tile_xcd = kXcds;              // XCDs for the tile -> workgroup mapping (F2M_TILE_XCD; 1: tile = workgroup)
+  // tile boundaries balanced by work (s_tile_balance) for the block_EF lists
+  // (F2M_TILE_BALANCE=1, default), for both list kinds (2) or for neither (0).
+  // n = 100k Gaussian mixture: 20.81 -> 20.60 ms per solve with the block_EF
+  // lists balanced.  Balancing the block_GLOB lists too evens their local sets,
+  // so more levels fit (5.4 sweeps per launch instead of 5.0) but the heavier
+  // launches made the solve slower (21.19 ms).
+  int balance = 1;
+  int64_t tile_cap = 0;              // most tiles of either list kind
   bool tiles_pending = false;
   int width = 1;  // sweeps per launch on the current lists: min(fuse, fit_min + 1)
   // halo levels the next build of short (0) / long (1) lists runs: one more
@@ -1327,15 +1437,24 @@ struct Runner {
       ts_ef = std::max(32, std::min(tw_ef == TW_EF ? TS : TS_W, env_int("F2M_TILE", balanced(tw_ef))));
       ts_glob = std::max(32, std::min(tw_glob == TW_EF ? TS : TS_W, env_int("F2M_TILE_GLOB", balanced(tw_glob))));
       const int64_t mt = (n + std::min(ts_ef, ts_glob) - 1) / std::min(ts_ef, ts_glob);  // most tiles
-      F2M_HIP(c, w.thdr.reserve((mt * THDR + 16) * 4));
+      tile_cap = mt;
+      F2M_HIP(c, w.thdr.reserve((mt * THDR + 16 + 2 * (mt + 1)) * 4));
       F2M_HIP(c, w.tnode.reserve(mt * TL_MAX * 16));
       F2M_HIP(c, w.tslot.reserve(mt * TL_MAX * TPP_MAX * 16));
       F2M_HIP(c, w.tovf.reserve(mt * TOV_MAX * 16));
       tile_xcd = std::max(1, std::min(64, env_int("F2M_TILE_XCD", kXcds)));
+      balance = std::max(0, std::min(2, env_int("F2M_TILE_BALANCE", 1)));
+      for (int kind = 0; kind < 2; ++kind) {  // uniform boundaries until a build of the kind balances them
+        const int ts = kind ? ts_glob : ts_ef, T = (int)((n + ts - 1) / ts);
+        hipLaunchKernelGGL(s_tile_uniform, dim3((T + 256) / 256), dim3(256), 0, c->stream, T, n, ts, tile_starts(kind));
+      }
+      F2M_LAUNCH_CHECK(c);
     }
     return F2M_OK;
   }
   int32_t* tile_fit() const { return w.thdr.as<int32_t>() + (int64_t)ntiles * THDR; }
+  // first ids of the tiles of list kind k (0: block_EF, 1: block_GLOB) and n
+  int32_t* tile_starts(int k) const { return w.thdr.as<int32_t>() + tile_cap * THDR + 16 + k * (tile_cap + 1); }
   void set_width(int32_t fit_min) {
     width = std::max(1, std::min(fuse, fit_min + 1));
     if (fit_min >= 0) depth_next[pending_kind] = fit_min + 1;
@@ -1501,13 +1620,20 @@ struct Runner {
         tw = long_lists ? tw_glob : tw_ef;
         pending_kind = kind;
         auto build = [&](auto kern, int threads) {
-          hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, ts, pp, tpp, depth,
+          hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, (const int32_t*)tile_starts(kind), pp,
+                             tpp, depth,
                              (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
                              w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
                              tile_fit());
         };
         if (tw == TW_GLOB) build(s_tile_build<TW_GLOB, 1024>, 1024);
         else build(s_tile_build<TW_EF>, TCfg<TW_EF>::TL);
+        if (balance > kind && ntiles <= BAL_MAX) {  // boundaries for the next build of this kind
+          const int cap = tw == TW_EF ? TS : TS_W;
+          hipLaunchKernelGGL(s_tile_balance, dim3(1), dim3(BAL_T), 0, c->stream, ntiles, n, ts, cap, fuse,
+                             tpp == 4 ? 256.0 : 2.0, (const int32_t*)w.thdr.as<int32_t>(),
+                             (const int32_t*)tile_fit(), tile_starts(kind));
+        }
         tiles_pending = true;
         width = 1;
       }

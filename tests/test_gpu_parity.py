@@ -149,8 +149,9 @@ def test_full_solve_c2(ctx, orc):
 def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     """Morton-numbered C2: the schedule's sweeps run several per launch on
     spatial tiles (halo levels recomputed in shared memory).  The default
-    width equals the oracle bit for bit, and every width 1..8 gives the same
-    potentials and sweep count; so do other tile sizes."""
+    width equals the oracle bit for bit, and every width 1..12 gives the same
+    potentials and sweep count; so do other tile sizes, workgroup mappings
+    and tile boundaries."""
     x0, y0, mode, K = W.c2()
     perm = ctx.spatial_order(x0, y0)
     x, y = x0[perm], y0[perm]
@@ -195,6 +196,18 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
         s2 = ctx.solve(g)
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], xcd
         assert np.array_equal(ctx.result()["F"], res["F"]), xcd
+    monkeypatch.delenv("F2M_TILE_XCD")
+    # tile boundaries: uniform, balanced by work for both list kinds, and with 100-node tiles
+    # balanced for both (the boundaries move every rebuild)
+    for bal, te in (("0", None), ("2", None), ("2", "100")):
+        monkeypatch.setenv("F2M_TILE_BALANCE", bal)
+        if te:
+            monkeypatch.setenv("F2M_TILE", te)
+            monkeypatch.setenv("F2M_TILE_GLOB", te)
+        say("solve C2 Morton, tile balance", bal, te)
+        s2 = ctx.solve(g)
+        assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], bal
+        assert np.array_equal(ctx.result()["F"], res["F"]), bal
 
 
 def test_full_solve_random_raw(ctx, orc):
