@@ -49,6 +49,7 @@ struct BArgs {
   ESlot* ef;       // compacted active lists, per instance at its slot base
                    // (16 B per slot: D doubles, then nbr ints, see Inst)
   int32_t fcap;    // doubles of dynamic shared memory for the potentials
+  int32_t sweep_regs;    // 1: runs of sweeps hold the lists in registers where they can (wg_sweeps_reg)
   const int32_t* order;  // instance ids of this bucket, decreasing size
   int32_t n_order;
   int32_t* queue;
@@ -283,6 +284,107 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
   return block_max_all(gmax, sh);  // its barriers publish the new shared values
 }
 
+// A run of cnt sweeps on fixed lists (sweep_n) for an instance of at most
+// NP*BS nodes whose potentials (both buffers) and active lists sit in shared
+// memory.  Thread u owns nodes u, u + BS, ...  Each node's first PASS_U
+// active slots are read once for the whole run and kept in registers: D and
+// the byte offsets of the two halves that dval_h adds to it, the lower id's
+// first (padded slots: D = +inf and the node itself twice).  So a sweep
+// reads only the halves, and the slots past PASS_U, from shared memory, and
+// never writes global memory.  During the run the two shared buffers hold
+// the halves 0.5*F; each owner keeps its nodes' full potentials in
+// registers and writes them back at the end, to the shared copy of F[c] and
+// to F[c] in global memory.  0.5*F is exact, and the operands and arithmetic
+// of every node update are wg_pass's, so the bits are the same.  Returns the
+// max |g| of the last sweep when want_max (else 0); barriers inside.
+template <int BS, int NP>
+__device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0, int cnt, bool use_frozen,
+                                                         double* sh, bool want_max) {
+  const int n = P.n;
+  const double* eD = b_dyn + P.lofs;
+  const uint16_t* eN16 = reinterpret_cast<const uint16_t*>(eD + P.ltot);
+  double Dk[NP][PASS_U];
+  uint32_t oa[NP][PASS_U], ob[NP][PASS_U];
+  int bb[NP], L[NP];
+  double fv[NP];
+  bool fz[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int v = threadIdx.x + p * BS;
+    bb[p] = 0;
+    L[p] = 0;
+    fv[p] = 0.0;
+    fz[p] = false;
+    if (v < n) {
+      bb[p] = P.efoff[v];
+      L[p] = P.efoff[v + 1] - bb[p];
+      fv[p] = b_dyn[c0 * n + v];
+      fz[p] = use_frozen && P.frozen[v] != 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PASS_U; ++k) {
+      int w = v;
+      Dk[p][k] = __builtin_inf();
+      if (k < L[p]) {
+        Dk[p][k] = eD[bb[p] + k];
+        w = (int)eN16[bb[p] + k];
+      }
+      const bool lo = v < w;
+      oa[p][k] = (uint32_t)(lo ? v : w) * 8u;
+      ob[p][k] = (uint32_t)(lo ? w : v) * 8u;
+    }
+    // the owner's entry from here on holds the half (read above by its owner only)
+    if (v < n) b_dyn[c0 * n + v] = 0.5 * fv[p];
+  }
+  __syncthreads();
+  int c = c0;
+  double gmax = 0.0;
+  for (int i = 0; i < cnt; ++i) {
+    const double* srcd = b_dyn + c * n;
+    const char* src = reinterpret_cast<const char*>(srcd);
+    double* dst = b_dyn + (c ^ 1) * n;
+    const bool last = i == cnt - 1;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int v = threadIdx.x + p * BS;
+      if (v < n) {
+        double d[PASS_U];
+#pragma unroll
+        for (int k = 0; k < PASS_U; ++k)
+          d[k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
+                 *reinterpret_cast<const double*>(src + ob[p][k]);
+        Top3 t = top3_of4(d[0], d[1], d[2], d[3]);  // the same as pushing them
+        const double hv = 0.5 * fv[p];
+        for (int k = PASS_U; k < L[p]; ++k) {
+          const int w = (int)eN16[bb[p] + k];
+          t.push(dval_h(eD[bb[p] + k], hv, srcd[w], v, w));
+        }
+        const double g = t.grad();
+        const double f = fz[p] ? fv[p] : fv[p] + g;
+        fv[p] = f;
+        dst[v] = 0.5 * f;
+        if (last) gmax = fmax(gmax, fabs(g));
+      }
+    }
+    __syncthreads();
+    c ^= 1;
+  }
+  // full potentials back (the owner's entries; nobody reads them before the
+  // barrier below)
+  double* Fg = c ? P.F1 : P.F0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int v = threadIdx.x + p * BS;
+    if (v < n) {
+      b_dyn[c * n + v] = fv[p];
+      Fg[v] = fv[p];
+    }
+  }
+  if (want_max) return block_max_all(gmax, sh);  // its barriers publish the potentials
+  __syncthreads();
+  return 0.0;
+}
+
 // update_EF (graph_cuda.cu:229-252) inside the workgroup: efoff = exclusive
 // scan of act (global memory), then each node's active prefix copied into the
 // split lists -- in shared memory at b_dyn + lofs (efD doubles, then efN as
@@ -381,6 +483,7 @@ struct Sched {
   int fmode;              // FS_HBM, FS_ONE or FS_TWO
   PView pv;               // the passes' view of I (lists in shared or global memory)
   int dynb;               // bytes of shared memory at b_dyn
+  int regs = 0;           // runs of sweeps may hold the lists in registers (wg_sweeps_reg)
   int cur = 0;
   int64_t sweeps = 0;
   double eps;
@@ -434,6 +537,13 @@ struct Sched {
     const int fm = fmode;
     int c = cur;
     double m = 0.0;
+    if (regs && cnt > 0 && fm == FS_TWO && P.lofs >= 0 && P.n <= 2 * BS) {
+      m = P.n <= BS ? wg_sweeps_reg<BS, 1>(P, c, cnt, frozen, shm, want_max)
+                    : wg_sweeps_reg<BS, 2>(P, c, cnt, frozen, shm, want_max);
+      cur = c ^ (cnt & 1);
+      sweeps += cnt;
+      return m;
+    }
     const int4 pre = first_bounds(P, BS);  // the lists do not change during these sweeps
     if (P.lofs >= 0) {
       for (int i = 0; i < cnt; ++i) {
@@ -853,6 +963,7 @@ __device__ __attribute__((noinline)) void solve_instance(const BArgs& a, int ins
   // the same space holds xdet in the exact stage, after the last pass
   const int fmode = 2 * I.n <= a.fcap ? FS_TWO : (I.n <= a.fcap ? FS_ONE : FS_HBM);
   Sched<BS, LPN> S{I, o, sh, shi, fmode, pview(I), a.fcap * 8};
+  S.regs = a.sweep_regs;
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
@@ -1305,6 +1416,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
     if (const char* e = std::getenv(nm)) key.insert(key.end(), e, e + std::strlen(e));
   if (const char* e = std::getenv("F2M_BATCH_LDS")) key.insert(key.end(), e, e + std::strlen(e));
   if (const char* e = std::getenv("F2M_BATCH_OVERLAP")) key.insert(key.end(), e, e + std::strlen(e));
+  if (const char* e = std::getenv("F2M_BATCH_REGS")) key.insert(key.end(), e, e + std::strlen(e));
   key.push_back(c->b_clk_on ? 1 : 0);
   {  // the warm-start source and every workspace are captured kernel arguments
     auto addp = [&](const void* p) {
@@ -1415,6 +1527,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   a.efoff = c->b_efoff.as<int32_t>();
   a.ef = c->b_ef.as<ESlot>();
   a.fcap = 0;  // set per bucket
+  a.sweep_regs = env_int("F2M_BATCH_REGS", 1) != 0 ? 1 : 0;
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;

@@ -112,6 +112,17 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
     monkeypatch.delenv("F2M_BATCH_LPN")
     monkeypatch.delenv("F2M_BATCH_BS")
     monkeypatch.delenv("F2M_BATCH_LDS")
+    # runs of sweeps with the lists held in registers (the default where the
+    # instance fits two nodes per thread) against the pass that reads them
+    # from shared memory every sweep
+    for regs in ("0", "1"):
+        monkeypatch.setenv("F2M_BATCH_REGS", regs)
+        say("batch sweeps with lists in registers", regs)
+        sts = ctx.solve_batched(b)
+        res = ctx.batch_result(b)
+        same_stats(sts, ref[0])
+        assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    monkeypatch.delenv("F2M_BATCH_REGS")
     # captured graph: first call captures, later calls replay; eager work between
     outs = []
     for rep in range(3):
