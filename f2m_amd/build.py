@@ -52,15 +52,24 @@ def lost_spills(asm_text: str):
     A spill placed ahead of that restore runs for the narrowed (often empty)
     set of lanes, so the other lanes' values are lost; the compiler has been
     seen to place spills there.  Returns (kernel, line number, instruction)
-    for every such spill."""
+    for every such spill.
+
+    Spills inside a whole-wave region are safe: the compiler saves the VGPR
+    that holds spilled SGPRs (v_writelane) with every lane on, bracketed by
+    `s_or_saveexec_b64 s[..], -1` and `s_mov_b64 exec, s[..]`."""
     out = []
     kernel = None
     block = []
     def flush():
         restore = next((i for i, (_, t) in enumerate(block) if t.startswith("s_or_b64 exec, exec")), None)
         if restore is not None:
+            wwm = False
             for ln, t in block[:restore]:
-                if _SPILL.search(t) and t.startswith(("scratch_", "buffer_", "v_accvgpr")):
+                if t.startswith("s_or_saveexec_b64") and t.replace(" ", "").endswith(",-1"):
+                    wwm = True
+                elif t.startswith("s_mov_b64 exec,"):
+                    wwm = False
+                elif not wwm and _SPILL.search(t) and t.startswith(("scratch_", "buffer_", "v_accvgpr")):
                     out.append((kernel, ln, t))
     for ln, raw in enumerate(asm_text.split("\n"), 1):
         line = raw.strip()

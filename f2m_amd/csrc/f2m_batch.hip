@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "f2m_exact_device.h"
@@ -286,23 +287,75 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 
 // A run of cnt sweeps on fixed lists (sweep_n) for an instance of at most
 // NP*BS nodes whose potentials (both buffers) and active lists sit in shared
-// memory.  Thread u owns nodes u, u + BS, ...  Each node's first PASS_U
-// active slots are read once for the whole run and kept in registers: D and
-// the byte offsets of the two halves that dval_h adds to it, the lower id's
-// first (padded slots: D = +inf and the node itself twice).  So a sweep
-// reads only the halves, and the slots past PASS_U, from shared memory, and
-// never writes global memory.  During the run the two shared buffers hold
-// the halves 0.5*F; each owner keeps its nodes' full potentials in
-// registers and writes them back at the end, to the shared copy of F[c] and
-// to F[c] in global memory.  0.5*F is exact, and the operands and arithmetic
-// of every node update are wg_pass's, so the bits are the same.  Returns the
-// max |g| of the last sweep when want_max (else 0); barriers inside.
-template <int BS, int NP>
+// memory.  Each node's first PASS_U active slots are read once for the whole
+// run and kept in registers: D and the byte offsets of the two halves that
+// dval_h adds to it, the lower id's first (padded slots: D = +inf and the
+// node itself twice).  So a sweep reads only the halves, and the slots past
+// PASS_U, from shared memory, and never writes global memory.
+//  - The thread's NP nodes are updated together: their 2*NP*PASS_U reads are
+//    issued before the first is used (a missing node reads slot 0 with
+//    D = +inf and is not written).
+//  - SORT: nodes owned in the order of their list lengths (a counting sort at
+//    the start of the run; thread u takes ranks u, u + BS, ...), so that the
+//    waves whose lists all fit the registers skip the loop over further
+//    slots and the others loop to similar lengths.  It pays for lists longer
+//    than PASS_U on average (the block_GLOB lists); for shorter ones the
+//    scattered reads of the owners' own halves cost more than it saves.
+//    Which thread updates a node does not change its operands.
+//  - During the run the two shared buffers hold the halves 0.5*F (sweep i
+//    reads buffer i & 1); the owner keeps its nodes' full potentials in
+//    registers and writes them back at the end, to the shared copy of F[c]
+//    and to F[c] in global memory.  The other shared buffer is left
+//    undefined: no pass reads it before writing it.
+//  - 0.5*F is exact, and the operands and arithmetic of every node update
+//    are wg_pass's, so the bits are the same.
+// Returns the max |g| of the last sweep when want_max (else 0); barriers
+// inside.  Shared scratch of the sort: the buffer F[c0 ^ 1] (n doubles,
+// undefined on entry), which holds its KSORT + n ints when n >= KSORT.
+constexpr int KSORT = 64;  // list lengths sorted by; longer lists share the last bin
+template <int BS, int NP, bool SORT>
 __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0, int cnt, bool use_frozen,
                                                          double* sh, bool want_max) {
   const int n = P.n;
   const double* eD = b_dyn + P.lofs;
   const uint16_t* eN16 = reinterpret_cast<const uint16_t*>(eD + P.ltot);
+  const double* Fc = b_dyn + c0 * n;
+  int* hist = reinterpret_cast<int*>(b_dyn + (c0 ^ 1) * n);
+  int* rank = hist + KSORT;  // n entries: node of each rank
+  if (SORT) {  // counting sort of the nodes by list length, in the free buffer
+    for (int k = threadIdx.x; k < KSORT; k += BS) hist[k] = 0;
+    __syncthreads();
+    int len[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int v = threadIdx.x + p * BS;
+      len[p] = 0;
+      if (v < n) {
+        len[p] = min(P.efoff[v + 1] - P.efoff[v], KSORT - 1);
+        atomicAdd(&hist[len[p]], 1);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the KSORT counts by one wave
+      static_assert(KSORT == 64, "one count per lane");
+      const int cnt_l = hist[threadIdx.x];
+      int x = cnt_l;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if ((int)threadIdx.x >= off) x += y;
+      }
+      hist[threadIdx.x] = x - cnt_l;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int v = threadIdx.x + p * BS;
+      if (v < n) rank[atomicAdd(&hist[len[p]], 1)] = v;
+    }
+    __syncthreads();
+  }
+  int vv[NP];
   double Dk[NP][PASS_U];
   uint32_t oa[NP][PASS_U], ob[NP][PASS_U];
   int bb[NP], L[NP];
@@ -310,7 +363,9 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
   bool fz[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
-    const int v = threadIdx.x + p * BS;
+    const int r = threadIdx.x + p * BS;
+    const int v = (SORT && r < n) ? rank[r] : r;
+    vv[p] = v;
     bb[p] = 0;
     L[p] = 0;
     fv[p] = 0.0;
@@ -318,63 +373,88 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
     if (v < n) {
       bb[p] = P.efoff[v];
       L[p] = P.efoff[v + 1] - bb[p];
-      fv[p] = b_dyn[c0 * n + v];
+      fv[p] = Fc[v];
       fz[p] = use_frozen && P.frozen[v] != 0;
     }
 #pragma unroll
     for (int k = 0; k < PASS_U; ++k) {
-      int w = v;
+      int w = v < n ? v : 0;  // a missing node reads slot 0, D = +inf
       Dk[p][k] = __builtin_inf();
       if (k < L[p]) {
         Dk[p][k] = eD[bb[p] + k];
         w = (int)eN16[bb[p] + k];
       }
-      const bool lo = v < w;
-      oa[p][k] = (uint32_t)(lo ? v : w) * 8u;
-      ob[p][k] = (uint32_t)(lo ? w : v) * 8u;
+      const int u = v < n ? v : 0;
+      const bool lo = u < w;
+      oa[p][k] = (uint32_t)(lo ? u : w) * 8u;
+      ob[p][k] = (uint32_t)(lo ? w : u) * 8u;
     }
-    // the owner's entry from here on holds the half (read above by its owner only)
-    if (v < n) b_dyn[c0 * n + v] = 0.5 * fv[p];
   }
+  __syncthreads();  // every read of F[c0] and of the ranks is done
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+    if (vv[p] < n) b_dyn[vv[p]] = 0.5 * fv[p];
   __syncthreads();
-  int c = c0;
   double gmax = 0.0;
-  for (int i = 0; i < cnt; ++i) {
-    const double* srcd = b_dyn + c * n;
-    const char* src = reinterpret_cast<const char*>(srcd);
-    double* dst = b_dyn + (c ^ 1) * n;
-    const bool last = i == cnt - 1;
+  // one sweep reading the halves of buffer Q, writing buffer Q ^ 1
+  auto sweep = [&](auto qc, auto lastc) {
+    constexpr int Q = decltype(qc)::value;
+    constexpr bool LAST = decltype(lastc)::value;
+    const char* src = reinterpret_cast<const char*>(b_dyn + Q * n);
+    const double* srcd = b_dyn + Q * n;
+    double* dst = b_dyn + (Q ^ 1) * n;
+    double d[NP][PASS_U];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int k = 0; k < PASS_U; ++k)
+        d[p][k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
+                  *reinterpret_cast<const double*>(src + ob[p][k]);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-      const int v = threadIdx.x + p * BS;
-      if (v < n) {
-        double d[PASS_U];
-#pragma unroll
-        for (int k = 0; k < PASS_U; ++k)
-          d[k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
-                 *reinterpret_cast<const double*>(src + ob[p][k]);
-        Top3 t = top3_of4(d[0], d[1], d[2], d[3]);  // the same as pushing them
+      const int v = vv[p];
+      Top3 t = top3_of4(d[p][0], d[p][1], d[p][2], d[p][3]);  // the same as pushing them
+      if (L[p] > PASS_U) {
         const double hv = 0.5 * fv[p];
         for (int k = PASS_U; k < L[p]; ++k) {
           const int w = (int)eN16[bb[p] + k];
           t.push(dval_h(eD[bb[p] + k], hv, srcd[w], v, w));
         }
+      }
+      if (v < n) {
         const double g = t.grad();
         const double f = fz[p] ? fv[p] : fv[p] + g;
         fv[p] = f;
         dst[v] = 0.5 * f;
-        if (last) gmax = fmax(gmax, fabs(g));
+        if (LAST) gmax = fmax(gmax, fabs(g));
       }
     }
     __syncthreads();
-    c ^= 1;
+  };
+  using Q0 = std::integral_constant<int, 0>;
+  using Q1 = std::integral_constant<int, 1>;
+  using NL = std::integral_constant<bool, false>;
+  using LA = std::integral_constant<bool, true>;
+  int i = 0;
+  for (; i + 2 < cnt; i += 2) {
+    sweep(Q0{}, NL{});
+    sweep(Q1{}, NL{});
   }
-  // full potentials back (the owner's entries; nobody reads them before the
-  // barrier below)
+  if (cnt - i == 2) {
+    sweep(Q0{}, NL{});
+    if (want_max) sweep(Q1{}, LA{});
+    else sweep(Q1{}, NL{});
+  } else if (cnt - i == 1) {
+    if (want_max) sweep(Q0{}, LA{});
+    else sweep(Q0{}, NL{});
+  }
+  // full potentials back, to F[c] (the owner's entries; nobody reads the
+  // shared potentials before the barrier below)
+  const int c = c0 ^ (cnt & 1);
   double* Fg = c ? P.F1 : P.F0;
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
-    const int v = threadIdx.x + p * BS;
+    const int v = vv[p];
     if (v < n) {
       b_dyn[c * n + v] = fv[p];
       Fg[v] = fv[p];
@@ -537,9 +617,14 @@ struct Sched {
     const int fm = fmode;
     int c = cur;
     double m = 0.0;
-    if (regs && cnt > 0 && fm == FS_TWO && P.lofs >= 0 && P.n <= 2 * BS) {
-      m = P.n <= BS ? wg_sweeps_reg<BS, 1>(P, c, cnt, frozen, shm, want_max)
-                    : wg_sweeps_reg<BS, 2>(P, c, cnt, frozen, shm, want_max);
+    if (regs && cnt > 0 && fm == FS_TWO && P.lofs >= 0 && P.n <= 2 * BS && P.n >= KSORT) {
+      // sorted ownership where the lists are longer than the registers hold on average
+      if (P.ltot > PASS_U * P.n)
+        m = P.n <= BS ? wg_sweeps_reg<BS, 1, true>(P, c, cnt, frozen, shm, want_max)
+                      : wg_sweeps_reg<BS, 2, true>(P, c, cnt, frozen, shm, want_max);
+      else
+        m = P.n <= BS ? wg_sweeps_reg<BS, 1, false>(P, c, cnt, frozen, shm, want_max)
+                      : wg_sweeps_reg<BS, 2, false>(P, c, cnt, frozen, shm, want_max);
       cur = c ^ (cnt & 1);
       sweeps += cnt;
       return m;

@@ -32,6 +32,26 @@ def test_lost_spill_is_flagged():
     assert lost_spills(BAD.replace(" ; 8-byte Folded Spill", "")) == []
 
 
+WWM = BAD.replace("""	scratch_store_dwordx2 off, v[24:25], off offset:1088 ; 8-byte Folded Spill
+""", """	v_writelane_b32 v123, s16, 7
+	s_or_saveexec_b64 s[100:101], -1
+	scratch_store_dword off, v123, s33 offset:696 ; 4-byte Folded Spill
+	s_mov_b64 exec, s[100:101]
+""")
+
+
+def test_whole_wave_spill_is_not_flagged():
+    # the save of the VGPR that holds spilled SGPRs runs with every lane on
+    assert WWM != BAD and lost_spills(WWM) == []
+    # the same store outside the whole-wave bracket is flagged
+    outside = WWM.replace("\ts_or_saveexec_b64 s[100:101], -1\n", "").replace("\ts_mov_b64 exec, s[100:101]\n", "")
+    assert len(lost_spills(outside)) == 1
+    # a spill after the bracket closes, still before the restore, is flagged
+    after = WWM.replace("\ts_mov_b64 exec, s[100:101]\n",
+                        "\ts_mov_b64 exec, s[100:101]\n\tscratch_store_dword off, v7, s33 offset:4 ; 4-byte Folded Spill\n")
+    assert len(lost_spills(after)) == 1
+
+
 def test_built_device_code_has_no_lost_spills():
     import glob
     import os
