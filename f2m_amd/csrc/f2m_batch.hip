@@ -764,7 +764,10 @@ struct Sched {
   }
   // shift_comb_IRREGULAR_B (graph_cuda_comb.hpp:365-403) on every comb with
   // irr == 2; type 1 shifts only the smallest violating node of each comb.
-  __device__ void shift(double sv, int type) {
+  // Not inlined: its divergent stores end in a join block, and inlined the
+  // compiler placed a spill of the caller's shift value at the top of that
+  // block, ahead of the exec restore (the build's ISA scan rejects that).
+  __device__ __attribute__((noinline)) void shift(double sv, int type) {
     double* F = I.F[cur];
     fok = false;  // F changes below
     for (int v = threadIdx.x; v < I.n; v += BS) {

@@ -60,11 +60,31 @@ __device__ inline int part_v4_node(int v, int64_t b, int dg, int kk, int32_t* __
                                    const double* __restrict__ F) {
   if (dg < kk) return dg;
   const double Fv = F[v];
-  double thr = dval(D[b], Fv, F[nbr[b]], v, nbr[b]);
-  for (int i = 1; i < kk; ++i) {
-    const int w = nbr[b + i];
-    const double d = dval(D[b + i], Fv, F[w], v, w);
-    thr = d > thr ? d : thr;
+  double thr;
+  if (kk >= 1 && kk <= 8) {  // the first kk slots loaded together
+    int w[8];
+    double dd[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = i < kk ? i : 0;
+      w[i] = nbr[b + k];
+      dd[i] = D[b + k];
+    }
+    thr = dval(dd[0], Fv, F[w[0]], v, w[0]);
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+      if (i < kk) {
+        const double d = dval(dd[i], Fv, F[w[i]], v, w[i]);
+        thr = d > thr ? d : thr;
+      }
+    }
+  } else {
+    thr = dval(D[b], Fv, F[nbr[b]], v, nbr[b]);
+    for (int i = 1; i < kk; ++i) {
+      const int w = nbr[b + i];
+      const double d = dval(D[b + i], Fv, F[w], v, w);
+      thr = d > thr ? d : thr;
+    }
   }
   return qpart_EF(v, b, dg, thr, nbr, D, orig, F);
 }

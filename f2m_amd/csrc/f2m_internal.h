@@ -286,34 +286,67 @@ __device__ inline unsigned long long part_mask(int v, int64_t base, int size, do
 }
 
 // The Hoare partition of qpart_B_loop driven by the predicate bits m of the
-// slots in their current order: i = the first slot from i not at or below
-// the threshold, j = the last slot to j at or below it, swapped while i < j.
-// The scans are bit scans and the bits follow the swaps, so the swaps (and
-// the returned pivot) are those of the step-by-step form.
+// slots in their current order.  Its scans (i: the next slot from i not at or
+// below the threshold, j: the previous slot from j at or below it, swapped
+// while i < j) pair the k-th such slot from the left with the k-th at-or-below
+// slot from the right, for as long as the first lies left of the second, and
+// move past both.  With p = popcount(m), those pairs are exactly the slots
+// above the threshold in [0, p) in ascending order against the slots at or
+// below it in [p, size) in descending order, every slot swapped at most once,
+// and the step-by-step form returns j + 1 = p.  So the swaps are computed from
+// the bits first and carried out in groups of four whose loads are issued
+// together: the same swaps, with one memory round trip per group instead of
+// one per swap.
 __device__ inline int qpart_mask(int64_t base, int size, unsigned long long m, int* __restrict__ nbr,
                                  double* __restrict__ D, int* __restrict__ orig) {
   const unsigned long long full = size >= 64 ? ~0ull : ((1ull << size) - 1ull);
   m &= full;
-  int i = 0, j = size - 1;
-  while (i < j) {
-    const unsigned long long z = ~m & full & (i >= 64 ? 0ull : (~0ull << i));  // not at or below, from i
-    i = z ? __builtin_ctzll(z) : size;
-    const unsigned long long o = m & (j >= 63 ? ~0ull : ((2ull << j) - 1ull));  // at or below, up to j
-    j = o ? 63 - __builtin_clzll(o) : -1;
-    if (i < j) {
-      const int64_t si = base + i, sj = base + j;
-      const int tn = nbr[si], to = orig[si];
-      const double tD = D[si];
-      nbr[si] = nbr[sj];
-      D[si] = D[sj];
-      orig[si] = orig[sj];
-      nbr[sj] = tn;
-      D[sj] = tD;
-      orig[sj] = to;
-      m = (m | (1ull << i)) & ~(1ull << j);
+  const int p = __popcll(m);
+  const unsigned long long left = p >= 64 ? ~0ull : ((1ull << p) - 1ull);
+  unsigned long long A = ~m & left;  // above the threshold, left of the pivot (ascending)
+  unsigned long long B = m & ~left;  // at or below it, right of the pivot (descending); |A| = |B|
+  constexpr int G = 4;
+  while (A) {
+    int si[G], sj[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      si[q] = -1;
+      sj[q] = -1;
+      if (A) {
+        si[q] = __builtin_ctzll(A);
+        sj[q] = 63 - __builtin_clzll(B);
+        A &= A - 1;
+        B &= ~(1ull << sj[q]);
+      }
+    }
+    int tn[G], to[G], un[G], uo[G];
+    double tD[G], uD[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      if (si[q] >= 0) {
+        const int64_t a = base + si[q], b = base + sj[q];
+        tn[q] = nbr[a];
+        to[q] = orig[a];
+        tD[q] = D[a];
+        un[q] = nbr[b];
+        uo[q] = orig[b];
+        uD[q] = D[b];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      if (si[q] >= 0) {
+        const int64_t a = base + si[q], b = base + sj[q];
+        nbr[a] = un[q];
+        orig[a] = uo[q];
+        D[a] = uD[q];
+        nbr[b] = tn[q];
+        orig[b] = to[q];
+        D[b] = tD[q];
+      }
     }
   }
-  return j + 1;
+  return p;
 }
 
 __device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
