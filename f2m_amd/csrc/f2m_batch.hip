@@ -287,19 +287,21 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 
 // A run of cnt sweeps on fixed lists (sweep_n) for an instance of at most
 // NP*BS nodes whose potentials (both buffers) and active lists sit in shared
-// memory.  Each node's first PASS_U active slots are read once for the whole
+// memory.  Each node's first RK active slots are read once for the whole
 // run and kept in registers: D and the byte offsets of the two halves that
 // dval_h adds to it, the lower id's first (padded slots: D = +inf and the
 // node itself twice).  So a sweep reads only the halves, and the slots past
-// PASS_U, from shared memory, and never writes global memory.
-//  - The thread's NP nodes are updated together: their 2*NP*PASS_U reads are
+// RK, from shared memory, and never writes global memory.  RK = 4 (PASS_U):
+// at two nodes per thread of a 1024-thread workgroup (128 registers) 6 or 8
+// slots in registers made the C5 solve slower (19.9 / 25.8 against 18.0 ms).
+//  - The thread's NP nodes are updated together: their 2*NP*RK reads are
 //    issued before the first is used (a missing node reads slot 0 with
 //    D = +inf and is not written).
 //  - SORT: nodes owned in the order of their list lengths (a counting sort at
 //    the start of the run; thread u takes ranks u, u + BS, ...), so that the
 //    waves whose lists all fit the registers skip the loop over further
 //    slots and the others loop to similar lengths.  It pays for lists longer
-//    than PASS_U on average (the block_GLOB lists); for shorter ones the
+//    than RK on average (the block_GLOB lists); for shorter ones the
 //    scattered reads of the owners' own halves cost more than it saves.
 //    Which thread updates a node does not change its operands.
 //  - During the run the two shared buffers hold the halves 0.5*F (sweep i
@@ -313,7 +315,7 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 // inside.  Shared scratch of the sort: the buffer F[c0 ^ 1] (n doubles,
 // undefined on entry), which holds its KSORT + n ints when n >= KSORT.
 constexpr int KSORT = 64;  // list lengths sorted by; longer lists share the last bin
-template <int BS, int NP, bool SORT>
+template <int BS, int NP, bool SORT, int RK>
 __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0, int cnt, bool use_frozen,
                                                          double* sh, bool want_max) {
   const int n = P.n;
@@ -356,8 +358,8 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
     __syncthreads();
   }
   int vv[NP];
-  double Dk[NP][PASS_U];
-  uint32_t oa[NP][PASS_U], ob[NP][PASS_U];
+  double Dk[NP][RK];
+  uint32_t oa[NP][RK], ob[NP][RK];
   int bb[NP], L[NP];
   double fv[NP];
   bool fz[NP];
@@ -377,7 +379,7 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
       fz[p] = use_frozen && P.frozen[v] != 0;
     }
 #pragma unroll
-    for (int k = 0; k < PASS_U; ++k) {
+    for (int k = 0; k < RK; ++k) {
       int w = v < n ? v : 0;  // a missing node reads slot 0, D = +inf
       Dk[p][k] = __builtin_inf();
       if (k < L[p]) {
@@ -388,6 +390,7 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
       const bool lo = u < w;
       oa[p][k] = (uint32_t)(lo ? u : w) * 8u;
       ob[p][k] = (uint32_t)(lo ? w : u) * 8u;
+
     }
   }
   __syncthreads();  // every read of F[c0] and of the ranks is done
@@ -403,20 +406,22 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
     const char* src = reinterpret_cast<const char*>(b_dyn + Q * n);
     const double* srcd = b_dyn + Q * n;
     double* dst = b_dyn + (Q ^ 1) * n;
-    double d[NP][PASS_U];
+    double d[NP][RK];
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
-      for (int k = 0; k < PASS_U; ++k)
+      for (int k = 0; k < RK; ++k)
         d[p][k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
                   *reinterpret_cast<const double*>(src + ob[p][k]);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       const int v = vv[p];
       Top3 t = top3_of4(d[p][0], d[p][1], d[p][2], d[p][3]);  // the same as pushing them
-      if (L[p] > PASS_U) {
+#pragma unroll
+      for (int k = 4; k < RK; ++k) t.push(d[p][k]);
+      if (L[p] > RK) {
         const double hv = 0.5 * fv[p];
-        for (int k = PASS_U; k < L[p]; ++k) {
+        for (int k = RK; k < L[p]; ++k) {
           const int w = (int)eN16[bb[p] + k];
           t.push(dval_h(eD[bb[p] + k], hv, srcd[w], v, w));
         }
@@ -620,11 +625,11 @@ struct Sched {
     if (regs && cnt > 0 && fm == FS_TWO && P.lofs >= 0 && P.n <= 2 * BS && P.n >= KSORT) {
       // sorted ownership where the lists are longer than the registers hold on average
       if (P.ltot > PASS_U * P.n)
-        m = P.n <= BS ? wg_sweeps_reg<BS, 1, true>(P, c, cnt, frozen, shm, want_max)
-                      : wg_sweeps_reg<BS, 2, true>(P, c, cnt, frozen, shm, want_max);
+        m = P.n <= BS ? wg_sweeps_reg<BS, 1, true, PASS_U>(P, c, cnt, frozen, shm, want_max)
+                      : wg_sweeps_reg<BS, 2, true, PASS_U>(P, c, cnt, frozen, shm, want_max);
       else
-        m = P.n <= BS ? wg_sweeps_reg<BS, 1, false>(P, c, cnt, frozen, shm, want_max)
-                      : wg_sweeps_reg<BS, 2, false>(P, c, cnt, frozen, shm, want_max);
+        m = P.n <= BS ? wg_sweeps_reg<BS, 1, false, PASS_U>(P, c, cnt, frozen, shm, want_max)
+                      : wg_sweeps_reg<BS, 2, false, PASS_U>(P, c, cnt, frozen, shm, want_max);
       cur = c ^ (cnt & 1);
       sweeps += cnt;
       return m;
