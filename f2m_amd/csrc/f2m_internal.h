@@ -296,8 +296,10 @@ __device__ inline unsigned long long part_mask(int v, int64_t base, int size, do
 // and the step-by-step form returns j + 1 = p.  So the swaps are computed from
 // the bits first and carried out in groups of four whose loads are issued
 // together: the same swaps, with one memory round trip per group instead of
-// one per swap.
-__device__ inline int qpart_mask(int64_t base, int size, unsigned long long m, int* __restrict__ nbr,
+// one per swap.  Not inlined: inside the batched solve's large function the
+// group's registers were spilled, with reloads placed ahead of an exec
+// restore (the build's ISA scan rejects that).
+__device__ __attribute__((noinline)) int qpart_mask(int64_t base, int size, unsigned long long m, int* __restrict__ nbr,
                                  double* __restrict__ D, int* __restrict__ orig) {
   const unsigned long long full = size >= 64 ? ~0ull : ((1ull << size) - 1ull);
   m &= full;
