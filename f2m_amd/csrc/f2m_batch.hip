@@ -1181,6 +1181,15 @@ __global__ void __launch_bounds__(BS, BS == 256 ? 3 : 1) b_persistent(BArgs a) {
   }
 }
 
+// the potentials of instance blockIdx.x from its current buffer
+__global__ void b_gather_F(const int64_t* __restrict__ node_off, const int32_t* __restrict__ cur,
+                           const double* __restrict__ F0, const double* __restrict__ F1, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  const int64_t a = node_off[i], e = node_off[i + 1];
+  const double* src = cur[i] ? F1 : F0;
+  for (int64_t v = a + threadIdx.x; v < e; v += blockDim.x) out[v] = src[v];
+}
+
 __global__ void b_zero_i32(int32_t* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1698,16 +1707,23 @@ f2m_status f2m_batch_result_copy(f2m_ctx* c, double* F, double* x, double* objec
   SolveWs& w = c->bws;
   const int n_inst = c->b_n_inst;
   if (F) {
-    // per instance the current buffer may be F0 or F1
-    std::vector<int32_t> cur(n_inst);
-    std::vector<int64_t> off(n_inst + 1);
-    F2M_HIP(c, hipMemcpy(cur.data(), c->b_cur.p, n_inst * 4ull, hipMemcpyDeviceToHost));
-    F2M_HIP(c, hipMemcpy(off.data(), c->b_node_off.p, (n_inst + 1) * 8ull, hipMemcpyDeviceToHost));
-    for (int i = 0; i < n_inst; ++i) {
-      const double* src = (cur[i] ? w.F1 : w.F0).as<double>() + off[i];
-      F2M_HIP(c, hipMemcpyAsync(F + off[i], src, (off[i + 1] - off[i]) * 8ull, hipMemcpyDefault, c->stream));
+    // per instance the current buffer may be F0 or F1: one gather launch, into
+    // F itself when it is device memory, else into a device buffer then copied
+    hipPointerAttribute_t pa;
+    const bool dev = hipPointerGetAttributes(&pa, F) == hipSuccess && pa.type == hipMemoryTypeDevice &&
+                     pa.device == c->device;
+    (void)hipGetLastError();  // a host pointer may leave an error behind
+    double* dst = F;
+    if (!dev) {
+      F2M_HIP(c, c->b_fout.reserve((size_t)c->b_n_total * 8));
+      dst = c->b_fout.as<double>();
     }
-    F2M_HIP(c, hipStreamSynchronize(c->stream));
+    hipLaunchKernelGGL(b_gather_F, dim3(n_inst), dim3(256), 0, c->stream, c->b_node_off.as<int64_t>(),
+                       (const int32_t*)c->b_cur.as<int32_t>(), (const double*)w.F0.as<double>(),
+                       (const double*)w.F1.as<double>(), dst);
+    F2M_LAUNCH_CHECK(c);
+    if (!dev) F2M_HIP(c, copy_out(c, F, dst, (size_t)c->b_n_total * 8));
+    else F2M_HIP(c, hipStreamSynchronize(c->stream));
   }
   F2M_HIP(c, copy_out(c, x, w.xc.p, c->b_nslots * 8ull));
   if (objective) {

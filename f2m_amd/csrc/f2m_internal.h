@@ -65,6 +65,8 @@ struct f2m_ctx {
   // kNN scratch
   f2m::DevBuf k_x, k_y, k_cell, k_cstart, k_order, k_sx, k_sy, k_knn, k_kcost, k_rev, k_cur,
       k_deg, k_red, k_scan, k_geom;
+  // symmetrisation scratch: reverse-only flags per directed edge, unsorted slots (owner, nbr, cost)
+  f2m::DevBuf k_flag, k_town, k_tnbr, k_tcost;
   // single-pass scan: per-tile status words, tagged with the call's epoch
   f2m::DevBuf k_lbst;
   uint32_t k_lb_epoch = 0;
@@ -77,6 +79,7 @@ struct f2m_ctx {
   f2m::SolveWs ws;
   // batched storage
   f2m::DevBuf b_node_off, b_rowptr, b_nbr, b_cost, b_order, b_stats, b_cur, b_knn, b_kc, b_ef, b_efoff, b_finit;
+  f2m::DevBuf b_fout;                              // potentials of the last batch, gathered per instance
   f2m::SolveWs bws;
   void* b_plan = nullptr;                          // captured launch plan (f2m_batch.hip)
   struct TuneEntry {

@@ -306,19 +306,27 @@ __global__ void __launch_bounds__(T) k_knn_b(int64_t n, int n_inst, int K, int m
   }
 }
 
+// v among w's K directed neighbours: every entry read, no early exit, so
+// that the row's loads are issued together
 __device__ __forceinline__ bool in_list(const int32_t* __restrict__ knn, int64_t w, int K, int32_t v) {
   const int32_t* l = knn + w * K;
-  for (int j = 0; j < K; ++j)
-    if (l[j] == v) return true;
-  return false;
+  bool f = false;
+#pragma unroll 4
+  for (int j = 0; j < K; ++j) f |= l[j] == v;
+  return f;
 }
 
-__global__ void k_rev_count(int64_t n, int K, const int32_t* __restrict__ knn, int32_t* __restrict__ rev) {
+// Directed edge i = (v -> w): flag[i] = 1 when v is not among w's neighbours
+// (the symmetric graph then needs the reverse slot w -> v), counted in rev[w].
+__global__ void k_rev_flag(int64_t n, int K, const int32_t* __restrict__ knn, int32_t* __restrict__ rev,
+                           uint8_t* __restrict__ flag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * K) return;
   int32_t v = (int32_t)(i / K);
   int32_t w = knn[i];
-  if (!in_list(knn, w, K, v)) atomicAdd(&rev[w], 1);
+  const bool need = !in_list(knn, w, K, v);
+  flag[i] = need ? 1 : 0;
+  if (need) atomicAdd(&rev[w], 1);
 }
 
 __global__ void k_deg(int64_t n, int K, const int32_t* __restrict__ rev, int32_t* __restrict__ deg) {
@@ -326,10 +334,12 @@ __global__ void k_deg(int64_t n, int K, const int32_t* __restrict__ rev, int32_t
   if (v < n) deg[v] = K + rev[v];
 }
 
+// Unsorted slots: v's K directed neighbours at rp[v] + j, then the reverse
+// slots of the edges flagged, in atomic order; town = the slot's node.
 template <class RP>
 __global__ void k_fill(int64_t n, int K, const int32_t* __restrict__ knn, const double* __restrict__ kc,
-                       const RP* __restrict__ rp, int32_t* __restrict__ cur, int32_t* __restrict__ nbr,
-                       double* __restrict__ cost) {
+                       const uint8_t* __restrict__ flag, const RP* __restrict__ rp, int32_t* __restrict__ cur,
+                       int32_t* __restrict__ town, int32_t* __restrict__ nbr, double* __restrict__ cost) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * K) return;
   int32_t v = (int32_t)(i / K);
@@ -337,35 +347,38 @@ __global__ void k_fill(int64_t n, int K, const int32_t* __restrict__ knn, const 
   int32_t w = knn[i];
   double c = kc[i];
   int64_t s = (int64_t)rp[v] + j;
+  town[s] = v;
   nbr[s] = w;
   cost[s] = c;
-  if (!in_list(knn, w, K, v)) {
+  if (flag[i]) {
     int64_t q = (int64_t)rp[w] + K + atomicAdd(&cur[w], 1);
+    town[q] = w;
     nbr[q] = v;
     cost[q] = c;
   }
 }
 
-// Sort each node's slots by (cost, nbr): the first K are already sorted, the
-// reverse-only entries are inserted.
+// Each node's slots in (cost, nbr) order: slot s goes to its rank among its
+// node's slots (the neighbours of a node are distinct, so the keys are, and
+// the ranks are a permutation).  One thread per slot; the node's slots are
+// read from the unsorted copy.
 template <class RP>
-__global__ void k_sort_seg(int64_t n, const RP* __restrict__ rp, int32_t* __restrict__ nbr,
-                           double* __restrict__ cost) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  int64_t b = rp[v], e = rp[v + 1];
-  for (int64_t i = b + 1; i < e; ++i) {
-    double c = cost[i];
-    int32_t w = nbr[i];
-    int64_t j = i;
-    while (j > b && (c < cost[j - 1] || (c == cost[j - 1] && w < nbr[j - 1]))) {
-      cost[j] = cost[j - 1];
-      nbr[j] = nbr[j - 1];
-      --j;
-    }
-    cost[j] = c;
-    nbr[j] = w;
+__global__ void k_rank_sort(int64_t m, const RP* __restrict__ rp, const int32_t* __restrict__ town,
+                            const int32_t* __restrict__ tnbr, const double* __restrict__ tcost,
+                            int32_t* __restrict__ nbr, double* __restrict__ cost) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const int32_t v = town[s];
+  const int64_t b = rp[v], e = rp[v + 1];
+  const double c = tcost[s];
+  const int32_t w = tnbr[s];
+  int64_t r = 0;
+  for (int64_t t = b; t < e; ++t) {
+    const double ct = tcost[t];
+    r += (ct < c || (ct == c && tnbr[t] < w)) ? 1 : 0;
   }
+  nbr[b + r] = w;
+  cost[b + r] = c;
 }
 
 // Directed kNN of one instance into (out_id, out_c) [n*K]; ids + id_offset.
@@ -480,9 +493,11 @@ static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, const int32_t* knn, c
                              DevBuf& nbb, DevBuf& cb, int64_t* nslots_out) {
   F2M_HIP(c, c->k_rev.reserve(n * 4));
   F2M_HIP(c, c->k_deg.reserve(n * 4));
+  F2M_HIP(c, c->k_flag.reserve(n * K));
   int32_t* rev = c->k_rev.as<int32_t>();
+  uint8_t* flag = c->k_flag.as<uint8_t>();
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
-  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K, knn, rev);
+  hipLaunchKernelGGL(k_rev_flag, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K, knn, rev, flag);
   hipLaunchKernelGGL(k_deg, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, K, rev, c->k_deg.as<int32_t>());
   F2M_LAUNCH_CHECK(c);
   F2M_HIP(c, rpb.reserve((n + 1) * sizeof(RP)));
@@ -494,11 +509,16 @@ static f2m_status symmetrise(f2m_ctx* c, int64_t n, int K, const int32_t* knn, c
   int64_t m = (int64_t)total;
   F2M_HIP(c, nbb.reserve(m * 4));
   F2M_HIP(c, cb.reserve(m * 8));
+  F2M_HIP(c, c->k_town.reserve(m * 4));
+  F2M_HIP(c, c->k_tnbr.reserve(m * 4));
+  F2M_HIP(c, c->k_tcost.reserve(m * 8));
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, rev, n);
   hipLaunchKernelGGL(k_fill<RP>, dim3(grid_for(n * K, 256)), dim3(256), 0, c->stream, n, K, knn, kc,
-                     (const RP*)rpb.p, rev, nbb.as<int32_t>(), cb.as<double>());
-  hipLaunchKernelGGL(k_sort_seg<RP>, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, n, (const RP*)rpb.p,
-                     nbb.as<int32_t>(), cb.as<double>());
+                     (const uint8_t*)flag, (const RP*)rpb.p, rev, c->k_town.as<int32_t>(), c->k_tnbr.as<int32_t>(),
+                     c->k_tcost.as<double>());
+  hipLaunchKernelGGL(k_rank_sort<RP>, dim3(grid_for(m, 256)), dim3(256), 0, c->stream, m, (const RP*)rpb.p,
+                     (const int32_t*)c->k_town.as<int32_t>(), (const int32_t*)c->k_tnbr.as<int32_t>(),
+                     (const double*)c->k_tcost.as<double>(), nbb.as<int32_t>(), cb.as<double>());
   F2M_LAUNCH_CHECK(c);
   *nslots_out = m;
   return F2M_OK;
