@@ -798,7 +798,7 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles, int nxcd) {
   return x * q + min(x, r) + i;
 }
 
-template <int S, bool FROZEN, int TPP, int W>
+template <int S, bool FROZEN, int TPP, int W, bool SELH = false>
 __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hdr, const int4* __restrict__ tnode,
                                                    const TSlot* __restrict__ tslot, const TSlot* __restrict__ tovf,
                                                    const double* __restrict__ Fin, double* __restrict__ Fout,
@@ -893,6 +893,13 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
   // (padded slots: the node itself twice, D = +inf).  The n = 100k
   // Gaussian-mixture solve: 21.73 -> 20.99 ms against selecting the halves
   // by value each sweep.
+  // SELH: instead, oa = the neighbour's offset, ob = 1 when the owner has
+  // the lower id, and the owner's half comes from its register, chosen by
+  // selects.  Used for the block_GLOB lists, whose records are ordered by
+  // list length, so the owners' halves sit at scattered local indices and
+  // their shared-memory reads conflict: n = 100k Gaussian mixture 20.54 ->
+  // 20.40 ms.  For the block_EF lists (owners' halves consecutive) the
+  // selects cost more than the reads: 20.34 against 20.42 ms.
   uint32_t oa[NP][TPP], ob[NP][TPP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -901,8 +908,8 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
     for (int k = 0; k < TPP; ++k) {
       const uint32_t nb = t_loc<W>(sl[p][k].loc) * 8u;
       const bool lo = nd[p].x < sl[p][k].nbr;
-      oa[p][k] = lo ? own : nb;
-      ob[p][k] = lo ? nb : own;
+      oa[p][k] = SELH ? nb : (lo ? own : nb);
+      ob[p][k] = SELH ? (lo ? 1u : 0u) : (lo ? nb : own);
     }
   }
   // sweeps 2..S from shared memory: sweep q for the local nodes of level <= S-q
@@ -920,10 +927,19 @@ __global__ void __launch_bounds__(W) s_sweep_tile(const int32_t* __restrict__ hd
         const double f0 = fv[p], hv = 0.5 * f0;
         const char* sb = reinterpret_cast<const char*>(src);
         double d[TPP];  // dval_h of every padded slot
+        if (SELH) {  // the owner's half from its register, the neighbour's from shared memory
+#pragma unroll
+          for (int k = 0; k < TPP; ++k) {
+            const double hw = *reinterpret_cast<const double*>(sb + oa[p][k]);
+            const bool lo = (ob[p][k] & 1u) != 0;
+            d[k] = (sl[p][k].D + (lo ? hv : hw)) + (lo ? hw : hv);
+          }
+        } else {
 #pragma unroll
         for (int k = 0; k < TPP; ++k)
           d[k] = (sl[p][k].D + *reinterpret_cast<const double*>(sb + oa[p][k])) +
                  *reinterpret_cast<const double*>(sb + ob[p][k]);
+        }
         Top3 t3 = top3_of4(d[0], d[1], d[2], d[3]);  // the same as pushing them
 #pragma unroll
         for (int k = 4; k < TPP; ++k) t3.push(d[k]);
@@ -1395,6 +1411,7 @@ struct Runner {
   int depth_next[2] = {0, 0};
   int pending_kind = 0;
   int tpp = 4;    // padded slots per local node on the current lists
+  bool selh = true;  // block_GLOB tile sweeps take the owner's half from a register (F2M_TILE_SELH)
   int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
   // with profiling on: sweeps run on the current lists not yet counted, and a
@@ -1495,6 +1512,7 @@ struct Runner {
       auto shape = [](int v) { return v == TW_EF ? TW_EF : TW_GLOB; };
       tw_ef = shape(env_int("F2M_TILE_W", TW_GLOB));
       tw_glob = shape(env_int("F2M_TILE_W_GLOB", TW_GLOB));
+      selh = env_int("F2M_TILE_SELH", 1) != 0;
       // tile nodes of a workgroup shape: the same whole number of tiles on every CU, as large as
       // the shape's cap allows (256 nodes for 384 threads, 512 for 768)
       auto balanced = [&](int tw_) {
@@ -1784,7 +1802,10 @@ struct Runner {
                            (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(), Fo(),
                            (const uint8_t*)w.frozen.as<uint8_t>(), mslots, tile_xcd);
     };
-    if (tpp == 6 && tw == TW_GLOB) {
+    if (tpp == 6 && tw == TW_GLOB && selh) {
+      if (frozen) go(s_sweep_tile<S, true, 6, TW_GLOB, true>);
+      else go(s_sweep_tile<S, false, 6, TW_GLOB, true>);
+    } else if (tpp == 6 && tw == TW_GLOB) {
       if (frozen) go(s_sweep_tile<S, true, 6, TW_GLOB>);
       else go(s_sweep_tile<S, false, 6, TW_GLOB>);
     } else if (tw == TW_GLOB) {

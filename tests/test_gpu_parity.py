@@ -175,6 +175,13 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
     assert np.array_equal(ctx.result()["F"], res["F"]) and np.array_equal(ctx.result()["x"], res["x"])
     monkeypatch.delenv("F2M_PART_LANES")
+    # block_GLOB tile sweeps reading the owner's half from shared memory instead of a register
+    monkeypatch.setenv("F2M_TILE_SELH", "0")
+    say("solve C2 Morton, F2M_TILE_SELH 0")
+    s2 = ctx.solve(g)
+    assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
+    assert np.array_equal(ctx.result()["F"], res["F"])
+    monkeypatch.delenv("F2M_TILE_SELH")
     # nodes per tile of the block_GLOB and block_EF lists; their workgroup shapes (384 threads
     # for 768 local nodes, 768 for 1536)
     for tg, te, wg, we in (("256", "256", "768", "384"), ("64", "128", "384", "768"), ("196", "196", "384", "384"),
