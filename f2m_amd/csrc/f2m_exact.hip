@@ -135,20 +135,37 @@ __global__ void x_zero_ring(int32_t* cnt) {
 
 // excesses of every node (COUNT; else as maintained) and the active lists
 // of round r
+// index of this lane's entry in a list that the wave's lanes with pred
+// append to: one atomic per wave on the list's count (-1 without pred).  The
+// lists' order does not matter to the rounds that read them.
+__device__ __forceinline__ int wave_append(int32_t* cnt, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return -1;
+  const int lane = threadIdx.x & 63, first = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == first) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, first);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// every lane of the launch runs the appends (whole waves), so no early return
 template <bool COUNT>
 __global__ void x_excess(XDev X, int64_t r) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= X.n) return;
-  int el, er;
-  if (COUNT) {
-    x_excess_node(X.S, v, X.rp[v], X.rp[v + 1], el, er);
-  } else {
-    el = X.S.eL[v];
-    er = X.S.eR[v];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int el = 0, er = 0;
+  if (v < X.n) {
+    if (COUNT) {
+      x_excess_node(X.S, v, X.rp[v], X.rp[v + 1], el, er);
+    } else {
+      el = X.S.eL[v];
+      er = X.S.eR[v];
+    }
   }
   const int k = (int)(r % RING);
-  if (el > 0) ((r & 1) ? X.actL1 : X.actL0)[atomicAdd(&X.cnt[2 * k], 1)] = v;
-  if (er > 0) X.actR[atomicAdd(&X.cnt[2 * k + 1], 1)] = v;
+  const int il = wave_append(&X.cnt[2 * k], el > 0);
+  if (il >= 0) ((r & 1) ? X.actL1 : X.actL0)[il] = v;
+  const int ir = wave_append(&X.cnt[2 * k + 1], er > 0);
+  if (ir >= 0) X.actR[ir] = v;
 }
 
 // L step of round r: every listed left node discharges; right nodes whose
