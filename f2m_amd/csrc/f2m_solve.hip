@@ -680,10 +680,14 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
 // Gaussian mixture a fit of the tiles' times gives ~280 node updates per
 // overflow slot for the block_EF lists, ~1.5 for the block_GLOB lists).  The
 // new boundaries cut the cumulative work, taken as uniform over each tile's
-// ids, into equal parts, halfway from the old boundaries.  Every tile keeps
-// between 1 and cap ids, else the boundaries go back to the uniform split
-// t*ts.  One workgroup of BAL_T threads; T <= BAL_MAX tiles (not called
-// above).  Deterministic: the headers are set sizes.
+// ids, into equal parts, halfway from the old boundaries.  Then every tile is
+// clamped to between 1 and cap ids (a pass from the first boundary, then one
+// from the last); if that still fails, the old boundaries stay.  (Falling
+// back to the uniform split t*ts instead, as before, undid the balance
+// whenever a light tile's share passed the cap: on the n = 100k Gaussian
+// mixture the block_EF tiles went back to uniform after seven builds.)  One
+// workgroup of BAL_T threads; T <= BAL_MAX tiles (not called above).
+// Deterministic: the headers are set sizes.
 constexpr int BAL_T = 1024, BAL_MAX = 4096;
 __global__ void __launch_bounds__(BAL_T) s_tile_balance(int T, int n, int ts, int cap, int fuse, double ovw,
                                                        const int32_t* __restrict__ hdr,
@@ -759,13 +763,21 @@ __global__ void __launch_bounds__(BAL_T) s_tile_balance(int T, int n, int ts, in
   m = 0;
   for (int k = tid; k <= T; k += BAL_T, ++m) old[k] = nb[m];
   __syncthreads();
+  if (tid == 0) {  // tile sizes into [1, cap]: from the first boundary, then from the last
+    old[0] = 0;
+    for (int k = 1; k < T; ++k) old[k] = min(max(old[k], old[k - 1] + 1), old[k - 1] + cap);
+    old[T] = n;
+    for (int k = T - 1; k >= 1; --k) old[k] = min(max(old[k], old[k + 1] - cap), old[k + 1] - 1);
+  }
+  __syncthreads();
   for (int k = tid; k < T; k += BAL_T) {
     const int sz = old[k + 1] - old[k];
     if (sz < 1 || sz > cap) bad = 1;
   }
   __syncthreads();
   const bool ok = bad == 0 && old[0] == 0 && old[T] == n;
-  for (int k = tid; k <= T; k += BAL_T) tst[k] = ok ? old[k] : (int)min((int64_t)k * ts, (int64_t)n);
+  if (ok)
+    for (int k = tid; k <= T; k += BAL_T) tst[k] = old[k];
 }
 // uniform tile boundaries t*ts (the last tile ends at n)
 __global__ void s_tile_uniform(int T, int n, int ts, int32_t* __restrict__ tst) {
