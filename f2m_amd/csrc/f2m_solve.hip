@@ -433,13 +433,21 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // may run up to fit_min + 1 sweeps.
 // The build runs T threads (TL by default: every level pass and the record
 // pass are one pass over the local nodes) for the tiles of the W-thread sweep.
+// With Fg set, the build also takes the gradient of the new lists at the
+// potentials Fg for the tile's own nodes (every node is in one tile) and the
+// max |g| into gmax (as s_sweep_pad<..., false> does for the convergence
+// checks that follow a list rebuild): each lane pushes its slots' d into a
+// Top3, the node's four are merged by shuffles.  Top3 keeps the same three
+// values whatever the order, so g has the bits of the separate gradient pass
+// whose launch this saves.
 template <int W, int T = TCfg<W>::TL>
 __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restrict__ tst, int pp, int tpp, int depth,
                                                    const int32_t* __restrict__ act,
                                                    const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
-                                                   int32_t* __restrict__ fit_min) {
+                                                   int32_t* __restrict__ fit_min, const double* __restrict__ Fg,
+                                                   unsigned long long* __restrict__ gmax) {
   constexpr int TW = T, TL = TCfg<W>::TL, THSZ = TCfg<W>::THSZ, TOV = TCfg<W>::TOV;
   static_assert(T % 64 == 0 && T <= 1024, "whole waves, one workgroup");
   // four lanes per local node in the level and record passes: lane q takes
@@ -663,6 +671,36 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
     for (int k = q; k < tpp; k += LN)
       if (k >= L) tslot[((int64_t)t * TL + j) * tpp + k] = TSlot{__builtin_inf(), v, jj};
   }
+  if (Fg) {  // gradient of the tile's own nodes (positions below c0, the first RP0 passes)
+    constexpr int RP0 = (TS_W + NG - 1) / NG;
+    static_assert(RP0 <= RP, "the tile's nodes are among the recorded ones");
+    double gm = 0.0;
+#pragma unroll
+    for (int r = 0; r < RP0; ++r) {
+      const int j = g + r * NG;
+      const bool own = j < c0;  // the same for a node's four lanes
+      Top3 t3;
+      int v = 0;
+      if (own) {
+        v = rv[r];
+        const int L = rL[r], ov = rov[r];
+        const double Fv = Fg[v];
+        for (int k = q; k < L; k += LN) {
+          const ESlot& e = k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k);
+          t3.push(dval(e.D, Fv, Fg[e.nbr], v, e.nbr));
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < LN; off <<= 1) {
+        const double a2 = __shfl_xor(t3.a, off, LN), b2 = __shfl_xor(t3.b, off, LN), c2 = __shfl_xor(t3.c, off, LN);
+        t3.push(a2);
+        t3.push(b2);
+        t3.push(c2);
+      }
+      if (own && q == 0) gm = fmax(gm, fabs(t3.grad()));
+    }
+    tile_max_to<T>(gm, gmax + (t & 31) * 16);
+  }
   __syncthreads();
   if (tid == 0) {
     const int f = s_ov > TOV ? -1 : fit;
@@ -763,18 +801,35 @@ __global__ void __launch_bounds__(BAL_T) s_tile_balance(int T, int n, int ts, in
   m = 0;
   for (int k = tid; k <= T; k += BAL_T, ++m) old[k] = nb[m];
   __syncthreads();
-  if (tid == 0) {  // tile sizes into [1, cap]: from the first boundary, then from the last
-    old[0] = 0;
-    for (int k = 1; k < T; ++k) old[k] = min(max(old[k], old[k - 1] + 1), old[k - 1] + cap);
-    old[T] = n;
-    for (int k = T - 1; k >= 1; --k) old[k] = min(max(old[k], old[k + 1] - cap), old[k + 1] - 1);
-  }
-  __syncthreads();
   for (int k = tid; k < T; k += BAL_T) {
     const int sz = old[k + 1] - old[k];
     if (sz < 1 || sz > cap) bad = 1;
   }
   __syncthreads();
+  if (bad) {  // tile sizes into [1, cap]: from the first boundary, then from the last
+    if (tid == 0) {
+      int prev = 0;
+      old[0] = 0;
+#pragma unroll 8
+      for (int k = 1; k < T; ++k) {
+        prev = min(max(old[k], prev + 1), prev + cap);
+        old[k] = prev;
+      }
+      old[T] = prev = n;
+#pragma unroll 8
+      for (int k = T - 1; k >= 1; --k) {
+        prev = min(max(old[k], prev - cap), prev - 1);
+        old[k] = prev;
+      }
+      bad = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < T; k += BAL_T) {
+      const int sz = old[k + 1] - old[k];
+      if (sz < 1 || sz > cap) bad = 1;
+    }
+    __syncthreads();
+  }
   const bool ok = bad == 0 && old[0] == 0 && old[T] == n;
   if (ok)
     for (int k = tid; k <= T; k += BAL_T) tst[k] = old[k];
@@ -1432,6 +1487,10 @@ struct Runner {
   int pending_kind = 0;
   int tpp = 4;    // padded slots per local node on the current lists
   bool selh = true;  // block_GLOB tile sweeps take the owner's half from a register (F2M_TILE_SELH)
+  // the convergence checks after a list rebuild take their gradient from the
+  // tile build instead of a separate pass (F2M_BUILD_GRAD=0: the pass)
+  bool build_grad = true;
+  bool grad_ready = false;  // region 0 holds the last build's max |g|
   int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
   // with profiling on: sweeps run on the current lists not yet counted, and a
@@ -1533,6 +1592,7 @@ struct Runner {
       tw_ef = shape(env_int("F2M_TILE_W", TW_GLOB));
       tw_glob = shape(env_int("F2M_TILE_W_GLOB", TW_GLOB));
       selh = env_int("F2M_TILE_SELH", 1) != 0;
+      build_grad = env_int("F2M_BUILD_GRAD", 1) != 0;
       // tile nodes of a workgroup shape: the same whole number of tiles on every CU, as large as
       // the shape's cap allows (256 nodes for 384 threads, 512 for 768)
       auto balanced = [&](int tw_) {
@@ -1703,7 +1763,9 @@ struct Runner {
   // update_EF (graph_cuda.cu:229-252): offsets = scan(act), copy the active
   // prefixes.  With profiling on, the sweeps run on the old lists are counted
   // first (on the device).
-  f2m_status ef_rebuild(bool long_lists = false) {
+  // grad: the convergence check that follows takes its gradient from the
+  // tile build (s_tile_build with Fg), when the lists are tiled
+  f2m_status ef_rebuild(bool long_lists = false, bool grad = false) {
     f2m_status st0 = flush_slots();
     if (st0) return st0;
     // offsets of the compacted lists: the padded layout needs them only for
@@ -1725,13 +1787,19 @@ struct Runner {
         const int ts = long_lists ? ts_glob : ts_ef;
         tw = long_lists ? tw_glob : tw_ef;
         pending_kind = kind;
+        const bool bg = grad && build_grad;
+        if (bg) {
+          f2m_status sz = zero_region(0);
+          if (sz) return sz;
+        }
         auto build = [&](auto kern, int threads) {
           hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, (const int32_t*)tile_starts(kind), pp,
                              tpp, depth,
                              (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
                              w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
-                             tile_fit());
+                             tile_fit(), bg ? (const double*)F() : (const double*)nullptr, bg ? maxs : nullptr);
         };
+        grad_ready = bg;
         if (tw == TW_GLOB) build(s_tile_build<TW_GLOB, 1024>, 1024);
         else build(s_tile_build<TW_EF>, TCfg<TW_EF>::TL);
         if (balance > kind && ntiles <= BAL_MAX) {  // boundaries for the next build of this kind
@@ -1791,6 +1859,10 @@ struct Runner {
   // gradient over the active lists + max |g| (host round trip)
   f2m_status grad_max(double& M) {
     f2m_status s;
+    if (grad_ready) {  // the last tile build took the gradient into region 0
+      grad_ready = false;
+      return read_max(0, M);
+    }
     if ((s = zero_region(0))) return s;
     if ((s = launch_sweep(false, false, maxs))) return s;
     return read_max(0, M);
@@ -1918,7 +1990,7 @@ struct Runner {
         hipLaunchKernelGGL(s_part_v4, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act),
                            o.glob_k, I(w.nbr), w.D.as<double>(), I(w.orig), F());
       F2M_LAUNCH_CHECK(c);
-      if ((s = ef_rebuild(long_lists))) return s;
+      if ((s = ef_rebuild(long_lists, true))) return s;
       double M;
       if ((s = grad_max(M))) return s;
       if (M < o.glob_exit) { ++go; break; }
@@ -1935,7 +2007,7 @@ struct Runner {
       hipLaunchKernelGGL(s_part_v2, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, I(w.deg), I(w.act), 1.0,
                          I(w.nbr), w.D.as<double>(), I(w.orig), F());
     F2M_LAUNCH_CHECK(c);
-    f2m_status s = ef_rebuild();
+    f2m_status s = ef_rebuild(false, true);
     if (s) return s;
     return grad_max(M);
   }

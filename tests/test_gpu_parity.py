@@ -175,13 +175,17 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
     assert np.array_equal(ctx.result()["F"], res["F"]) and np.array_equal(ctx.result()["x"], res["x"])
     monkeypatch.delenv("F2M_PART_LANES")
-    # block_GLOB tile sweeps reading the owner's half from shared memory instead of a register
-    monkeypatch.setenv("F2M_TILE_SELH", "0")
-    say("solve C2 Morton, F2M_TILE_SELH 0")
-    s2 = ctx.solve(g)
-    assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"]
-    assert np.array_equal(ctx.result()["F"], res["F"])
-    monkeypatch.delenv("F2M_TILE_SELH")
+    # block_GLOB tile sweeps reading the owner's half from shared memory instead of a register;
+    # the convergence checks' gradient from a separate pass instead of the tile build
+    for var in ("F2M_TILE_SELH", "F2M_BUILD_GRAD"):
+        monkeypatch.setenv(var, "0")
+        say("solve C2 Morton,", var, "0")
+        s2 = ctx.solve(g)
+        assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], var
+        assert s2["glob_rounds"] == st["glob_rounds"] and s2["outer_rounds"] == st["outer_rounds"], var
+        assert s2["max_grad"] == st["max_grad"], var
+        assert np.array_equal(ctx.result()["F"], res["F"]), var
+        monkeypatch.delenv(var)
     # nodes per tile of the block_GLOB and block_EF lists; their workgroup shapes (384 threads
     # for 768 local nodes, 768 for 1536)
     for tg, te, wg, we in (("256", "256", "768", "384"), ("64", "128", "384", "768"), ("196", "196", "384", "384"),
