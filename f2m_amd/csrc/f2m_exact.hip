@@ -35,6 +35,7 @@
 // a node joins the other side's list when its excess crosses 0 -> 1.
 // Discharge steps: 16 lanes per node, grid-stride over the active list.
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -129,6 +130,23 @@ __global__ void x_flow_sat_lanes(XDev X, double eps) {
   if (l == 0 && dl) S.eL[v] += dl;
 }
 
+// The stage's counters for the host (XPOST_W words from X.cnt: the ring,
+// flags, rounds, the last x_rounds_wg's round count, the certificate sums),
+// stored by one lane to pinned host memory with system scope, then seq with
+// a release: a host that reads seq there reads the values too.
+constexpr int XPOST_W = 12;
+__global__ void x_post(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ host,
+                       unsigned long long seq) {
+  if (threadIdx.x == 0) {
+    unsigned long long v[XPOST_W];
+#pragma unroll
+    for (int k = 0; k < XPOST_W; ++k) v[k] = src[k];
+#pragma unroll
+    for (int k = 0; k < XPOST_W; ++k) __hip_atomic_store(&host[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&host[XPOST_W], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 __global__ void x_zero_ring(int32_t* cnt) {
   if (threadIdx.x < 2 * RING) cnt[threadIdx.x] = 0;
 }
@@ -168,9 +186,37 @@ __global__ void x_excess(XDev X, int64_t r) {
   if (ir >= 0) X.actR[ir] = v;
 }
 
+// A workgroup's appends to a global list: collected in shared memory, then
+// one atomic on the list's count per workgroup (a step's appends otherwise
+// all go to one address); past XQ entries they go to the list directly.
+// The lists' order does not matter to the steps that read them.
+constexpr int XQ = 512;
+struct BlockQueue {
+  int32_t* q;  // [XQ] shared
+  int* n;      // shared count
+  __device__ __forceinline__ void push(int32_t* list, int32_t* cnt, int v) {
+    const int k = atomicAdd(n, 1);
+    if (k < XQ) q[k] = v;
+    else list[atomicAdd(cnt, 1)] = v;
+  }
+  // every thread of the workgroup calls it, after its pushes
+  __device__ __forceinline__ void flush(int32_t* list, int32_t* cnt, int* base) {
+    __syncthreads();
+    const int m = min(*n, XQ);
+    if (threadIdx.x == 0 && m > 0) *base = atomicAdd(cnt, m);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) list[*base + i] = q[i];
+  }
+};
+
 // L step of round r: every listed left node discharges; right nodes whose
-// excess crosses 0 -> 1 join the R list of round r
+// excess crosses 0 -> 1 join the R list of round r.  XR > 0: rows of at
+// most XG XR slots discharge from registers (x_discharge_l_reg), and the
+// appends go through a BlockQueue.
+template <int XR>
 __global__ void __launch_bounds__(XBS) x_lstep(XDev X, int64_t r, double eps) {
+  __shared__ int32_t sq[XQ];
+  __shared__ int sn, sbase;
   const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
   if (blockIdx.x == 0 && threadIdx.x == 0) X.cnt[2 * k1] = 0;  // nL of round r+1, filled by R(r)
   const int na = X.cnt[2 * k];
@@ -179,15 +225,27 @@ __global__ void __launch_bounds__(XBS) x_lstep(XDev X, int64_t r, double eps) {
   const int groups = gridDim.x * (XBS / XG);
   int32_t* cr = &X.cnt[2 * k + 1];
   int32_t* lr = X.actR;
+  BlockQueue bq{sq, &sn};
+  if (XR > 0) {
+    if (threadIdx.x == 0) sn = 0;
+    __syncthreads();
+  }
   for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
     const int v = list[i];
-    x_discharge_l(X.S, v, X.rp[v], X.rp[v + 1], lane, eps, [&](int w) { lr[atomicAdd(cr, 1)] = w; });
+    x_discharge_l_any<XR>(X.S, v, X.rp[v], X.rp[v + 1], lane, eps, [&](int w) {
+      if (XR > 0) bq.push(lr, cr, w);
+      else lr[atomicAdd(cr, 1)] = w;
+    });
   }
+  if (XR > 0) bq.flush(lr, cr, &sbase);
 }
 
 // R step of round r: every listed right node discharges; left nodes whose
 // excess crosses 0 -> 1 join the L list of round r+1
+template <int XR>
 __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
+  __shared__ int32_t sq[XQ];
+  __shared__ int sn, sbase;
   const int k = (int)(r % RING), k1 = (int)((r + 1) % RING);
   const int na = X.cnt[2 * k + 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -198,10 +256,19 @@ __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
   int32_t* cl = &X.cnt[2 * k1];
   const int lane = threadIdx.x % XG;
   const int groups = gridDim.x * (XBS / XG);
+  BlockQueue bq{sq, &sn};
+  if (XR > 0) {
+    if (threadIdx.x == 0) sn = 0;
+    __syncthreads();
+  }
   for (int i = blockIdx.x * (XBS / XG) + threadIdx.x / XG; i < na; i += groups) {
     const int w = X.actR[i];
-    x_discharge_r(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) { next[atomicAdd(cl, 1)] = v; });
+    x_discharge_r_any<XR>(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) {
+      if (XR > 0) bq.push(next, cl, v);
+      else next[atomicAdd(cl, 1)] = v;
+    });
   }
+  if (XR > 0) bq.flush(next, cl, &sbase);
 }
 
 // Rounds r0, r0+1, ... of a phase inside one workgroup, used once the active
@@ -210,11 +277,12 @@ __global__ void __launch_bounds__(XBS) x_rstep(XDev X, int64_t r, double eps) {
 // counters in shared memory, the lists in the same global arrays.  Stops
 // after the first round without an active node, after max_rounds rounds with
 // one, or when the next L list reaches `small` nodes (back to the grid
-// steps).  On exit the ring holds the counts of the last round and of the
+// steps); runs no round when round r0's L list already has `small`.  On exit the ring holds the counts of the last round and of the
 // next one (as the grid steps leave it), X.rounds is advanced and *ran =
 // rounds run.  The steps' outcome does not depend on the order in which a
 // step's nodes are processed, so the bits equal the grid steps'.
 constexpr int XWG = 1024;
+template <int XR>
 __global__ void __launch_bounds__(XWG) x_rounds_wg(XDev X, int64_t r0, double eps, int64_t max_rounds, int small,
                                                    int32_t* ran) {
   __shared__ int sc[3];  // nL(r), nR(r), nL(r+1)
@@ -226,20 +294,25 @@ __global__ void __launch_bounds__(XWG) x_rounds_wg(XDev X, int64_t r0, double ep
     sc[2] = 0;
   }
   __syncthreads();
+  if (sc[0] >= small) {  // lists still long: back to the grid steps, ring untouched
+    if (threadIdx.x == 0) *ran = 0;
+    return;
+  }
   int64_t r = r0, counted = 0;
   for (;;) {
     const int nl = sc[0];
     const int32_t* list = (r & 1) ? X.actL1 : X.actL0;
     for (int i = gid; i < nl; i += NG) {
       const int v = list[i];
-      x_discharge_l(X.S, v, X.rp[v], X.rp[v + 1], lane, eps, [&](int w) { X.actR[atomicAdd(&sc[1], 1)] = w; });
+      x_discharge_l_any<XR>(X.S, v, X.rp[v], X.rp[v + 1], lane, eps,
+                            [&](int w) { X.actR[atomicAdd(&sc[1], 1)] = w; });
     }
     __syncthreads();
     const int nr = sc[1];
     int32_t* next = (r & 1) ? X.actL0 : X.actL1;
     for (int i = gid; i < nr; i += NG) {
       const int w = X.actR[i];
-      x_discharge_r(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) { next[atomicAdd(&sc[2], 1)] = v; });
+      x_discharge_r_any<XR>(X.S, w, X.rp[w], X.rp[w + 1], lane, eps, [&](int v) { next[atomicAdd(&sc[2], 1)] = v; });
     }
     __syncthreads();
     const int nl1 = sc[2];
@@ -428,17 +501,52 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   double* sums = reinterpret_cast<double*>(X.cnt + 20);
   const dim3 gn(grid_for(n, XBS)), bs(XBS);
   hipStream_t st = c->stream;
+  // the host's copy of X.cnt [0, 24): ring, flags, rounds, ran, sums
+  struct Post {
+    int32_t cnt[8];
+    int32_t flag[8];
+    unsigned long long rounds;
+    int32_t ran;  // rounds run by the last x_rounds_wg
+    int32_t pad;
+    double P, D;
+  };
+  static_assert(sizeof(Post) == XPOST_W * 8, "x_post copies the Post words");
+  unsigned long long* hpost = static_cast<unsigned long long*>(c->pinned) + 3072;
+  Post hv;
+  // waits for the kernels so far and reads X.cnt's words (x_post; a stream
+  // synchronisation after 50 ms, which also reports any error)
+  auto post = [&]() -> f2m_status {
+    const unsigned long long want = ++c->read_seq;
+    hipLaunchKernelGGL(x_post, dim3(1), dim3(64), 0, st, reinterpret_cast<const unsigned long long*>(X.cnt), hpost,
+                       want);
+    F2M_LAUNCH_CHECK(c);
+    bool seen = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 1;; ++it) {
+      if (__atomic_load_n(&hpost[XPOST_W], __ATOMIC_ACQUIRE) == want) {
+        seen = true;
+        break;
+      }
+      if ((it & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+    }
+    if (!seen) F2M_HIP(c, hipStreamSynchronize(st));
+    unsigned long long wv[XPOST_W];
+    for (int k = 0; k < XPOST_W; ++k) wv[k] = __atomic_load_n(&hpost[k], __ATOMIC_ACQUIRE);
+    std::memcpy(&hv, wv, sizeof(hv));
+    return F2M_OK;
+  };
   hipLaunchKernelGGL(x_zero_i32, dim3(1), dim3(64), 0, st, X.cnt, 20);
   hipLaunchKernelGGL(x_mirror, dim3(grid_for((int64_t)n * XM_LANES, bs.x)), bs, 0, st, n, rp, nbr, cost,
                      w.xmir.as<int32_t>(), flag);
   F2M_LAUNCH_CHECK(c);
-  int32_t* hp = static_cast<int32_t*>(c->pinned);
-  F2M_HIP(c, hipMemcpyAsync(hp, flag, 12, hipMemcpyDeviceToHost, st));
-  F2M_HIP(c, hipStreamSynchronize(st));
+  {
+    f2m_status s = post();
+    if (s) return s;
+  }
   z.exact_ran = 1;
-  if (hp[0]) { rc = -1; return F2M_OK; }
-  if (hp[1]) { rc = -2; return F2M_OK; }
-  const bool integral = hp[2] == 0;
+  if (hv.flag[0]) { rc = -1; return F2M_OK; }
+  if (hv.flag[1]) { rc = -2; return F2M_OK; }
+  const bool integral = hv.flag[2] == 0;
   hipLaunchKernelGGL(x_prices, gn, bs, 0, st, n, F, dmin, X.S.pL, X.S.pR);
   hipLaunchKernelGGL((x_flow<false>), gn, bs, 0, st, X, 0.0);
   F2M_LAUNCH_CHECK(c);
@@ -448,25 +556,44 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   int phases = 0;
   bool stuck = false, certified = false;
   double P = 0, D = 0;
-  struct Ring {
-    int32_t cnt[8];
-    int32_t flag[8];
-    unsigned long long rounds;
-    int32_t ran;  // rounds run by the last x_rounds_wg
-  };
   int32_t* ran = reinterpret_cast<int32_t*>(X.rounds + 1);
   // lists below this many nodes run their rounds inside one workgroup
   const int small = env_int("F2M_EXACT_WG", 512);
-  Ring* hr = static_cast<Ring*>(c->pinned);
-  auto cert = [&]() -> f2m_status {
+  // rows of at most 64 slots discharge from registers (0: gather per push)
+  const bool reg = env_int("F2M_EXACT_REG", 1) != 0;
+  // a phase's first pass: fuse_k grid rounds, the in-workgroup rounds and
+  // the certificate, launched together with one wait (0: a wait after each)
+  const int fuse_k = small > 0 ? env_int("F2M_EXACT_FUSE", 6) : 0;
+  bool fuse = fuse_k > 0;
+  auto grid_rounds = [&](int64_t kk) {
+    for (int64_t i = 0; i < kk; ++i, ++r) {
+      if (reg) {
+        hipLaunchKernelGGL(x_lstep<4>, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+        hipLaunchKernelGGL(x_rstep<4>, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+      } else {
+        hipLaunchKernelGGL(x_lstep<0>, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+        hipLaunchKernelGGL(x_rstep<0>, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
+      }
+    }
+  };
+  // at most 65536 rounds per launch, so that no launch runs for seconds
+  auto wg_rounds = [&](int64_t maxr) {
+    if (reg) hipLaunchKernelGGL(x_rounds_wg<4>, dim3(1), dim3(XWG), 0, st, X, r, eps,
+                                std::min<int64_t>(maxr, 65536), small, ran);
+    else hipLaunchKernelGGL(x_rounds_wg<0>, dim3(1), dim3(XWG), 0, st, X, r, eps,
+                            std::min<int64_t>(maxr, 65536), small, ran);
+  };
+  auto cert_launch = [&]() {
     hipLaunchKernelGGL(x_cert_chunks, dim3(DET_T), dim3(XC_T), 0, st, X, part);
     hipLaunchKernelGGL(x_det_final, dim3(1), dim3(256), 0, st, part, sums);
+  };
+  auto cert = [&]() -> f2m_status {
+    cert_launch();
     F2M_LAUNCH_CHECK(c);
-    double h[2];
-    F2M_HIP(c, hipMemcpyAsync(h, sums, 16, hipMemcpyDeviceToHost, st));
-    F2M_HIP(c, hipStreamSynchronize(st));
-    P = h[0];
-    D = h[1];
+    f2m_status s = post();
+    if (s) return s;
+    P = hv.P;
+    D = hv.D;
     return F2M_OK;
   };
   for (;;) {
@@ -477,43 +604,64 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     else hipLaunchKernelGGL((x_excess<false>), gn, bs, 0, st, X, r);
     F2M_LAUNCH_CHECK(c);
     int chunk = 2;
-    bool in_wg = false;
+    bool in_wg = false, first = true, have_cert = false;
     for (;;) {
       const int64_t left = o.exact_max_rounds - done;
       if (left <= 0) { stuck = true; break; }
-      if (in_wg) {
-        // at most 65536 rounds per launch, so that no launch runs for seconds
-        hipLaunchKernelGGL(x_rounds_wg, dim3(1), dim3(XWG), 0, st, X, r, eps, std::min<int64_t>(left, 65536), small,
-                           ran);
+      if (fuse && first && left > fuse_k) {
+        // the usual course of a phase in one go; the in-workgroup rounds
+        // return at once if the lists are still long, and the certificate
+        // is used only if the phase has ended
+        first = false;
+        grid_rounds(fuse_k);
+        wg_rounds(left - fuse_k);
+        cert_launch();
         F2M_LAUNCH_CHECK(c);
-        F2M_HIP(c, hipMemcpyAsync(hr, X.cnt, sizeof(Ring), hipMemcpyDeviceToHost, st));
-        F2M_HIP(c, hipStreamSynchronize(st));
-        r += hr->ran;
-        done = (int64_t)hr->rounds;
+        f2m_status s = post();
+        if (s) return s;
+        r += hv.ran;
+        done = (int64_t)hv.rounds;
         const int kl = (int)((r - 1) % RING);
-        if (hr->cnt[2 * kl] + hr->cnt[2 * kl + 1] == 0) break;
-        in_wg = hr->cnt[2 * (int)(r % RING)] < small;  // else the lists grew: grid steps
+        if (hv.cnt[2 * kl] + hv.cnt[2 * kl + 1] == 0) {
+          P = hv.P;
+          D = hv.D;
+          have_cert = true;
+          break;
+        }
+        fuse = false;  // this graph's phases outlast the fused pass: a wait per launch group from here
+        in_wg = hv.cnt[2 * (int)(r % RING)] < small;
+        continue;
+      }
+      first = false;
+      if (in_wg) {
+        wg_rounds(left);
+        F2M_LAUNCH_CHECK(c);
+        f2m_status s = post();
+        if (s) return s;
+        r += hv.ran;
+        done = (int64_t)hv.rounds;
+        const int kl = (int)((r - 1) % RING);
+        if (hv.cnt[2 * kl] + hv.cnt[2 * kl + 1] == 0) break;
+        in_wg = hv.cnt[2 * (int)(r % RING)] < small;  // else the lists grew: grid steps
         chunk = 2;
         continue;
       }
-      const int64_t kk = std::min<int64_t>(chunk, left);
-      for (int64_t i = 0; i < kk; ++i, ++r) {
-        hipLaunchKernelGGL(x_lstep, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
-        hipLaunchKernelGGL(x_rstep, dim3(XGRID), dim3(XBS), 0, st, X, r, eps);
-      }
+      grid_rounds(std::min<int64_t>(chunk, left));
       F2M_LAUNCH_CHECK(c);
-      F2M_HIP(c, hipMemcpyAsync(hr, X.cnt, sizeof(Ring), hipMemcpyDeviceToHost, st));
-      F2M_HIP(c, hipStreamSynchronize(st));
-      done = (int64_t)hr->rounds;
+      f2m_status s = post();
+      if (s) return s;
+      done = (int64_t)hv.rounds;
       const int kl = (int)((r - 1) % RING);
-      if (hr->cnt[2 * kl] + hr->cnt[2 * kl + 1] == 0) break;
+      if (hv.cnt[2 * kl] + hv.cnt[2 * kl + 1] == 0) break;
       chunk = std::min(chunk * 2, 64);
-      in_wg = small > 0 && hr->cnt[2 * (int)(r % RING)] < small;
+      in_wg = small > 0 && hv.cnt[2 * (int)(r % RING)] < small;
     }
     ++phases;
     if (stuck) break;
-    f2m_status s = cert();
-    if (s) return s;
+    if (!have_cert) {
+      f2m_status s = cert();
+      if (s) return s;
+    }
     const double tol = std::max(integral ? 0.4 : 0.0, o.exact_tol * std::fabs(P));
     if (P - D <= tol) { certified = true; break; }
     eps = eps / (double)o.exact_alpha;
@@ -530,9 +678,11 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   hipLaunchKernelGGL(x_zero_i32, dim3(1), dim3(64), 0, st, flag + 4, 1);
   hipLaunchKernelGGL(x_half_root, gn, bs, 0, st, X, par, root, flag + 4);
   F2M_LAUNCH_CHECK(c);
-  F2M_HIP(c, hipMemcpyAsync(hp, flag + 4, 4, hipMemcpyDeviceToHost, st));
-  F2M_HIP(c, hipStreamSynchronize(st));
-  z.n_combs = hp[0];
+  {
+    f2m_status s = post();
+    if (s) return s;
+  }
+  z.n_combs = hv.flag[4];
   z.exact_phases = phases;
   z.exact_rounds = done;
   z.exact_gap = P - D;

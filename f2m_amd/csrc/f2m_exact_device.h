@@ -130,6 +130,199 @@ __device__ __forceinline__ void x_discharge_r(const XState& X, int w, int b, int
   }
 }
 
+// The same discharges with the node's row gathered once into registers
+// (lane `lane` holds slots b + lane + XG j, j < R; rows of at most XG R
+// slots).  Within one step nothing a discharge reads changes except what
+// that discharge writes: pR moves in R steps only and pL in L steps only;
+// the flow of arc v -> w is set by L_v's discharge (L step) and cleared by
+// R_w's (R step), so each arc has one writer per step.  The per-lane best
+// uses the same (key, slot) order as x_l_best / x_r_best and the tests
+// the same operands, so the outcome is the same bit for bit; the gathers
+// go from one per push (plus one for the last relabel) to one.
+template <int R, class Cross>
+__device__ __forceinline__ void x_discharge_l_reg(const XState& X, int v, int b, int e, int lane, double eps,
+                                                  Cross on_cross) {
+  int ex = X.eL[v];
+  if (ex <= 0) return;
+  double p = X.pL[v];
+  double key[R], cs[R], pw[R];
+  int wn[R];
+  unsigned ok = 0;  // bit j: slot j is a residual arc (f == 0)
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = b + lane + XG * j;
+    key[j] = 0.0;
+    cs[j] = 0.0;
+    pw[j] = 0.0;
+    wn[j] = 0;
+    if (s < e) {
+      wn[j] = X.nbr[s];
+      cs[j] = X.c[s];
+      if (!X.f[s]) ok |= 1u << j;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (b + lane + XG * j < e) {
+      pw[j] = X.pR[wn[j]];
+      key[j] = pw[j] - cs[j];
+    }
+  }
+  // best residual arc, and whether it is admissible at the current p
+  auto best = [&](double& bv, int& bs, bool& adm) {
+    bv = -INFINITY;
+    bs = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int s = b + lane + XG * j;
+      if (((ok >> j) & 1u) && (key[j] > bv || (key[j] == bv && s < bs))) {
+        bv = key[j];
+        bs = s;
+      }
+    }
+    xg_best(bv, bs);
+    int a = 0;
+    if (bs != INT_MAX && (bs - b) % XG == lane) {
+      const int jb = (bs - b) / XG;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j == jb) a = (cs[j] + p) - pw[j] < 0.0 ? 1 : 0;
+    }
+    adm = __shfl(a, (bs - b) % XG, XG) != 0;
+  };
+  // the excess atomics' old values are tested after the pushes, so that
+  // they are in flight together (each slot is pushed at most once)
+  unsigned pushed = 0;
+  int old[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) old[j] = 1;
+  while (ex > 0) {
+    double bv;
+    int bs;
+    bool adm;
+    best(bv, bs, adm);
+    if (bs == INT_MAX) break;
+    if (!adm) p = bv - eps;
+    if ((bs - b) % XG == lane) {
+      const int jb = (bs - b) / XG;
+      ok &= ~(1u << jb);
+      pushed |= 1u << jb;
+      X.f[bs] = 1;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j == jb) old[j] = atomicAdd(&X.eR[wn[j]], 1);
+    }
+    --ex;
+  }
+  {
+    double bv;
+    int bs;
+    bool adm;
+    best(bv, bs, adm);
+    if (bs != INT_MAX && !adm) p = bv - eps;
+  }
+  if (lane == 0) {
+    X.pL[v] = p;
+    X.eL[v] = ex;
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (((pushed >> j) & 1u) && old[j] == 0) on_cross(wn[j]);
+}
+
+template <int R, class Cross>
+__device__ __forceinline__ void x_discharge_r_reg(const XState& X, int w, int b, int e, int lane, double eps,
+                                                  Cross on_cross) {
+  int ex = X.eR[w];
+  if (ex <= 0) return;
+  double p = X.pR[w];
+  double key[R];
+  int vn[R], mr[R];
+  unsigned ok = 0;  // bit j: the mirror of slot j carries flow (f == 1)
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = b + lane + XG * j;
+    key[j] = 0.0;
+    vn[j] = 0;
+    mr[j] = 0;
+    if (s < e) {
+      vn[j] = X.nbr[s];
+      mr[j] = X.mir[s];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = b + lane + XG * j;
+    if (s < e) {
+      key[j] = X.c[s] + X.pL[vn[j]];
+      if (X.f[mr[j]]) ok |= 1u << j;
+    }
+  }
+  auto best = [&](double& bt, int& bs) {
+    bt = -INFINITY;
+    bs = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int s = b + lane + XG * j;
+      if (((ok >> j) & 1u) && (key[j] > bt || (key[j] == bt && s < bs))) {
+        bt = key[j];
+        bs = s;
+      }
+    }
+    xg_best(bt, bs);
+  };
+  unsigned pushed = 0;
+  int old[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) old[j] = 1;
+  while (ex > 0) {
+    double bt;
+    int bs;
+    best(bt, bs);
+    if (bs == INT_MAX) break;
+    if (!(bt - p > 0.0)) p = bt - eps;
+    if ((bs - b) % XG == lane) {
+      const int jb = (bs - b) / XG;
+      ok &= ~(1u << jb);
+      pushed |= 1u << jb;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j == jb) {
+          X.f[mr[j]] = 0;
+          old[j] = atomicAdd(&X.eL[vn[j]], 1);
+        }
+    }
+    --ex;
+  }
+  {
+    double bt;
+    int bs;
+    best(bt, bs);
+    if (bs != INT_MAX && !(bt - p > 0.0)) p = bt - eps;
+  }
+  if (lane == 0) {
+    X.pR[w] = p;
+    X.eR[w] = ex;
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (((pushed >> j) & 1u) && old[j] == 0) on_cross(vn[j]);
+}
+
+// R > 0: the register form for rows of at most XG R slots, else the gathers
+template <int R, class Cross>
+__device__ __forceinline__ void x_discharge_l_any(const XState& X, int v, int b, int e, int lane, double eps,
+                                                  Cross on_cross) {
+  if (R > 0 && e - b <= XG * R) x_discharge_l_reg<(R > 0 ? R : 1)>(X, v, b, e, lane, eps, on_cross);
+  else x_discharge_l(X, v, b, e, lane, eps, on_cross);
+}
+template <int R, class Cross>
+__device__ __forceinline__ void x_discharge_r_any(const XState& X, int w, int b, int e, int lane, double eps,
+                                                  Cross on_cross) {
+  if (R > 0 && e - b <= XG * R) x_discharge_r_reg<(R > 0 ? R : 1)>(X, w, b, e, lane, eps, on_cross);
+  else x_discharge_r(X, w, b, e, lane, eps, on_cross);
+}
+
 // mirror slot of every slot of v (rows [rp(u), rp(u+1))); returns false on a
 // missing or asymmetric mirror (then mir[s] = s).  Rows sorted by (cost, id)
 // are searched by bisection; a row out of that order falls back to a scan.

@@ -56,27 +56,52 @@ def test_exact_lattice_bit_exact_and_optimal(ctx, orc, golden, n):
     _feasible(rp, nb, res["x"])
 
 
-def test_exact_rounds_in_workgroup_or_grid(ctx, orc, monkeypatch):
-    # the rounds of small active lists run inside one workgroup (x_rounds_wg),
-    # the others as grid-wide steps: all rounds in grid steps (0), the
-    # default switch, and all rounds inside one workgroup give the same bits
-    x, y, mode, K = W.c3(n=20000)
-    g = ctx.knn_build(x, y, K, mode)
-    rp, nb, c = ctx.graph_csr(g)
+EXACT_ENV = ("F2M_EXACT_WG", "F2M_EXACT_REG", "F2M_EXACT_FUSE")
+
+
+def _variants_equal(ctx, orc, monkeypatch, rp, nb, c, name, variants, **kw):
     ref = None
-    for wg in ("0", None, "100000000"):
-        if wg is None:
-            monkeypatch.delenv("F2M_EXACT_WG", raising=False)
-        else:
-            monkeypatch.setenv("F2M_EXACT_WG", wg)
-        st, res = _compare(ctx, orc, rp, nb, c, f"lattice20000 wg={wg}")
+    for env in variants:
+        for k in EXACT_ENV:
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        st, res = _compare(ctx, orc, rp, nb, c, f"{name} {env}", **kw)
         assert st["exact_certified"] == 1
         if ref is None:
             ref = (st, res)
         else:
             for k in EXACT_KEYS:
-                assert st[k] == ref[0][k], (wg, k)
+                assert st[k] == ref[0][k], (env, k)
             assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    for k in EXACT_ENV:
+        monkeypatch.delenv(k, raising=False)
+
+
+def test_exact_rounds_in_workgroup_or_grid(ctx, orc, monkeypatch):
+    # the rounds of small active lists run inside one workgroup (x_rounds_wg),
+    # the others as grid-wide steps: all rounds in grid steps (WG 0), the
+    # defaults, all rounds inside one workgroup, discharges gathering their
+    # row per push (REG 0) and a wait after every launch group (FUSE 0) give
+    # the same bits
+    x, y, mode, K = W.c3(n=20000)
+    g = ctx.knn_build(x, y, K, mode)
+    rp, nb, c = ctx.graph_csr(g)
+    _variants_equal(ctx, orc, monkeypatch, rp, nb, c, "lattice20000",
+                    [{}, {"F2M_EXACT_WG": "0"}, {"F2M_EXACT_WG": "100000000"}, {"F2M_EXACT_REG": "0"},
+                     {"F2M_EXACT_FUSE": "0"}])
+
+
+def test_exact_rows_beyond_register_cache(ctx, orc, monkeypatch):
+    # rows of more than 64 slots discharge by gathers, shorter ones from
+    # registers; K = 56 gives both in one graph (forced exact stage)
+    x, y = W.uniform_int(3000, 10**6, 23)
+    g = ctx.knn_build(x, y, 56, W.COST_NINT)
+    rp, nb, c = ctx.graph_csr(g)
+    deg = np.diff(rp)
+    assert deg.min() <= 64 < deg.max()
+    _variants_equal(ctx, orc, monkeypatch, rp, nb, c, "uniform3000 K56",
+                    [{}, {"F2M_EXACT_REG": "0"}, {"F2M_EXACT_WG": "0", "F2M_EXACT_FUSE": "0"}], exact=2)
 
 
 def test_exact_forced_raw_costs(ctx, orc):
