@@ -20,6 +20,7 @@
 // shuffles; the merge is a selection, so every LPN gives identical bits.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,8 +62,9 @@ __global__ void s_set_i32(int32_t* p, int32_t v) {
 // and, with fit, the tiles' fit counter, stored to pinned host memory
 // (host[0], host[1]), then seq to host[2] with a system-scope release, so a
 // host that reads seq there reads the values too; the slots are cleared for
-// the region's next use.
-__global__ void s_read_max(unsigned long long* __restrict__ region, const int32_t* __restrict__ fit,
+// the region's next use, the fit counter reset to INT_MAX for the next tile
+// build's atomicMin.
+__global__ void s_read_max(unsigned long long* __restrict__ region, int32_t* __restrict__ fit,
                            unsigned long long* __restrict__ host, unsigned long long seq) {
   const int k = threadIdx.x;
   unsigned long long b = k < 32 ? region[16 * k] : 0ull;
@@ -73,9 +75,10 @@ __global__ void s_read_max(unsigned long long* __restrict__ region, const int32_
   }
   if (k < 32) region[16 * k] = 0ull;
   if (k == 0) {  // system scope: written through to host memory
+    const int32_t f = fit ? *fit : 0;
+    if (fit) *fit = INT_MAX;
     __hip_atomic_store(&host[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&host[1], fit ? (unsigned long long)(uint32_t)*fit : 0ull, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&host[1], (unsigned long long)(uint32_t)f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[2], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
@@ -244,14 +247,19 @@ constexpr int PP_MAX = 8;  // largest padded stride (workspace sizing)
 
 // Four lanes per node, lane q copying slots q, q + 4, ...: the padded slots
 // of consecutive nodes are consecutive, so the stores of a wave coalesce.
+// ovo == nullptr: only the first pp slots of each list (what the tile build
+// reads from this copy); set_ptr: *set_ptr = set_val (the tiles' fit counter
+// before a build).
 constexpr int PF_LANES = 4;
 __global__ void s_pad_fill(int n, int pp, const int32_t* __restrict__ rp, const int32_t* __restrict__ act,
                            const int32_t* __restrict__ ovo, const int32_t* __restrict__ nbr,
-                           const double* __restrict__ D, ESlot* __restrict__ sl) {
+                           const double* __restrict__ D, ESlot* __restrict__ sl, int32_t* __restrict__ set_ptr,
+                           int32_t set_val) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0 && set_ptr) *set_ptr = set_val;
   const int v = (int)(t / PF_LANES), q = (int)(t % PF_LANES);
   if (v >= n) return;
-  const int b = rp[v], L = act[v], oo = ovo[v];
+  const int b = rp[v], L = ovo ? act[v] : min(act[v], pp), oo = ovo ? ovo[v] : 0;
   for (int k = q; k < L; k += PF_LANES) {
     const int64_t idx = k < pp ? (int64_t)v * pp + k : (int64_t)n * pp + oo + (k - pp);
     sl[idx] = ESlot{D[b + k], nbr[b + k], 0};
@@ -431,6 +439,11 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // tovf[t*TOV + offset + k - tpp].  Positions of a level are the same range as
 // its local indices, so a sweep's prefix of levels is a prefix of positions.  *fit_min = min over tiles of fit: a launch
 // may run up to fit_min + 1 sweeps.
+// Slot k of node v comes from the padded copy psl[v*pp + k] when psl is set
+// and k < pp (readable together with the list length), else from the node's
+// row, whose active slots come first: D[rp[v] + k], nbr[rp[v] + k].  Every
+// block_GLOB list has more than pp slots, so a node needs its row offset
+// before some of its slots anyway, and those builds run without the copy.
 // The build runs T threads (TL by default: every level pass and the record
 // pass are one pass over the local nodes) for the tiles of the W-thread sweep.
 // With Fg set, the build also takes the gradient of the new lists at the
@@ -442,8 +455,9 @@ __device__ __forceinline__ void tile_max_to(double v, unsigned long long* slot) 
 // whose launch this saves.
 template <int W, int T = TCfg<W>::TL>
 __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restrict__ tst, int pp, int tpp, int depth,
-                                                   const int32_t* __restrict__ act,
-                                                   const int32_t* __restrict__ povo, const ESlot* __restrict__ psl,
+                                                   const int32_t* __restrict__ act, const int32_t* __restrict__ rp,
+                                                   const double* __restrict__ D, const int32_t* __restrict__ nbr,
+                                                   const ESlot* __restrict__ psl,
                                                    int32_t* __restrict__ hdr, int4* __restrict__ tnode,
                                                    TSlot* __restrict__ tslot, TSlot* __restrict__ tovf,
                                                    int32_t* __restrict__ fit_min, const double* __restrict__ Fg,
@@ -465,8 +479,12 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
   const int t = blockIdx.x, tid = threadIdx.x;
   const int q = tid & (LN - 1), g = tid / LN;
   const int t0 = tst[t], c0 = tst[t + 1] - t0;  // the tile: ids [t0, t0 + c0)
-  // slot k >= pp of node v, whose overflow offset is ov = povo[v]
-  auto slot_at = [&](int ov, int k) -> const ESlot& { return psl[(int64_t)n * pp + ov + (k - pp)]; };
+  const bool cp = psl != nullptr;  // the padded copy holds slots k < pp
+  // slot k of node v, whose row starts at b
+  auto slot_at = [&](int v, int b, int k) -> ESlot {
+    if (cp && k < pp) return psl[(int64_t)v * pp + k];
+    return ESlot{D[b + k], nbr[b + k], 0};
+  };
   for (int i = tid; i < THSZ; i += TW) {
     hkey[i] = -1;
     hval[i] = -1;
@@ -489,14 +507,14 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
     for (int j0 = lo; j0 < hi; j0 += NG) {
       const int j = j0 + g;
       const bool live = j < hi;
-      int v = 0, L = 0, ov = 0, wq = -1;
+      int v = 0, L = 0, rb = 0, wq = -1;
       if (live) {
         v = lg[j];
         L = act[v];
-        ov = povo[v];
-        // slot q of the padded row read with the list length, not after it
+        rb = rp[v];
+        // slot q of the padded copy read with the list length, not after it
         // (in bounds for any length): one dependent global read per level
-        if (q < pp) wq = psl[(int64_t)v * pp + q].nbr;
+        if (cp && q < pp) wq = psl[(int64_t)v * pp + q].nbr;
       }
       // claim a hash entry for neighbour w (has: the slot exists) outside the
       // tile and not seen yet; local indices for the wave's fresh nodes come
@@ -536,10 +554,29 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
           }
         }
       };
-      for (int k = q;; k += LN) {
-        const bool has = live && k < L;
-        if (!__any(has)) break;
-        claim(has, has ? (k == q && q < pp ? wq : k < pp ? psl[(int64_t)v * pp + k].nbr : slot_at(ov, k).nbr) : -1);
+      auto nbr_at = [&](int k) { return cp && k < pp ? psl[(int64_t)v * pp + k].nbr : nbr[rb + k]; };
+      // the node's slots q, q + LN, ... below KB LN read together, then claimed
+      constexpr int KB = 3;
+      int wk[KB];
+#pragma unroll
+      for (int i = 0; i < KB; ++i) {
+        const int k = q + LN * i;
+        wk[i] = -1;
+        if (live && k < L) wk[i] = (cp && i == 0 && q < pp) ? wq : nbr_at(k);
+      }
+      bool more = true;
+#pragma unroll
+      for (int i = 0; i < KB; ++i) {
+        const bool has = live && q + LN * i < L;
+        if (more && !__any(has)) more = false;
+        if (more) claim(has, wk[i]);
+      }
+      if (more) {
+        for (int k = q + LN * KB;; k += LN) {
+          const bool has = live && k < L;
+          if (!__any(has)) break;
+          claim(has, has ? nbr_at(k) : -1);
+        }
       }
     }
     __syncthreads();
@@ -604,22 +641,28 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
     if (kj[r] >= 0) s_pos[atomicAdd(&s_key[kj[r]], 1)] = (short)(tid + r * TW);
   }
   __syncthreads();
-  // records: every pass's list lengths and first slots are read before any
-  // is used (one dependent global level for the whole pass set)
+  // records: every pass's list lengths and row offsets are read before any
+  // is used, then every pass's slots q and q + LN (rq, rq2) together
   constexpr int RP = (TL + NG - 1) / NG;
-  int rv[RP], rL[RP], rov[RP];
-  ESlot rq[RP];
+  int rv[RP], rL[RP], rb[RP];
+  ESlot rq[RP], rq2[RP];
 #pragma unroll
   for (int r = 0; r < RP; ++r) {
     const int j = g + r * NG;
-    rv[r] = rL[r] = rov[r] = 0;
+    rv[r] = rL[r] = rb[r] = 0;
+    rq[r] = rq2[r] = ESlot{0.0, 0, 0};
     if (j < hi) {
       const int v = lg[s_pos[j]];
       rv[r] = v;
       rL[r] = act[v];
-      rov[r] = povo[v];
-      if (q < pp) rq[r] = psl[(int64_t)v * pp + q];
+      rb[r] = rp[v];
+      if (cp && q < pp) rq[r] = psl[(int64_t)v * pp + q];
     }
+  }
+#pragma unroll
+  for (int r = 0; r < RP; ++r) {
+    if (!(cp && q < pp) && q < rL[r]) rq[r] = slot_at(rv[r], rb[r], q);
+    if (q + LN < rL[r]) rq2[r] = slot_at(rv[r], rb[r], q + LN);
   }
 #pragma unroll
   for (int r = 0; r < RP; ++r) {  // the four lanes of a node run together
@@ -637,7 +680,7 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
     if (lane == 63 && incl > 0) base = atomicAdd(&s_ov, incl);
     base = __shfl(base, 63);
     if (j >= hi) continue;
-    const int v = rv[r], L = rL[r], ov = rov[r], jj = s_pos[j];
+    const int v = rv[r], L = rL[r], b = rb[r], jj = s_pos[j];
     const int ovo = __shfl(base + incl - x, 0, LN);
     if (q == 0) tnode[(int64_t)t * TL + j] = make_int4(v, L, ovo, jj);
     auto rec = [&](int k, const ESlot& e) {
@@ -661,11 +704,9 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
       if (k < tpp) tslot[((int64_t)t * TL + j) * tpp + k] = ts;
       else if (ovo + (k - tpp) < TOV) tovf[(int64_t)t * TOV + ovo + (k - tpp)] = ts;
     };
-    if (q < L) {
-      if (q < pp) rec(q, rq[r]);
-      else rec(q, slot_at(ov, q));
-    }
-    for (int k = q + LN; k < L; k += LN) rec(k, k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k));
+    if (q < L) rec(q, rq[r]);
+    if (q + LN < L) rec(q + LN, rq2[r]);
+    for (int k = q + 2 * LN; k < L; k += LN) rec(k, slot_at(v, b, k));
     // padded slots past the list: D = +inf and the node itself, so that a
     // sweep computes them unmasked, as +inf (above every sentinel of Top3)
     for (int k = q; k < tpp; k += LN)
@@ -683,10 +724,12 @@ __global__ void __launch_bounds__(T) s_tile_build(int n, const int32_t* __restri
       int v = 0;
       if (own) {
         v = rv[r];
-        const int L = rL[r], ov = rov[r];
+        const int L = rL[r], b = rb[r];
         const double Fv = Fg[v];
-        for (int k = q; k < L; k += LN) {
-          const ESlot& e = k < pp ? psl[(int64_t)v * pp + k] : slot_at(ov, k);
+        if (q < L) t3.push(dval(rq[r].D, Fv, Fg[rq[r].nbr], v, rq[r].nbr));
+        if (q + LN < L) t3.push(dval(rq2[r].D, Fv, Fg[rq2[r].nbr], v, rq2[r].nbr));
+        for (int k = q + 2 * LN; k < L; k += LN) {
+          const ESlot e = slot_at(v, b, k);
           t3.push(dval(e.D, Fv, Fg[e.nbr], v, e.nbr));
         }
       }
@@ -1491,6 +1534,12 @@ struct Runner {
   // tile build instead of a separate pass (F2M_BUILD_GRAD=0: the pass)
   bool build_grad = true;
   bool grad_ready = false;  // region 0 holds the last build's max |g|
+  // what the tile build reads (F2M_BUILD_ROWS): 0 the full padded lists,
+  // built with every list rebuild; 1 the rows, plus the padded copy of the
+  // first slots for the block_EF lists; 2 the rows only.  The full padded
+  // lists are then built only before a padded sweep uses them.
+  int build_rows = 1;
+  bool pad_ready = false;  // the padded lists (psl, povo) hold the current lists
   int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
   // with profiling on: sweeps run on the current lists not yet counted, and a
@@ -1593,6 +1642,7 @@ struct Runner {
       tw_glob = shape(env_int("F2M_TILE_W_GLOB", TW_GLOB));
       selh = env_int("F2M_TILE_SELH", 1) != 0;
       build_grad = env_int("F2M_BUILD_GRAD", 1) != 0;
+      build_rows = std::min(2, std::max(0, env_int("F2M_BUILD_ROWS", 1)));
       // tile nodes of a workgroup shape: the same whole number of tiles on every CU, as large as
       // the shape's cap allows (256 nodes for 384 threads, 512 for 768)
       auto balanced = [&](int tw_) {
@@ -1614,11 +1664,14 @@ struct Runner {
         const int ts = kind ? ts_glob : ts_ef, T = (int)((n + ts - 1) / ts);
         hipLaunchKernelGGL(s_tile_uniform, dim3((T + 256) / 256), dim3(256), 0, c->stream, T, n, ts, tile_starts(kind));
       }
+      hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), INT_MAX);
       F2M_LAUNCH_CHECK(c);
     }
     return F2M_OK;
   }
-  int32_t* tile_fit() const { return w.thdr.as<int32_t>() + (int64_t)ntiles * THDR; }
+  // the tiles' fit counter: INT_MAX until a build takes the min of its tiles'
+  // fits into it; reset to INT_MAX where the host reads it
+  int32_t* tile_fit() const { return w.thdr.as<int32_t>() + tile_cap * THDR; }
   // first ids of the tiles of list kind k (0: block_EF, 1: block_GLOB) and n
   int32_t* tile_starts(int k) const { return w.thdr.as<int32_t>() + tile_cap * THDR + 16 + k * (tile_cap + 1); }
   void set_width(int32_t fit_min) {
@@ -1692,6 +1745,10 @@ struct Runner {
       }
       if (!padded) F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
     }
+    if (padded && !pad_ready) {
+      f2m_status sp = pad_lists();
+      if (sp) return sp;
+    }
     if (padded) launch_padded(write, frozen, mslots, timed ? c->prof_ev[2 * prof_used] : nullptr,
                               timed ? c->prof_ev[2 * prof_used + 1] : nullptr);
     else switch (lpn) {
@@ -1760,6 +1817,17 @@ struct Runner {
     c->prof.active_slots = active_slots;
     return F2M_OK;
   }
+  // The padded lists in full (s_sweep_pad's layout): overflow offsets by an
+  // exclusive scan of max(act - pp, 0), then the copy.
+  f2m_status pad_lists() {
+    F2M_HIP(c, scan_excess_i32(c, I(w.act), pp, I(w.povo), n, nullptr, 0));
+    hipLaunchKernelGGL(s_pad_fill, dim3(grid_for((int64_t)n * PF_LANES, BS)), dim3(BS), 0, c->stream, n, pp, rp,
+                       I(w.act), (const int32_t*)I(w.povo), I(w.nbr), w.D.as<double>(), w.psl.as<ESlot>(),
+                       (int32_t*)nullptr, 0);
+    F2M_LAUNCH_CHECK(c);
+    pad_ready = true;
+    return F2M_OK;
+  }
   // update_EF (graph_cuda.cu:229-252): offsets = scan(act), copy the active
   // prefixes.  With profiling on, the sweeps run on the old lists are counted
   // first (on the device).
@@ -1773,15 +1841,26 @@ struct Runner {
     if (!padded || c->prof_every > 0) F2M_HIP(c, scan_i32_to_i32(c, I(w.act), I(w.efoff), n, nullptr));
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
-      // overflow offsets: exclusive scan of max(act - pp, 0); the same launch
-      // sets the tiles' fit counter to the halo depth the build will run
       const int kind = long_lists ? 1 : 0;
       const int depth = depth_next[kind] > 0 ? std::min(fuse - 1, depth_next[kind]) : fuse - 1;
       const bool tiled = fuse > 1;
       if (tiled) ntiles = (n + (long_lists ? ts_glob : ts_ef) - 1) / (long_lists ? ts_glob : ts_ef);
-      F2M_HIP(c, scan_excess_i32(c, I(w.act), pp, I(w.povo), n, tiled ? tile_fit() : nullptr, (int32_t)depth));
-      hipLaunchKernelGGL(s_pad_fill, dim3(grid_for((int64_t)n * PF_LANES, BS)), bs, 0, c->stream, n, pp, rp, I(w.act),
-                         I(w.povo), I(w.nbr), w.D.as<double>(), w.psl.as<ESlot>());
+      // the tile build reads the lists' rows; the padded copy of the first pp
+      // slots only for the block_EF lists (one dependent read fewer per
+      // level), the full padded lists (overflow offsets by a scan) only when a
+      // padded sweep runs on them (pad_lists).  (n = 100k Gaussian mixture:
+      // 19.30 -> 18.82 ms against building the full padded lists with every
+      // rebuild.)
+      const bool copy = tiled && build_rows < 2 && !long_lists;
+      pad_ready = false;
+      if (!tiled || build_rows == 0) {
+        f2m_status sp = pad_lists();
+        if (sp) return sp;
+      } else if (copy) {
+        hipLaunchKernelGGL(s_pad_fill, dim3(grid_for((int64_t)n * PF_LANES, BS)), bs, 0, c->stream, n, pp, rp,
+                           I(w.act), (const int32_t*)nullptr, I(w.nbr), w.D.as<double>(), w.psl.as<ESlot>(),
+                           (int32_t*)nullptr, 0);
+      }
       if (fuse > 1) {
         tpp = long_lists ? 6 : 4;
         const int ts = long_lists ? ts_glob : ts_ef;
@@ -1795,7 +1874,8 @@ struct Runner {
         auto build = [&](auto kern, int threads) {
           hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), 0, c->stream, n, (const int32_t*)tile_starts(kind), pp,
                              tpp, depth,
-                             (const int32_t*)I(w.act), (const int32_t*)I(w.povo), (const ESlot*)w.psl.as<ESlot>(),
+                             (const int32_t*)I(w.act), rp, (const double*)w.D.as<double>(), (const int32_t*)I(w.nbr),
+                             (pad_ready || copy) ? (const ESlot*)w.psl.as<ESlot>() : (const ESlot*)nullptr,
                              w.thdr.as<int32_t>(), w.tnode.as<int4>(), w.tslot.as<TSlot>(), w.tovf.as<TSlot>(),
                              tile_fit(), bg ? (const double*)F() : (const double*)nullptr, bg ? maxs : nullptr);
         };
@@ -1875,6 +1955,8 @@ struct Runner {
       F2M_HIP(c, hipMemcpyAsync(hfit, tile_fit(), 4, hipMemcpyDeviceToHost, c->stream));
       F2M_HIP(c, hipStreamSynchronize(c->stream));
       set_width(*hfit);
+      hipLaunchKernelGGL(s_set_i32, dim3(1), dim3(64), 0, c->stream, tile_fit(), INT_MAX);
+      F2M_LAUNCH_CHECK(c);
     }
     out = width;
     return F2M_OK;

@@ -104,6 +104,28 @@ def test_schedule_phases_bit_exact(ctx, orc, phase):
     assert np.array_equal(res["F"], r["F"])
 
 
+@pytest.mark.parametrize("rows", ["0", "1", "2"])
+def test_single_sweeps_after_tile_builds(ctx, orc, monkeypatch, rows):
+    # one sweep per block_GLOB round and block_EF chunks of 7 (remainders of one
+    # sweep): single sweeps run the padded kernel on lists whose tiles were
+    # just built, so the padded lists are built for them on demand when the
+    # tile build reads the rows (F2M_BUILD_ROWS 1, 2)
+    from f2m_amd.solver import default_opts
+    monkeypatch.setenv("F2M_BUILD_ROWS", rows)
+    x, y = W.uniform_int(5000, 10**6, 29)
+    g = ctx.knn_build(x, y, 12, W.COST_NINT)
+    rp, nb, c = ctx.graph_csr(g)
+    kw = dict(glob_sweeps=1, glob_rounds=40, check_every=7)
+    say("solve uniform5000 single sweeps, F2M_BUILD_ROWS", rows)
+    st = ctx.solve(g, default_opts(**kw))
+    res = ctx.result()
+    r = orc.solve(rp, nb, c, orc.default_opts(**kw))
+    so = r["stats"]
+    for k in ("sweeps", "glob_rounds", "outer_rounds", "max_grad", "status"):
+        assert st[k] == so[k], (k, st[k], so[k])
+    assert np.array_equal(res["F"], r["F"]) and np.array_equal(res["x"], r["x"])
+
+
 def _full_compare(ctx, orc, x, y, K, mode, name, lp_opt=None, exact_x=True):
     g = ctx.knn_build(x, y, K, mode)
     rp, nb, c = ctx.graph_csr(g)
@@ -176,10 +198,12 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     assert np.array_equal(ctx.result()["F"], res["F"]) and np.array_equal(ctx.result()["x"], res["x"])
     monkeypatch.delenv("F2M_PART_LANES")
     # block_GLOB tile sweeps reading the owner's half from shared memory instead of a register;
-    # the convergence checks' gradient from a separate pass instead of the tile build
-    for var in ("F2M_TILE_SELH", "F2M_BUILD_GRAD"):
-        monkeypatch.setenv(var, "0")
-        say("solve C2 Morton,", var, "0")
+    # the convergence checks' gradient from a separate pass instead of the tile build; the tile
+    # build reading the full padded lists (ROWS 0) or the rows only (ROWS 2)
+    for var, val in (("F2M_TILE_SELH", "0"), ("F2M_BUILD_GRAD", "0"), ("F2M_BUILD_ROWS", "0"),
+                     ("F2M_BUILD_ROWS", "2")):
+        monkeypatch.setenv(var, val)
+        say("solve C2 Morton,", var, val)
         s2 = ctx.solve(g)
         assert s2["sweeps"] == st["sweeps"] and s2["objective"] == st["objective"], var
         assert s2["glob_rounds"] == st["glob_rounds"] and s2["outer_rounds"] == st["outer_rounds"], var
