@@ -1527,6 +1527,10 @@ struct Runner {
   // than the last build of that kind kept (0: fuse - 1).  Levels no launch
   // uses are not built; the sweeps' bits do not depend on the width.
   int depth_next[2] = {0, 0};
+  // builds of each kind so far: the level past the last fit is probed on
+  // every probe_every-th build only (F2M_TILE_PROBE; 1: every build)
+  int builds[2] = {0, 0};
+  int probe_every = 8;
   int pending_kind = 0;
   int tpp = 4;    // padded slots per local node on the current lists
   bool selh = true;  // block_GLOB tile sweeps take the owner's half from a register (F2M_TILE_SELH)
@@ -1643,6 +1647,7 @@ struct Runner {
       selh = env_int("F2M_TILE_SELH", 1) != 0;
       build_grad = env_int("F2M_BUILD_GRAD", 1) != 0;
       build_rows = std::min(2, std::max(0, env_int("F2M_BUILD_ROWS", 1)));
+      probe_every = std::max(1, env_int("F2M_TILE_PROBE", 8));
       // tile nodes of a workgroup shape: the same whole number of tiles on every CU, as large as
       // the shape's cap allows (256 nodes for 384 threads, 512 for 768)
       auto balanced = [&](int tw_) {
@@ -1842,7 +1847,11 @@ struct Runner {
     const dim3 gn(grid_for(n, BS)), bs(BS);
     if (padded) {
       const int kind = long_lists ? 1 : 0;
-      const int depth = depth_next[kind] > 0 ? std::min(fuse - 1, depth_next[kind]) : fuse - 1;
+      // (a probe level that does not fit in every tile costs a level pass, a
+      // larger sort and more records on every build: n = 100k Gaussian
+      // mixture, block_GLOB lists fit four levels and the fifth is probed)
+      const bool probe = builds[kind]++ % probe_every == 0;
+      const int depth = depth_next[kind] > 0 ? std::min(fuse - 1, depth_next[kind] - (probe ? 0 : 1)) : fuse - 1;
       const bool tiled = fuse > 1;
       if (tiled) ntiles = (n + (long_lists ? ts_glob : ts_ef) - 1) / (long_lists ? ts_glob : ts_ef);
       // the tile build reads the lists' rows; the padded copy of the first pp

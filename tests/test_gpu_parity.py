@@ -199,9 +199,10 @@ def test_full_solve_c2_morton_every_sweep_width(ctx, orc, monkeypatch):
     monkeypatch.delenv("F2M_PART_LANES")
     # block_GLOB tile sweeps reading the owner's half from shared memory instead of a register;
     # the convergence checks' gradient from a separate pass instead of the tile build; the tile
-    # build reading the full padded lists (ROWS 0) or the rows only (ROWS 2)
+    # build reading the full padded lists (ROWS 0) or the rows only (ROWS 2); every build probing
+    # the halo level past the last fit (PROBE 1)
     for var, val in (("F2M_TILE_SELH", "0"), ("F2M_BUILD_GRAD", "0"), ("F2M_BUILD_ROWS", "0"),
-                     ("F2M_BUILD_ROWS", "2")):
+                     ("F2M_BUILD_ROWS", "2"), ("F2M_TILE_PROBE", "1")):
         monkeypatch.setenv(var, val)
         say("solve C2 Morton,", var, val)
         s2 = ctx.solve(g)
