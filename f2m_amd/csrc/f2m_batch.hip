@@ -906,7 +906,7 @@ __device__ int wg_exact(const Inst& I, const f2m_opts& o, double dmin, double* F
       if (threadIdx.x == 0) xcnt[par ^ 1] = 0;  // L list of round r+1, filled by the R step
       for (int i = gid; i < nl; i += NG) {
         const int v = I.xactL[i];
-        x_discharge_l(X, v, (int)I.b(v), (int)I.b(v + 1), lane, eps,
+        x_discharge_l_any<4>(X, v, (int)I.b(v), (int)I.b(v + 1), lane, eps,
                       [&](int w) { I.xactR[atomicAdd(&xcnt[2 + par], 1)] = w; });
       }
       __syncthreads();
@@ -914,7 +914,7 @@ __device__ int wg_exact(const Inst& I, const f2m_opts& o, double dmin, double* F
       if (threadIdx.x == 0) xcnt[2 + (par ^ 1)] = 0;  // R list of round r+1, filled by its L step
       for (int i = gid; i < nr; i += NG) {
         const int w = I.xactR[i];
-        x_discharge_r(X, w, (int)I.b(w), (int)I.b(w + 1), lane, eps,
+        x_discharge_r_any<4>(X, w, (int)I.b(w), (int)I.b(w + 1), lane, eps,
                       [&](int v) { I.xactL[atomicAdd(&xcnt[par ^ 1], 1)] = v; });
       }
       __syncthreads();
