@@ -151,24 +151,39 @@ __global__ void x_zero_ring(int32_t* cnt) {
   if (threadIdx.x < 2 * RING) cnt[threadIdx.x] = 0;
 }
 
-// excesses of every node (COUNT; else as maintained) and the active lists
-// of round r
-// index of this lane's entry in a list that the wave's lanes with pred
-// append to: one atomic per wave on the list's count (-1 without pred).  The
-// lists' order does not matter to the rounds that read them.
-__device__ __forceinline__ int wave_append(int32_t* cnt, bool pred) {
-  const unsigned long long m = __ballot(pred);
-  if (!m) return -1;
-  const int lane = threadIdx.x & 63, first = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == first) base = atomicAdd(cnt, __popcll(m));
-  base = __shfl(base, first);
-  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+// Entries of the L and R lists of a workgroup's threads: each wave's lanes
+// with a flag take consecutive slots of a shared-memory count, then one
+// atomic per workgroup and list reserves them in the global list (one per
+// wave had ~1,500 waves' atomics queue on two addresses at n = 100k).
+// Every thread of the workgroup calls it; returns the global indices (-1:
+// no entry).
+__device__ __forceinline__ int2 block_append2(int32_t* cl, int32_t* cr, bool pl, bool pr) {
+  __shared__ int s_n[2], s_base[2];
+  if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long ml = __ballot(pl), mr = __ballot(pr);
+  int ol = 0, orr = 0;
+  if (lane == 0) {
+    if (ml) ol = atomicAdd(&s_n[0], __popcll(ml));
+    if (mr) orr = atomicAdd(&s_n[1], __popcll(mr));
+  }
+  ol = __shfl(ol, 0) + __popcll(ml & below);
+  orr = __shfl(orr, 0) + __popcll(mr & below);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base[0] = s_n[0] ? atomicAdd(cl, s_n[0]) : 0;
+  if (threadIdx.x == 1) s_base[1] = s_n[1] ? atomicAdd(cr, s_n[1]) : 0;
+  __syncthreads();
+  return make_int2(pl ? s_base[0] + ol : -1, pr ? s_base[1] + orr : -1);
 }
 
-// every lane of the launch runs the appends (whole waves), so no early return
+// excesses of every node (COUNT; else as maintained) and the active lists
+// of round r; every thread of the launch runs the appends (whole
+// workgroups), so no early return
+constexpr int XE_T = 1024;
 template <bool COUNT>
-__global__ void x_excess(XDev X, int64_t r) {
+__global__ void __launch_bounds__(XE_T) x_excess(XDev X, int64_t r) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   int el = 0, er = 0;
   if (v < X.n) {
@@ -180,10 +195,9 @@ __global__ void x_excess(XDev X, int64_t r) {
     }
   }
   const int k = (int)(r % RING);
-  const int il = wave_append(&X.cnt[2 * k], el > 0);
-  if (il >= 0) ((r & 1) ? X.actL1 : X.actL0)[il] = v;
-  const int ir = wave_append(&X.cnt[2 * k + 1], er > 0);
-  if (ir >= 0) X.actR[ir] = v;
+  const int2 ix = block_append2(&X.cnt[2 * k], &X.cnt[2 * k + 1], el > 0, er > 0);
+  if (ix.x >= 0) ((r & 1) ? X.actL1 : X.actL0)[ix.x] = v;
+  if (ix.y >= 0) X.actR[ix.y] = v;
 }
 
 // A workgroup's appends to a global list: collected in shared memory, then
@@ -600,8 +614,9 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     if (phases > 0)
       hipLaunchKernelGGL(x_flow_sat_lanes, dim3(grid_for((int64_t)n * XF_LANES, bs.x)), bs, 0, st, X, eps);
     hipLaunchKernelGGL(x_zero_ring, dim3(1), dim3(64), 0, st, X.cnt);
-    if (phases == 0) hipLaunchKernelGGL((x_excess<true>), gn, bs, 0, st, X, r);
-    else hipLaunchKernelGGL((x_excess<false>), gn, bs, 0, st, X, r);
+    const dim3 ge(grid_for(n, XE_T)), be(XE_T);
+    if (phases == 0) hipLaunchKernelGGL((x_excess<true>), ge, be, 0, st, X, r);
+    else hipLaunchKernelGGL((x_excess<false>), ge, be, 0, st, X, r);
     F2M_LAUNCH_CHECK(c);
     int chunk = 2;
     bool in_wg = false, first = true, have_cert = false;
