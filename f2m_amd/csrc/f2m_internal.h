@@ -354,6 +354,69 @@ __device__ __attribute__((noinline)) int qpart_mask(int64_t base, int size, unsi
   return p;
 }
 
+// The swaps of qpart_mask shared by the NL lanes of a node (lane q of
+// 0..NL-1; every lane holds the same bits m): the same pairs, lane q
+// carrying out pairs q, q + NL, q + 2 NL, ... in groups of G whose loads are
+// issued together.  A slot is in at most one pair, so the lanes' swaps are
+// independent.  Returns p = popcount(m & the list) on every lane.
+template <int NL, int G>
+__device__ __forceinline__ int qpart_mask_lanes(int64_t base, int size, unsigned long long m, int q,
+                                                int* __restrict__ nbr, double* __restrict__ D,
+                                                int* __restrict__ orig) {
+  const unsigned long long full = size >= 64 ? ~0ull : ((1ull << size) - 1ull);
+  m &= full;
+  const int p = __popcll(m);
+  const unsigned long long left = p >= 64 ? ~0ull : ((1ull << p) - 1ull);
+  unsigned long long A = ~m & left;  // above the threshold, left of the pivot (ascending)
+  unsigned long long B = m & ~left;  // at or below it, right of the pivot (descending); |A| = |B|
+  auto drop = [&]() {                // the next pair, not this lane's
+    A &= A - 1;
+    B &= ~(1ull << (63 - __builtin_clzll(B)));
+  };
+  for (int i = 0; i < q && A; ++i) drop();
+  while (A) {
+    int si[G], sj[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      si[g] = -1;
+      sj[g] = -1;
+      if (A) {
+        si[g] = __builtin_ctzll(A);
+        sj[g] = 63 - __builtin_clzll(B);
+        drop();
+        for (int i = 1; i < NL && A; ++i) drop();
+      }
+    }
+    int tn[G], to[G], un[G], uo[G];
+    double tD[G], uD[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (si[g] >= 0) {
+        const int64_t a = base + si[g], b = base + sj[g];
+        tn[g] = nbr[a];
+        to[g] = orig[a];
+        tD[g] = D[a];
+        un[g] = nbr[b];
+        uo[g] = orig[b];
+        uD[g] = D[b];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (si[g] >= 0) {
+        const int64_t a = base + si[g], b = base + sj[g];
+        nbr[a] = un[g];
+        orig[a] = uo[g];
+        D[a] = uD[g];
+        nbr[b] = tn[g];
+        orig[b] = to[g];
+        D[b] = tD[g];
+      }
+    }
+  }
+  return p;
+}
+
 __device__ inline int qpart_B(int v, int64_t base, int size, double thr, int* __restrict__ nbr,
                               double* __restrict__ D, int* __restrict__ orig,
                               const double* __restrict__ F) {

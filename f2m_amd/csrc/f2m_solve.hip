@@ -1093,9 +1093,10 @@ __global__ void s_part_v4(int n, const int32_t* __restrict__ rp, const int32_t* 
 // slots q, q + 4, ... (a row's slots are loaded four at a time, consecutive),
 // the threshold is the max of the first kk d's in slot order (kk <= 8; they
 // sit in the lanes' first two chunks), the predicate bits are or-ed across
-// the four lanes, and the group's first lane runs qpart_mask on them: the
-// bits, swaps and pivot of part_v4_node (p >= kk >= 3, so qpart_EF's raise
-// loop never runs).  Lists above 64 slots, kk < 3 or kk > 8 take
+// the four lanes, and the lanes carry out qpart_mask's swaps for them
+// between them (qpart_mask_lanes): the bits, swaps and pivot of
+// part_v4_node (p >= kk >= 3, so qpart_EF's raise loop never runs).  A
+// row's first 32 slots are read together.  Lists above 64 slots, kk < 3 or kk > 8 take
 // part_v4_node on the first lane.
 constexpr int PV4_LANES = 4;
 __device__ __attribute__((noinline)) int part_v4_long(int v, int64_t b, int dg, int kk, int32_t* __restrict__ nbr,
@@ -1121,31 +1122,33 @@ __global__ void __launch_bounds__(BS) s_part_v4_q(int n, const int32_t* __restri
     return;
   }
   const double Fv = F[v];
-  // slots q and q + 4: the threshold's candidates
-  double d0 = 0.0, d1 = 0.0;
+  // the row's first 32 slots read together (lane q: slots q, q + 4, ..., q + 28)
+  constexpr int U = 8;
+  double d[U];
   {
-    int w[2];
-    double dd[2];
+    int w[U];
+    double dd[U];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int k = q + 4 * u < dg ? q + 4 * u : dg - 1;  // past the list: repeats the last slot, unused
       w[u] = nbr[b + k];
       dd[u] = D[b + k];
     }
-    d0 = dval(dd[0], Fv, F[w[0]], v, w[0]);
-    d1 = dval(dd[1], Fv, F[w[1]], v, w[1]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = dval(dd[u], Fv, F[w[u]], v, w[u]);
   }
-  double thr = 0.0;
+  double thr = 0.0;  // the max of the first kk d's in slot order: slots q (d[0]) and q + 4 (d[1])
 #pragma unroll
   for (int i = 0; i < 8; ++i)
     if (i < kk) {
-      const double di = __shfl(i < 4 ? d0 : d1, i & 3, PV4_LANES);
+      const double di = __shfl(i < 4 ? d[0] : d[1], i & 3, PV4_LANES);
       thr = (i == 0 || di > thr) ? di : thr;
     }
   unsigned long long m = 0;
-  if (q < dg && d0 <= thr) m |= 1ull << q;
-  if (q + 4 < dg && d1 <= thr) m |= 1ull << (q + 4);
-  for (int k0 = 8; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (q + 4 * u < dg && d[u] <= thr) m |= 1ull << (q + 4 * u);
+  for (int k0 = 4 * U; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
     int w[4];
     double dd[4];
 #pragma unroll
@@ -1165,14 +1168,15 @@ __global__ void __launch_bounds__(BS) s_part_v4_q(int n, const int32_t* __restri
   hi |= __shfl_xor(hi, 1, PV4_LANES);
   lo |= __shfl_xor(lo, 2, PV4_LANES);
   hi |= __shfl_xor(hi, 2, PV4_LANES);
-  if (q == 0) act[v] = qpart_mask(b, dg, ((unsigned long long)hi << 32) | lo, nbr, D, orig);
+  const int p = qpart_mask_lanes<PV4_LANES, 4>(b, dg, ((unsigned long long)hi << 32) | lo, q, nbr, D, orig);
+  if (q == 0) act[v] = p;
 }
 
 // check_opt's partition (part_v2_node: lists of 10 or more slots at thr 1.0)
 // with four lanes per node as s_part_v4_q.  Each pass: the four lanes read
 // the row in its current order (four slots at a time), or their predicate
 // bits and take the least d above thr (a min: the same in any order); the
-// first lane swaps (qpart_mask).  While fewer than 3 slots lie at or below
+// lanes swap (qpart_mask_lanes).  While fewer than 3 slots lie at or below
 // thr, thr is raised (raise_thr, the same on every lane) and the permuted row
 // is partitioned again, as qpart_EF does.  Lists above 64 slots take
 // part_v2_node on the first lane.
@@ -1202,7 +1206,27 @@ __global__ void __launch_bounds__(BS) s_part_v2_q(int n, const int32_t* __restri
   for (int pass = 0;; ++pass) {
     unsigned long long m = 0;
     double ab = 1.0 / 0.0;
-    for (int k0 = 0; k0 < dg; k0 += 16) {  // 16 slots (four per lane) per step
+    {  // the row's first 32 slots read together (lane q: slots q, q + 4, ..., q + 28)
+      constexpr int U = 8;
+      int w[U];
+      double dd[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = q + 4 * u < dg ? q + 4 * u : dg - 1;  // past the list: the last slot, unused
+        w[u] = nbr[b + k];
+        dd[u] = D[b + k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = q + 4 * u;
+        if (k < dg) {
+          const double d = dval(dd[u], Fv, F[w[u]], v, w[u]);
+          if (d <= thr) m |= 1ull << k;
+          else ab = d < ab ? d : ab;
+        }
+      }
+    }
+    for (int k0 = 32; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
       int w[4];
       double dd[4];
 #pragma unroll
@@ -1229,14 +1253,12 @@ __global__ void __launch_bounds__(BS) s_part_v2_q(int n, const int32_t* __restri
       const double a2 = __shfl_xor(ab, o, PV4_LANES);
       ab = a2 < ab ? a2 : ab;
     }
-    int p = 0;
-    if (q == 0) p = qpart_mask(b, dg, ((unsigned long long)hi << 32) | lo, nbr, D, orig);
-    p = __shfl(p, 0, PV4_LANES);
+    const int p = qpart_mask_lanes<PV4_LANES, 4>(b, dg, ((unsigned long long)hi << 32) | lo, q, nbr, D, orig);
     if (p >= 3) {
       if (q == 0) act[v] = p;
       return;
     }
-    __threadfence_block();  // the first lane's swaps, before the lanes read the row again
+    __threadfence_block();  // the lanes' swaps, before they read the row again
     thr = raise_thr(thr, ab);
   }
 }
