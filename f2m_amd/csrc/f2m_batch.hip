@@ -50,7 +50,7 @@ struct BArgs {
   ESlot* ef;       // compacted active lists, per instance at its slot base
                    // (16 B per slot: D doubles, then nbr ints, see Inst)
   int32_t fcap;    // doubles of dynamic shared memory for the potentials
-  int32_t sweep_regs;    // 1: runs of sweeps hold the lists in registers where they can (wg_sweeps_reg)
+  int32_t sweep_regs;    // 1, 2: runs of sweeps hold the lists in registers where they can (wg_sweeps_reg; 2: SH)
   const int32_t* order;  // instance ids of this bucket, decreasing size
   int32_t n_order;
   int32_t* queue;
@@ -314,8 +314,13 @@ __device__ __forceinline__ double wg_pass(const PView I, int cur, bool write, bo
 // Returns the max |g| of the last sweep when want_max (else 0); barriers
 // inside.  Shared scratch of the sort: the buffer F[c0 ^ 1] (n doubles,
 // undefined on entry), which holds its KSORT + n ints when n >= KSORT.
+//  - SH: a slot's owner half comes from the owner's register and only the
+//    neighbour's from shared memory, the two put in id order by selects:
+//    one shared-memory read per slot instead of two.  At 2000 nodes a sweep
+//    reads 128 KB of halves otherwise, ~1000 cycles of a CU's shared-memory
+//    bandwidth.
 constexpr int KSORT = 64;  // list lengths sorted by; longer lists share the last bin
-template <int BS, int NP, bool SORT, int RK>
+template <int BS, int NP, bool SORT, int RK, bool SH = false>
 __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0, int cnt, bool use_frozen,
                                                          double* sh, bool want_max) {
   const int n = P.n;
@@ -360,6 +365,7 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
   int vv[NP];
   double Dk[NP][RK];
   uint32_t oa[NP][RK], ob[NP][RK];
+  uint32_t lom[NP];  // SH: bit k = the owner has the lower id of slot k
   int bb[NP], L[NP];
   double fv[NP];
   bool fz[NP];
@@ -372,6 +378,7 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
     L[p] = 0;
     fv[p] = 0.0;
     fz[p] = false;
+    lom[p] = 0u;
     if (v < n) {
       bb[p] = P.efoff[v];
       L[p] = P.efoff[v + 1] - bb[p];
@@ -388,8 +395,14 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
       }
       const int u = v < n ? v : 0;
       const bool lo = u < w;
-      oa[p][k] = (uint32_t)(lo ? u : w) * 8u;
-      ob[p][k] = (uint32_t)(lo ? w : u) * 8u;
+      if (SH) {
+        oa[p][k] = (uint32_t)w * 8u;
+        ob[p][k] = 0u;
+        if (lo) lom[p] |= 1u << k;
+      } else {
+        oa[p][k] = (uint32_t)(lo ? u : w) * 8u;
+        ob[p][k] = (uint32_t)(lo ? w : u) * 8u;
+      }
 
     }
   }
@@ -408,11 +421,20 @@ __device__ __attribute__((noinline)) double wg_sweeps_reg(const PView P, int c0,
     double* dst = b_dyn + (Q ^ 1) * n;
     double d[NP][RK];
 #pragma unroll
-    for (int p = 0; p < NP; ++p)
+    for (int p = 0; p < NP; ++p) {
+      const double hv = 0.5 * fv[p];
 #pragma unroll
-      for (int k = 0; k < RK; ++k)
-        d[p][k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
-                  *reinterpret_cast<const double*>(src + ob[p][k]);
+      for (int k = 0; k < RK; ++k) {
+        if (SH) {
+          const double hw = *reinterpret_cast<const double*>(src + oa[p][k]);
+          const bool lo = ((lom[p] >> k) & 1u) != 0;
+          d[p][k] = (Dk[p][k] + (lo ? hv : hw)) + (lo ? hw : hv);
+        } else {
+          d[p][k] = (Dk[p][k] + *reinterpret_cast<const double*>(src + oa[p][k])) +
+                    *reinterpret_cast<const double*>(src + ob[p][k]);
+        }
+      }
+    }
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       const int v = vv[p];
@@ -624,12 +646,20 @@ struct Sched {
     double m = 0.0;
     if (regs && cnt > 0 && fm == FS_TWO && P.lofs >= 0 && P.n <= 2 * BS && P.n >= KSORT) {
       // sorted ownership where the lists are longer than the registers hold on average
-      if (P.ltot > PASS_U * P.n)
+      if (regs == 2) {  // the owners' halves from registers
+        if (P.ltot > PASS_U * P.n)
+          m = P.n <= BS ? wg_sweeps_reg<BS, 1, true, PASS_U, true>(P, c, cnt, frozen, shm, want_max)
+                        : wg_sweeps_reg<BS, 2, true, PASS_U, true>(P, c, cnt, frozen, shm, want_max);
+        else
+          m = P.n <= BS ? wg_sweeps_reg<BS, 1, false, PASS_U, true>(P, c, cnt, frozen, shm, want_max)
+                        : wg_sweeps_reg<BS, 2, false, PASS_U, true>(P, c, cnt, frozen, shm, want_max);
+      } else if (P.ltot > PASS_U * P.n) {
         m = P.n <= BS ? wg_sweeps_reg<BS, 1, true, PASS_U>(P, c, cnt, frozen, shm, want_max)
                       : wg_sweeps_reg<BS, 2, true, PASS_U>(P, c, cnt, frozen, shm, want_max);
-      else
+      } else {
         m = P.n <= BS ? wg_sweeps_reg<BS, 1, false, PASS_U>(P, c, cnt, frozen, shm, want_max)
                       : wg_sweeps_reg<BS, 2, false, PASS_U>(P, c, cnt, frozen, shm, want_max);
+      }
       cur = c ^ (cnt & 1);
       sweeps += cnt;
       return m;
@@ -1629,7 +1659,8 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   a.efoff = c->b_efoff.as<int32_t>();
   a.ef = c->b_ef.as<ESlot>();
   a.fcap = 0;  // set per bucket
-  a.sweep_regs = env_int("F2M_BATCH_REGS", 1) != 0 ? 1 : 0;
+  // 2: with the owners' halves from registers (C5's forward 19.8-20.1 -> 19.6 ms)
+  a.sweep_regs = std::max(0, std::min(2, env_int("F2M_BATCH_REGS", 2)));
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;

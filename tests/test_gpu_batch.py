@@ -114,8 +114,9 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
     monkeypatch.delenv("F2M_BATCH_LDS")
     # runs of sweeps with the lists held in registers (the default where the
     # instance fits two nodes per thread) against the pass that reads them
-    # from shared memory every sweep
-    for regs in ("0", "1"):
+    # from shared memory every sweep, with the owners' halves read from shared memory (1) or from
+    # registers (2, the default)
+    for regs in ("0", "1", "2"):
         monkeypatch.setenv("F2M_BATCH_REGS", regs)
         say("batch sweeps with lists in registers", regs)
         sts = ctx.solve_batched(b)
