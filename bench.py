@@ -138,8 +138,9 @@ def main():
     log("timed", args.steps, "solves:", ms, "ms/solve")
 
     # roofline of the dominant sweep kernel, from separate instrumented solves:
-    # every sample_every-th launch of the full width timed with kernel
-    # start/end events, its algorithmic bytes counted on the device
+    # each run of back-to-back launches of the full width timed with one pair
+    # of HIP events on the solver's stream (the gaps between them included),
+    # their algorithmic bytes counted on the device
     ctx.set_profile(args.sample_every)
     prof_tot = {"sweep_launches": 0, "sweep_samples": 0, "sweep_ms_sampled": 0.0, "sweeps": 0,
                 "sampled_sweeps": 0, "sampled_alg_bytes": 0.0}

@@ -1566,7 +1566,7 @@ struct Runner {
   // lists are then built only before a padded sweep uses them.
   int build_rows = 1;
   bool pad_ready = false;  // the padded lists (psl, povo) hold the current lists
-  int64_t launches = 0, wide_launches = 0, sampled_sweeps = 0;
+  int64_t launches = 0, sampled_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
   // with profiling on: sweeps run on the current lists not yet counted, and a
   // device accumulator of active slots x sweeps (no host round trip per list)
@@ -1789,6 +1789,7 @@ struct Runner {
     if (timed) {
       if (!padded) F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
       ++prof_used;
+      ++sampled_launches;
       sample_slots(1);
     }
     if (write) {
@@ -1838,7 +1839,7 @@ struct Runner {
     c->prof.sweeps = sweeps;
     c->prof.sampled_sweeps = sampled_sweeps;
     c->prof.sampled_alg_bytes = kAlgSlotBytes * (double)sslots + kAlgNodeBytes * (double)n * (double)sampled_sweeps;
-    c->prof.sweep_samples = (int64_t)prof_used;
+    c->prof.sweep_samples = sampled_launches;
     c->prof.sweep_ms_sampled = ms;
     c->prof.sweep_alg_bytes = alg_bytes;
     c->prof.active_slots = active_slots;
@@ -1993,19 +1994,13 @@ struct Runner {
     return F2M_OK;
   }
   template <int S>
-  void launch_tile_t(bool frozen, unsigned long long* mslots, hipEvent_t ev0, hipEvent_t ev1) {
+  void launch_tile_t(bool frozen, unsigned long long* mslots) {
     const int TW = tw;
     auto go = [&](auto kern) {
-      if (ev0)
-        hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, ev0, ev1, 0,
-                              (const int32_t*)w.thdr.as<int32_t>(), (const int4*)w.tnode.as<int4>(),
-                              (const TSlot*)w.tslot.as<TSlot>(), (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(),
-                              Fo(), (const uint8_t*)w.frozen.as<uint8_t>(), mslots, tile_xcd);
-      else
-        hipLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, (const int32_t*)w.thdr.as<int32_t>(),
-                           (const int4*)w.tnode.as<int4>(), (const TSlot*)w.tslot.as<TSlot>(),
-                           (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(), Fo(),
-                           (const uint8_t*)w.frozen.as<uint8_t>(), mslots, tile_xcd);
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(TW), 0, c->stream, (const int32_t*)w.thdr.as<int32_t>(),
+                         (const int4*)w.tnode.as<int4>(), (const TSlot*)w.tslot.as<TSlot>(),
+                         (const TSlot*)w.tovf.as<TSlot>(), (const double*)F(), Fo(),
+                         (const uint8_t*)w.frozen.as<uint8_t>(), mslots, tile_xcd);
     };
     if (tpp == 6 && tw == TW_GLOB && selh) {
       if (frozen) go(s_sweep_tile<S, true, 6, TW_GLOB, true>);
@@ -2024,42 +2019,23 @@ struct Runner {
       else go(s_sweep_tile<S, false, 4, TW_EF>);
     }
   }
-  // S sweeps in one launch over the tiles; timed like launch_sweep (only
-  // launches of the configured width, the dominant kernel, are sampled)
+  // S sweeps in one launch over the tiles
   f2m_status launch_tile(int S, bool frozen, unsigned long long* mslots) {
-    const bool wide = S == fuse && tpp == 4;  // the dominant kernel: block_EF lists at full width
-    const bool timed = wide && c->prof_every > 0 && (wide_launches % c->prof_every) == 0;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timed) {
-      while (c->prof_ev.size() < 2 * (prof_used + 1)) {
-        hipEvent_t e;
-        F2M_HIP(c, hipEventCreate(&e));
-        c->prof_ev.push_back(e);
-      }
-      e0 = c->prof_ev[2 * prof_used];
-      e1 = c->prof_ev[2 * prof_used + 1];
-    }
     switch (S) {
-      case 1: launch_tile_t<1>(frozen, mslots, e0, e1); break;
-      case 2: launch_tile_t<2>(frozen, mslots, e0, e1); break;
-      case 3: launch_tile_t<3>(frozen, mslots, e0, e1); break;
-      case 4: launch_tile_t<4>(frozen, mslots, e0, e1); break;
-      case 5: launch_tile_t<5>(frozen, mslots, e0, e1); break;
-      case 6: launch_tile_t<6>(frozen, mslots, e0, e1); break;
-      case 7: launch_tile_t<7>(frozen, mslots, e0, e1); break;
-      case 8: launch_tile_t<8>(frozen, mslots, e0, e1); break;
-      case 9: launch_tile_t<9>(frozen, mslots, e0, e1); break;
-      case 10: launch_tile_t<10>(frozen, mslots, e0, e1); break;
-      case 11: launch_tile_t<11>(frozen, mslots, e0, e1); break;
-      default: launch_tile_t<12>(frozen, mslots, e0, e1); break;
+      case 1: launch_tile_t<1>(frozen, mslots); break;
+      case 2: launch_tile_t<2>(frozen, mslots); break;
+      case 3: launch_tile_t<3>(frozen, mslots); break;
+      case 4: launch_tile_t<4>(frozen, mslots); break;
+      case 5: launch_tile_t<5>(frozen, mslots); break;
+      case 6: launch_tile_t<6>(frozen, mslots); break;
+      case 7: launch_tile_t<7>(frozen, mslots); break;
+      case 8: launch_tile_t<8>(frozen, mslots); break;
+      case 9: launch_tile_t<9>(frozen, mslots); break;
+      case 10: launch_tile_t<10>(frozen, mslots); break;
+      case 11: launch_tile_t<11>(frozen, mslots); break;
+      default: launch_tile_t<12>(frozen, mslots); break;
     }
     F2M_LAUNCH_CHECK(c);
-    if (timed) {
-      ++prof_used;
-      sampled_sweeps += S;
-      sample_slots(S);
-    }
-    if (wide) ++wide_launches;
     ++launches;
     w.cur ^= 1;
     sweeps += S;
@@ -2073,10 +2049,31 @@ struct Runner {
     if (fuse > 1 && count > 1)
       if ((s = tile_width(wd))) return s;
     if (wd > 1) {
-      for (int done = 0; done < count;) {
+      // instrumented solves: one event pair around every run of two or more
+      // back-to-back launches of the configured width on the block_EF lists
+      // (the dominant kernel), the gaps between them included; a shorter
+      // last launch stays outside
+      const int full = count / wd;
+      const bool win = c->prof_every > 0 && wd == fuse && tpp == 4 && full >= 2;
+      if (win) {
+        while (c->prof_ev.size() < 2 * (prof_used + 1)) {
+          hipEvent_t e;
+          F2M_HIP(c, hipEventCreate(&e));
+          c->prof_ev.push_back(e);
+        }
+        F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used], c->stream));
+      }
+      for (int done = 0, i = 0; done < count; ++i) {
         const int S = std::min(wd, count - done);
         done += S;
         if ((s = launch_tile(S, frozen, done == count ? mlast : nullptr))) return s;
+        if (win && i + 1 == full) {
+          F2M_HIP(c, hipEventRecord(c->prof_ev[2 * prof_used + 1], c->stream));
+          ++prof_used;
+          sampled_launches += full;
+          sampled_sweeps += (int64_t)full * wd;
+          sample_slots(full * wd);
+        }
       }
       return F2M_OK;
     }

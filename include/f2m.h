@@ -160,13 +160,16 @@ f2m_status f2m_ctx_synchronize(f2m_ctx* ctx);
 
 /* ---- instrumentation (new; the reference prints gettimeofday deltas,
  * graph_cuda_define.hpp:29-34) -------------------------------------------------
- * With sample_every > 0, every sample_every-th sweep launch of f2m_solve
- * of the dominant kind (F2M_FUSE sweeps on spatial tiles, default 10; one
- * sweep with F2M_FUSE=1) is timed by a pair of HIP events that the runtime
- * records at the kernel's own start and end (hipExtLaunchKernelGGL, context
- * stream); f2m_get_profile reports the sampled kernel time, the sweeps and
- * algorithmic bytes of those launches, and the algorithmic bytes of all
- * sweeps:
+ * With sample_every > 0, the sweep launches of f2m_solve of the dominant
+ * kind are timed with HIP events on the context stream: on spatial tiles
+ * (F2M_FUSE sweeps per launch, default 10, on the block_EF lists) every run
+ * of two or more back-to-back launches of that width, by one event pair
+ * around the run (the gaps between its launches included; the sweeps'
+ * bits do not depend on it); with F2M_FUSE=1 every sample_every-th single
+ * sweep, by events the runtime records at the kernel's own start and end.
+ * f2m_get_profile reports the timed launches and their time, the sweeps
+ * and algorithmic bytes of those launches, and the algorithmic bytes of
+ * all sweeps:
  * 8 B per active slot (nbr int32 + a 4-byte cost) + 20 B per node (list
  * offset, F in, F out).  The solver stores D as a double (12 B per slot
  * moved); the count is the algorithm's, not the layout's. */
