@@ -91,19 +91,49 @@ __global__ void x_prices(int n, const double* __restrict__ F, double dmin, doubl
   pR[v] = -p;
 }
 
-// init (SAT = false): f = [rc < 0]; later phases: saturate / empty the arcs
-// violating eps-optimality at the new eps, excesses updated with the flips
-template <bool SAT>
-__global__ void x_flow(XDev X, double eps) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < X.n) x_flow_node<SAT, SAT>(X.S, v, X.rp[v], X.rp[v + 1], eps);
+// The initial flows f = [rc < 0] with XF_LANES lanes per node (lane l:
+// slots l, l + 8, ...; the arithmetic of x_flow_node<false>), and the
+// excesses from them (lane sums of the flows out and in, then the first lane)
+constexpr int XF_LANES = 8;
+__global__ void x_flow_init_lanes(XDev X) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
+  if (v >= X.n) return;  // the lanes of a node leave together
+  const XState& S = X.S;
+  const int b = X.rp[v], e = X.rp[v + 1];
+  const double pv = S.pL[v];
+  for (int s = b + l; s < e; s += XF_LANES) {
+    const double r = (S.c[s] + pv) - S.pR[S.nbr[s]];
+    S.f[s] = r < 0.0 ? 1 : 0;
+  }
+}
+__global__ void x_count_lanes(XDev X) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
+  if (v >= X.n) return;
+  const XState& S = X.S;
+  const int b = X.rp[v], e = X.rp[v + 1];
+  int out = 0, in = 0;
+  for (int s = b + l; s < e; s += XF_LANES) {
+    out += S.f[s];
+    in += S.f[S.mir[s]];
+  }
+#pragma unroll
+  for (int o = 1; o < XF_LANES; o <<= 1) {
+    out += __shfl_xor(out, o, XF_LANES);
+    in += __shfl_xor(in, o, XF_LANES);
+  }
+  if (l == 0) {
+    S.eL[v] = 2 - out;
+    S.eR[v] = in - 2;
+  }
 }
 
-// The later phases' x_flow<true> with XF_LANES lanes per node: lane l flips
-// the arcs of slots l, l + 8, ... (each arc's test and flip are its own; eR
-// moves by atomics as there) and the lanes' changes of eL[v], integers, are
-// summed with shuffles before the first lane applies them.
-constexpr int XF_LANES = 8;
+// Later phases: the arcs violating eps-optimality at the new eps saturated
+// or emptied (x_flow_node<true, true>) with XF_LANES lanes per node: lane l
+// flips the arcs of slots l, l + 8, ... (each arc's test and flip are its
+// own; eR moves by atomics as there) and the lanes' changes of eL[v],
+// integers, are summed with shuffles before the first lane applies them.
 __global__ void x_flow_sat_lanes(XDev X, double eps) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
@@ -178,21 +208,16 @@ __device__ __forceinline__ int2 block_append2(int32_t* cl, int32_t* cr, bool pl,
   return make_int2(pl ? s_base[0] + ol : -1, pr ? s_base[1] + orr : -1);
 }
 
-// excesses of every node (COUNT; else as maintained) and the active lists
-// of round r; every thread of the launch runs the appends (whole
-// workgroups), so no early return
+// the active lists of round r from the excesses (counted by x_count_lanes
+// in the first phase, then maintained); every thread of the launch runs
+// the appends (whole workgroups), so no early return
 constexpr int XE_T = 1024;
-template <bool COUNT>
 __global__ void __launch_bounds__(XE_T) x_excess(XDev X, int64_t r) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   int el = 0, er = 0;
   if (v < X.n) {
-    if (COUNT) {
-      x_excess_node(X.S, v, X.rp[v], X.rp[v + 1], el, er);
-    } else {
-      el = X.S.eL[v];
-      er = X.S.eR[v];
-    }
+    el = X.S.eL[v];
+    er = X.S.eR[v];
   }
   const int k = (int)(r % RING);
   const int2 ix = block_append2(&X.cnt[2 * k], &X.cnt[2 * k + 1], el > 0, er > 0);
@@ -444,16 +469,16 @@ __global__ void __launch_bounds__(256) x_det_final(const double* __restrict__ pa
 }
 
 // outputs: x and its X-flag per canonical slot, F = (pL - pR) / d_min
-__global__ void x_out(XDev X, double dmin, double* __restrict__ xc, uint8_t* __restrict__ fc,
+// (a thread per slot, and per node for F)
+__global__ void x_out(XDev X, int64_t m, double dmin, double* __restrict__ xc, uint8_t* __restrict__ fc,
                       double* __restrict__ F) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= X.n) return;
-  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < m) {
     const double x = 0.5 * (double)(X.S.f[s] + X.S.f[X.S.mir[s]]);
     xc[s] = x;
     fc[s] = x > 0.0 ? 1 : 0;
   }
-  F[v] = (X.S.pL[v] - X.S.pR[v]) / dmin;
+  if (s < X.n) F[s] = (X.S.pL[s] - X.S.pR[s]) / dmin;
 }
 __global__ void x_uf_init(int n, int32_t* par) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,7 +587,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   if (hv.flag[1]) { rc = -2; return F2M_OK; }
   const bool integral = hv.flag[2] == 0;
   hipLaunchKernelGGL(x_prices, gn, bs, 0, st, n, F, dmin, X.S.pL, X.S.pR);
-  hipLaunchKernelGGL((x_flow<false>), gn, bs, 0, st, X, 0.0);
+  const dim3 gl(grid_for((int64_t)n * XF_LANES, bs.x));
+  hipLaunchKernelGGL(x_flow_init_lanes, gl, bs, 0, st, X);
   F2M_LAUNCH_CHECK(c);
   double eps = o.exact_eps0 * dmin;
   int64_t r = 0;          // rounds launched (ring / parity index)
@@ -615,8 +641,8 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
       hipLaunchKernelGGL(x_flow_sat_lanes, dim3(grid_for((int64_t)n * XF_LANES, bs.x)), bs, 0, st, X, eps);
     hipLaunchKernelGGL(x_zero_ring, dim3(1), dim3(64), 0, st, X.cnt);
     const dim3 ge(grid_for(n, XE_T)), be(XE_T);
-    if (phases == 0) hipLaunchKernelGGL((x_excess<true>), ge, be, 0, st, X, r);
-    else hipLaunchKernelGGL((x_excess<false>), ge, be, 0, st, X, r);
+    if (phases == 0) hipLaunchKernelGGL(x_count_lanes, gl, bs, 0, st, X);
+    hipLaunchKernelGGL(x_excess, ge, be, 0, st, X, r);
     F2M_LAUNCH_CHECK(c);
     int chunk = 2;
     bool in_wg = false, first = true, have_cert = false;
@@ -686,7 +712,7 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
     f2m_status s = cert();
     if (s) return s;
   }
-  hipLaunchKernelGGL(x_out, gn, bs, 0, st, X, dmin, xc, fc, F);
+  hipLaunchKernelGGL(x_out, dim3(grid_for(std::max<int64_t>(m, n), XBS)), bs, 0, st, X, m, dmin, xc, fc, F);
   int32_t* par = X.actL0;  // free now
   hipLaunchKernelGGL(x_uf_init, gn, bs, 0, st, n, par);
   hipLaunchKernelGGL(x_half_union, gn, bs, 0, st, X, par);

@@ -1445,20 +1445,41 @@ __device__ __forceinline__ double block_sum_det(double v) {
     for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += wsum[k];
   return s;
 }
-// partials: [0] objective, [1] sum F, [2] sum min(0,d)
+// partials: [0] objective, [1] sum F, [2] sum min(0,d).  SUM_LANES lanes per
+// node (lane q: slots q, q + 4, ..., read together four at a time); the
+// order of the sums is fixed by the launch shape, so the results are
+// deterministic.
+constexpr int SUM_LANES = 4;
 __global__ void __launch_bounds__(BS) s_sums(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ gnbr,
                                              const double* __restrict__ gcost, const double* __restrict__ xc,
                                              const int32_t* __restrict__ wnbr, const double* __restrict__ wD,
                                              const double* __restrict__ F, double* __restrict__ part) {
   double obj = 0.0, sf = 0.0, sneg = 0.0;
-  for (int v = blockIdx.x * BS + threadIdx.x; v < n; v += gridDim.x * BS) {
+  const int q = threadIdx.x % SUM_LANES;
+  const int64_t groups = (int64_t)gridDim.x * (BS / SUM_LANES);
+  for (int64_t vv = (int64_t)blockIdx.x * (BS / SUM_LANES) + threadIdx.x / SUM_LANES; vv < n; vv += groups) {
+    const int v = (int)vv;
     const int b = rp[v], e = rp[v + 1];
     const double Fv = F[v];
-    sf += Fv;
-    for (int s = b; s < e; ++s) {
-      if (gnbr[s] > v) obj += xc[s] * gcost[s];
-      const int w = wnbr[s];
-      if (w > v) sneg += fmin(0.0, dval(wD[s], Fv, F[w], v, w));
+    if (q == 0) sf += Fv;
+    for (int s0 = b + q; s0 < e; s0 += 4 * SUM_LANES) {
+      int gw[4], ww[4];
+      double x[4], cs[4], dd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int s = s0 + u * SUM_LANES < e ? s0 + u * SUM_LANES : e - 1;  // past the list: unused
+        gw[u] = gnbr[s];
+        x[u] = xc[s];
+        cs[u] = gcost[s];
+        ww[u] = wnbr[s];
+        dd[u] = wD[s];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (s0 + u * SUM_LANES >= e) continue;
+        if (gw[u] > v) obj += x[u] * cs[u];
+        if (ww[u] > v) sneg += fmin(0.0, dval(dd[u], Fv, F[ww[u]], v, ww[u]));
+      }
     }
   }
   obj = block_sum_det(obj);
@@ -1485,8 +1506,11 @@ __global__ void s_sums_final(const double* __restrict__ part, double* __restrict
 
 // MAX edge rank (graph_tour.hpp:169-180): over edges in X, min of the two
 // endpoint ranks (0-based position in the node's cost-sorted list).
+// With mir (the exact stage's mirror slots: v's position in w's list), the
+// reverse rank is read instead of searched for.
 __global__ void s_maxrank(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ gnbr,
-                          const uint8_t* __restrict__ fc, int32_t* __restrict__ out) {
+                          const uint8_t* __restrict__ fc, const int32_t* __restrict__ mir,
+                          int32_t* __restrict__ out) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   int best = 0;
@@ -1494,8 +1518,12 @@ __global__ void s_maxrank(int n, const int32_t* __restrict__ rp, const int32_t* 
     const int w = gnbr[s];
     if (w <= v || !fc[s]) continue;
     int r = s - rp[v];
-    for (int q = rp[w]; q < rp[w + 1]; ++q)
-      if (gnbr[q] == v) { r = min(r, q - rp[w]); break; }
+    if (mir) {
+      r = min(r, mir[s] - rp[w]);
+    } else {
+      for (int q = rp[w]; q < rp[w + 1]; ++q)
+        if (gnbr[q] == v) { r = min(r, q - rp[w]); break; }
+    }
     best = max(best, r);
   }
   if (best > 0) atomicMax(out, best);
@@ -1565,6 +1593,7 @@ struct Runner {
   // first slots for the block_EF lists; 2 the rows only.  The full padded
   // lists are then built only before a padded sweep uses them.
   int build_rows = 1;
+  bool mir_ok = false;  // the exact stage left every canonical slot's mirror in ws.xmir
   bool pad_ready = false;  // the padded lists (psl, povo) hold the current lists
   int64_t launches = 0, sampled_launches = 0, sampled_sweeps = 0;
   // instrumentation: active slots of the current lists, totals
@@ -2301,6 +2330,7 @@ struct Runner {
     int rc = 0;
     f2m_status s = exact_stage(c, w, n, m, rp, gnbr, gcost, dmin, o, F(), w.xc.as<double>(), fc(), I(w.root), t, rc);
     if (s) return s;
+    mir_ok = rc >= 0;  // every slot's mirror found
     z.exact_ran = t.exact_ran;
     if (rc >= 0) {
       z = t;
@@ -2315,7 +2345,8 @@ struct Runner {
                        I(w.nbr), w.D.as<double>(), F(), part);
     hipLaunchKernelGGL(s_sums_final, dim3(1), dim3(BS), 0, c->stream, part, sums);
     hipLaunchKernelGGL(s_zero_i32, dim3(1), dim3(64), 0, c->stream, ibuf, (int64_t)4);
-    hipLaunchKernelGGL(s_maxrank, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, gnbr, fc(), ibuf);
+    hipLaunchKernelGGL(s_maxrank, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, rp, gnbr, fc(),
+                       mir_ok ? (const int32_t*)w.xmir.as<int32_t>() : (const int32_t*)nullptr, ibuf);
     F2M_LAUNCH_CHECK(c);
     double hs[3];
     int32_t mr = 0;
