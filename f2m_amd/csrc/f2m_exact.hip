@@ -484,22 +484,40 @@ __global__ void x_uf_init(int n, int32_t* par) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) par[v] = v;
 }
+// XF_LANES lanes per node (lane l: slots l, l + 8, ...); the unions link
+// the larger root under the smaller, so the roots do not depend on their order
 __global__ void x_half_union(XDev X, int32_t* par) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
   if (v >= X.n) return;
-  for (int s = X.rp[v]; s < X.rp[v + 1]; ++s) {
+  for (int s = X.rp[v] + l; s < X.rp[v + 1]; s += XF_LANES) {
     const int w = X.S.nbr[s];
     if (w > v && X.S.f[s] + X.S.f[X.S.mir[s]] == 1) uf_unite(par, v, w);
   }
 }
-__global__ void x_half_root(XDev X, int32_t* par, int32_t* __restrict__ root, int32_t* __restrict__ nroots) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= X.n) return;
+__global__ void __launch_bounds__(1024) x_half_root(XDev X, int32_t* par, int32_t* __restrict__ root,
+                                                    int32_t* __restrict__ nroots) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(t / XF_LANES), l = (int)(t % XF_LANES);
   bool has = false;
-  for (int s = X.rp[v]; s < X.rp[v + 1] && !has; ++s) has = X.S.f[s] + X.S.f[X.S.mir[s]] == 1;
-  const int r = has ? uf_find(par, v) : -1;
-  root[v] = r;
-  if (r == v) atomicAdd(nroots, 1);
+  if (v < X.n)
+    for (int s = X.rp[v] + l; s < X.rp[v + 1]; s += XF_LANES) has = has || X.S.f[s] + X.S.f[X.S.mir[s]] == 1;
+  // any lane of the node
+  const unsigned long long any = __ballot(has);
+  const int g0 = (threadIdx.x & 63) & ~(XF_LANES - 1);
+  has = ((any >> g0) & ((1ull << XF_LANES) - 1ull)) != 0;
+  int r = -1;
+  if (v < X.n && l == 0) {
+    r = has ? uf_find(par, v) : -1;
+    root[v] = r;
+  }
+  const unsigned long long nr = __ballot(v < X.n && l == 0 && r == v);
+  if ((threadIdx.x & 63) == 0 && nr) atomicAdd(&cnt, __popcll(nr));
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nroots, cnt);  // one per workgroup
 }
 
 __global__ void x_zero_i32(int32_t* p, int k) {
@@ -715,9 +733,10 @@ f2m_status exact_stage(f2m_ctx* c, SolveWs& w, int n, int64_t m, const int32_t* 
   hipLaunchKernelGGL(x_out, dim3(grid_for(std::max<int64_t>(m, n), XBS)), bs, 0, st, X, m, dmin, xc, fc, F);
   int32_t* par = X.actL0;  // free now
   hipLaunchKernelGGL(x_uf_init, gn, bs, 0, st, n, par);
-  hipLaunchKernelGGL(x_half_union, gn, bs, 0, st, X, par);
+  hipLaunchKernelGGL(x_half_union, gl, bs, 0, st, X, par);
   hipLaunchKernelGGL(x_zero_i32, dim3(1), dim3(64), 0, st, flag + 4, 1);
-  hipLaunchKernelGGL(x_half_root, gn, bs, 0, st, X, par, root, flag + 4);
+  hipLaunchKernelGGL(x_half_root, dim3(grid_for((int64_t)n * XF_LANES, 1024)), dim3(1024), 0, st, X, par, root,
+                     flag + 4);
   F2M_LAUNCH_CHECK(c);
   {
     f2m_status s = post();

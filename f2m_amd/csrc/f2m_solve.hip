@@ -2360,16 +2360,21 @@ struct Runner {
   }
 };
 
-__global__ void s_count_roots(int n, const int32_t* __restrict__ root, int32_t* out) {
+// one atomic per workgroup (one per wave queued ~1,500 atomics on one
+// address at n = 100k)
+__global__ void __launch_bounds__(1024) s_count_roots(int n, const int32_t* __restrict__ root, int32_t* out) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  int f = (v < n && root[v] == v) ? 1 : 0;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
-  if ((threadIdx.x & 63) == 0 && f) atomicAdd(out, f);
+  const unsigned long long m = __ballot(v < n && root[v] == v);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&cnt, __popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(out, cnt);
 }
 
 f2m_status Runner::count_roots(int32_t& out) {
-  hipLaunchKernelGGL(s_count_roots, dim3(grid_for(n, BS)), dim3(BS), 0, c->stream, n, I(w.root), ibuf);
+  hipLaunchKernelGGL(s_count_roots, dim3(grid_for(n, 1024)), dim3(1024), 0, c->stream, n, I(w.root), ibuf);
   F2M_LAUNCH_CHECK(c);
   F2M_HIP(c, hipMemcpyAsync(&out, ibuf, 4, hipMemcpyDeviceToHost, c->stream));
   F2M_HIP(c, hipStreamSynchronize(c->stream));
