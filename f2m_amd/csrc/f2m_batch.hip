@@ -51,6 +51,7 @@ struct BArgs {
                    // (16 B per slot: D doubles, then nbr ints, see Inst)
   int32_t fcap;    // doubles of dynamic shared memory for the potentials
   int32_t sweep_regs;    // 1, 2: runs of sweeps hold the lists in registers where they can (wg_sweeps_reg; 2: SH)
+  int32_t part_lanes;    // 1: partitions with PV4_LANES lanes per node (part_v4_lanes / part_v2_lanes)
   const int32_t* order;  // instance ids of this bucket, decreasing size
   int32_t n_order;
   int32_t* queue;
@@ -581,6 +582,19 @@ __device__ int wg_compact(int n, P pred, int32_t* out, int* shi) {
   return total;
 }
 
+// The lane partitions as calls: inlined, their registers join the
+// persistent solve's and it spills at 1024 threads
+__device__ __attribute__((noinline)) int b_part_v4_lanes(int v, int q, int64_t b, int dg, int kk,
+                                                         int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                         int32_t* __restrict__ orig, const double* __restrict__ F) {
+  return part_v4_lanes(v, q, b, dg, kk, nbr, D, orig, F);
+}
+__device__ __attribute__((noinline)) int b_part_v2_lanes(int v, int q, int64_t b, int dg, double thr,
+                                                         int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                         int32_t* __restrict__ orig, const double* __restrict__ F) {
+  return part_v2_lanes(v, q, b, dg, thr, nbr, D, orig, F);
+}
+
 template <int BS, int LPN>
 struct Sched {
   const Inst& I;
@@ -591,6 +605,7 @@ struct Sched {
   PView pv;               // the passes' view of I (lists in shared or global memory)
   int dynb;               // bytes of shared memory at b_dyn
   int regs = 0;           // runs of sweeps may hold the lists in registers (wg_sweeps_reg)
+  int plane = 0;          // partitions with PV4_LANES lanes per node
   int cur = 0;
   int64_t sweeps = 0;
   double eps;
@@ -616,6 +631,27 @@ struct Sched {
     } else {
       const double* F = b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
       for (int v = threadIdx.x; v < I.n; v += BS) I.act[v] = part(v, F);
+    }
+    __syncthreads();
+  }
+  // the same with PV4_LANES lanes per node: part(v, q, F) is called by the
+  // node's four lanes together and its result is valid on lane q = 0
+  template <class P>
+  __device__ void partition_lanes(P part) {
+    fs_load();
+    const int q = threadIdx.x % PV4_LANES;
+    if (fmode == FS_HBM) {
+      const double* F = I.F[cur];
+      for (int v = threadIdx.x / PV4_LANES; v < I.n; v += BS / PV4_LANES) {
+        const int p = part(v, q, F);
+        if (q == 0) I.act[v] = p;
+      }
+    } else {
+      const double* F = b_dyn + (fmode == FS_TWO ? cur * I.n : 0);
+      for (int v = threadIdx.x / PV4_LANES; v < I.n; v += BS / PV4_LANES) {
+        const int p = part(v, q, F);
+        if (q == 0) I.act[v] = p;
+      }
     }
     __syncthreads();
   }
@@ -687,9 +723,15 @@ struct Sched {
     int go = 0;
     for (; go < o.glob_rounds; ++go) {
       sweep_n(o.glob_sweeps, false);
-      partition([&](int v, const double* F) {
-        return part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
-      });
+      if (plane)
+        partition_lanes([&](int v, int q, const double* F) {
+          const int dg = I.deg[v];
+          return dg < o.glob_k ? dg : b_part_v4_lanes(v, q, I.b(v), dg, o.glob_k, I.nbr, I.D, I.orig, F);
+        });
+      else
+        partition([&](int v, const double* F) {
+          return part_v4_node(v, I.b(v), I.deg[v], o.glob_k, I.nbr, I.D, I.orig, F);
+        });
       rebuild();
       const double M = grad_act();
       if (M < o.glob_exit) { ++go; break; }
@@ -697,9 +739,15 @@ struct Sched {
     return go;
   }
   __device__ double check_opt() {
-    partition([&](int v, const double* F) {
-      return part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
-    });
+    if (plane)
+      partition_lanes([&](int v, int q, const double* F) {
+        const int dg = I.deg[v];
+        return dg < 10 ? dg : b_part_v2_lanes(v, q, I.b(v), dg, 1.0, I.nbr, I.D, I.orig, F);
+      });
+    else
+      partition([&](int v, const double* F) {
+        return part_v2_node(v, I.b(v), I.deg[v], 1.0, I.nbr, I.D, I.orig, F);
+      });
     rebuild();
     return grad_act();
   }
@@ -1087,6 +1135,7 @@ __device__ __attribute__((noinline)) void solve_instance(const BArgs& a, int ins
   const int fmode = 2 * I.n <= a.fcap ? FS_TWO : (I.n <= a.fcap ? FS_ONE : FS_HBM);
   Sched<BS, LPN> S{I, o, sh, shi, fmode, pview(I), a.fcap * 8};
   S.regs = a.sweep_regs;
+  S.plane = a.part_lanes;
   S.eps = o.err_grad * o.err_comb_mult;
   z.glob_rounds = S.block_glob();  // from F_init on a warm start
   double M = 0.0;
@@ -1661,6 +1710,7 @@ static f2m_status solve_batched_impl(f2m_ctx* c, const f2m_batch* b, const f2m_o
   a.fcap = 0;  // set per bucket
   // 2: with the owners' halves from registers (C5's forward 19.8-20.1 -> 19.6 ms)
   a.sweep_regs = std::max(0, std::min(2, env_int("F2M_BATCH_REGS", 2)));
+  a.part_lanes = env_int("F2M_BATCH_PART_LANES", 1) != 0;
   a.o = oo;
   a.stats = dstats;
   a.cur_out = curs;

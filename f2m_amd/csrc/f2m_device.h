@@ -95,6 +95,153 @@ __device__ inline int part_v2_node(int v, int64_t b, int dg, double thr, int32_t
                                    const double* __restrict__ F) {
   return dg < 10 ? dg : qpart_EF(v, b, dg, thr, nbr, D, orig, F);
 }
+// The same partition with four lanes per node: lane q computes the d of
+// slots q, q + 4, ... (a row's slots are loaded four at a time, consecutive),
+// the threshold is the max of the first kk d's in slot order (kk <= 8; they
+// sit in the lanes' first two chunks), the predicate bits are or-ed across
+// the four lanes, and the lanes carry out qpart_mask's swaps for them
+// between them (qpart_mask_lanes): the bits, swaps and pivot of
+// part_v4_node (p >= kk >= 3, so qpart_EF's raise loop never runs).  A
+// row's first 32 slots are read together.  Lists above 64 slots, kk < 3 or kk > 8 take
+// part_v4_node on the first lane.  The
+// four lanes of a node call together (shuffles of width 4); the result is
+// valid on lane q = 0.  Lists of fewer than kk slots are the caller's.
+constexpr int PV4_LANES = 4;
+__device__ __attribute__((noinline)) int part_v4_long(int v, int64_t b, int dg, int kk, int32_t* __restrict__ nbr,
+                                                      double* __restrict__ D, int32_t* __restrict__ orig,
+                                                      const double* __restrict__ F) {
+  return part_v4_node(v, b, dg, kk, nbr, D, orig, F);
+}
+__device__ __forceinline__ int part_v4_lanes(int v, int q, int64_t b, int dg, int kk, int32_t* __restrict__ nbr,
+                                             double* __restrict__ D, int32_t* __restrict__ orig,
+                                             const double* __restrict__ F) {
+  if (dg > 64 || kk < 3 || kk > 8) return q == 0 ? part_v4_long(v, b, dg, kk, nbr, D, orig, F) : 0;
+  const double Fv = F[v];
+  // the row's first 32 slots read together (lane q: slots q, q + 4, ..., q + 28)
+  constexpr int U = 8;
+  double d[U];
+  {
+    int w[U];
+    double dd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = q + 4 * u < dg ? q + 4 * u : dg - 1;  // past the list: repeats the last slot, unused
+      w[u] = nbr[b + k];
+      dd[u] = D[b + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = dval(dd[u], Fv, F[w[u]], v, w[u]);
+  }
+  double thr = 0.0;  // the max of the first kk d's in slot order: slots q (d[0]) and q + 4 (d[1])
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (i < kk) {
+      const double di = __shfl(i < 4 ? d[0] : d[1], i & 3, PV4_LANES);
+      thr = (i == 0 || di > thr) ? di : thr;
+    }
+  unsigned long long m = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (q + 4 * u < dg && d[u] <= thr) m |= 1ull << (q + 4 * u);
+  for (int k0 = 4 * U; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
+    int w[4];
+    double dd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + q + 4 * u < dg ? k0 + q + 4 * u : dg - 1;
+      w[u] = nbr[b + k];
+      dd[u] = D[b + k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + q + 4 * u;
+      if (k < dg && dval(dd[u], Fv, F[w[u]], v, w[u]) <= thr) m |= 1ull << k;
+    }
+  }
+  unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+  lo |= __shfl_xor(lo, 1, PV4_LANES);
+  hi |= __shfl_xor(hi, 1, PV4_LANES);
+  lo |= __shfl_xor(lo, 2, PV4_LANES);
+  hi |= __shfl_xor(hi, 2, PV4_LANES);
+  return qpart_mask_lanes<PV4_LANES, 4>(b, dg, ((unsigned long long)hi << 32) | lo, q, nbr, D, orig);
+}
+
+// check_opt's partition (part_v2_node: lists of 10 or more slots at thr 1.0)
+// with four lanes per node as s_part_v4_q.  Each pass: the four lanes read
+// the row in its current order (four slots at a time), or their predicate
+// bits and take the least d above thr (a min: the same in any order); the
+// lanes swap (qpart_mask_lanes).  While fewer than 3 slots lie at or below
+// thr, thr is raised (raise_thr, the same on every lane) and the permuted row
+// is partitioned again, as qpart_EF does.  Lists above 64 slots take
+// part_v2_node on the first lane.  Called
+// as part_v4_lanes; lists of fewer than 10 slots are the caller's.
+__device__ __attribute__((noinline)) int part_v2_long(int v, int64_t b, int dg, double thr,
+                                                      int32_t* __restrict__ nbr, double* __restrict__ D,
+                                                      int32_t* __restrict__ orig, const double* __restrict__ F) {
+  return part_v2_node(v, b, dg, thr, nbr, D, orig, F);
+}
+__device__ __forceinline__ int part_v2_lanes(int v, int q, int64_t b, int dg, double thr, int32_t* __restrict__ nbr,
+                                             double* __restrict__ D, int32_t* __restrict__ orig,
+                                             const double* __restrict__ F) {
+  if (dg > 64) return q == 0 ? part_v2_long(v, b, dg, thr, nbr, D, orig, F) : 0;
+  const double Fv = F[v];
+  for (int pass = 0;; ++pass) {
+    unsigned long long m = 0;
+    double ab = 1.0 / 0.0;
+    {  // the row's first 32 slots read together (lane q: slots q, q + 4, ..., q + 28)
+      constexpr int U = 8;
+      int w[U];
+      double dd[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = q + 4 * u < dg ? q + 4 * u : dg - 1;  // past the list: the last slot, unused
+        w[u] = nbr[b + k];
+        dd[u] = D[b + k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = q + 4 * u;
+        if (k < dg) {
+          const double d = dval(dd[u], Fv, F[w[u]], v, w[u]);
+          if (d <= thr) m |= 1ull << k;
+          else ab = d < ab ? d : ab;
+        }
+      }
+    }
+    for (int k0 = 32; k0 < dg; k0 += 16) {  // the rest, 16 slots (four per lane) per step
+      int w[4];
+      double dd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + q + 4 * u < dg ? k0 + q + 4 * u : dg - 1;  // past the list: the last slot, unused
+        w[u] = nbr[b + k];
+        dd[u] = D[b + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + q + 4 * u;
+        if (k < dg) {
+          const double d = dval(dd[u], Fv, F[w[u]], v, w[u]);
+          if (d <= thr) m |= 1ull << k;
+          else ab = d < ab ? d : ab;
+        }
+      }
+    }
+    unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+#pragma unroll
+    for (int o = 1; o < PV4_LANES; o <<= 1) {
+      lo |= __shfl_xor(lo, o, PV4_LANES);
+      hi |= __shfl_xor(hi, o, PV4_LANES);
+      const double a2 = __shfl_xor(ab, o, PV4_LANES);
+      ab = a2 < ab ? a2 : ab;
+    }
+    const int p = qpart_mask_lanes<PV4_LANES, 4>(b, dg, ((unsigned long long)hi << 32) | lo, q, nbr, D, orig);
+    if (p >= 3) return p;
+    __threadfence_block();  // the lanes' swaps, before they read the row again
+    thr = raise_thr(thr, ab);
+  }
+}
+
 // cuda_partition_v3 at 0.25 over the active prefix (graph_cuda_comb.hpp:301)
 // and the class counts over that prefix (graph_cuda_grad.hpp:18-39):
 // A = #{d < -eps} (x = 1), T = #{-eps <= d <= eps} (tight).

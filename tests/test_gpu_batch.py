@@ -124,6 +124,14 @@ def test_batch_schedules_identical_and_graph_replay(ctx, batch_case, monkeypatch
         same_stats(sts, ref[0])
         assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
     monkeypatch.delenv("F2M_BATCH_REGS")
+    # partitions with a thread per node against four lanes per node (the default)
+    monkeypatch.setenv("F2M_BATCH_PART_LANES", "0")
+    say("batch partitions with a thread per node")
+    sts = ctx.solve_batched(b)
+    res = ctx.batch_result(b)
+    same_stats(sts, ref[0])
+    assert np.array_equal(res["F"], ref[1]["F"]) and np.array_equal(res["x"], ref[1]["x"])
+    monkeypatch.delenv("F2M_BATCH_PART_LANES")
     # captured graph: first call captures, later calls replay; eager work between
     outs = []
     for rep in range(3):
